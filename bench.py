@@ -1,0 +1,150 @@
+"""Throughput of the fused per-level advance on one MI355X.
+
+Workload: the 4096x4096 FP64 plain-Roe dam break (flat bed, no friction,
+CFL 0.45, transmissive sides).  One step = the fused advance kernel, which
+also reduces the CFL estimate of the next step on the device (no host sync).
+Inputs are resident in HBM before the timed region starts.
+
+Prints one JSON line: cell-updates/s, the HBM-roofline fraction of the
+advance kernel (algorithmic 64 B per cell-update over its average duration,
+HIP events on the kernel's stream) and the oracle's CPU throughput on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096, help="grid side of the dam break")
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--solver", choices=["Roe", "HydrostaticRoe"], default="Roe")
+    ap.add_argument("--rows", type=int, default=0, help="strip rows per wavefront (0 = default)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extra", action="store_true", help="also time the other workloads")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, budget_s):
+    """Oracle (FP64 C/OpenMP restatement) on the GPU host: whole steps of the
+    same workload until the budget is spent (at least 2)."""
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    solver = O.ROE if w.solver == "Roe" else O.HYDROSTATIC_ROE
+    args = (w.z, w.n, w.dx, w.dy, w.cfl)
+    t0 = time.perf_counter()
+    O.run(solver, w.h, w.hu, w.hv, *args, 2, w.bc_lo, w.bc_hi)      # warm-up and calibration
+    per_step = (time.perf_counter() - t0) / 2
+    steps = int(min(1000, max(2, budget_s / max(per_step, 1e-9))))
+    t0 = time.perf_counter()
+    O.run(solver, w.h, w.hu, w.hv, *args, steps, w.bc_lo, w.bc_hi)
+    el = time.perf_counter() - t0
+    return {"value": w.cells * steps / el / 1e6, "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} steps of the {w.name} workload ({w.cells} cells) with the oracle, "
+                      f"{el:.1f} s, OMP threads {threads}"}
+
+
+def time_gpu(w, dtype, steps, warmup, rows):
+    import torch
+    from hydroexa_amd import Level, SolverVariant, set_strip_rows
+
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    if rows:
+        set_strip_rows(w.solver, tdt, rows)
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi, dtype=tdt)
+    sv = SolverVariant(w.solver)
+    sv.run(lv, w.cfl, warmup)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    dth, vh = sv.run(lv, w.cfl, steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    del lv
+    return ev_ms, wall, dth
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+                                   "note only (halo exchange + 8-byte dt min all-reduce)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    from hydroexa_amd import workloads as W
+
+    if args.solver == "Roe":
+        w = W.dam_break(args.n, args.steps)
+    else:
+        w = W.lake_at_rest(args.n, args.steps)
+    es = 8 if args.dtype == "f64" else 4
+    ev_ms, wall, dth = time_gpu(w, args.dtype, args.steps, args.warmup, args.rows)
+    ms_per_step = ev_ms / args.steps
+    cups = w.cells * args.steps / (ev_ms / 1e3)
+    bytes_per_launch = 8 * es * w.cells          # read h,hu,hv,z,n; write h,hu,hv
+    achieved = bytes_per_launch / (ms_per_step / 1e3) / 1e9
+    out = {
+        "metric": "Mcell-updates/s/GPU + %HBM roofline",
+        "value": cups / 1e6,
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": w.name if args.dtype == "f64" else w.name.replace("fp64", "fp32"),
+                   "cells": w.cells, "solver": w.solver, "cfl": w.cfl,
+                   "step": "fused advance with device CFL reduction", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": ms_per_step},
+        "wall_s": wall,
+        "final_dt": float(dth[-1]) if len(dth) else None,
+    }
+    if args.extra:
+        extra = {}
+        for name, wk, dt_ in (("lake_at_rest_1024_f64", W.lake_at_rest(1024, 500), "f64"),
+                              ("lake_at_rest_1024_f32", W.lake_at_rest(1024, 500), "f32"),
+                              ("dam_break_4096_f32", W.dam_break(4096, 500), "f32")):
+            ms, _, _ = time_gpu(wk, dt_, 500, 20, 0)
+            e = 8 if dt_ == "f64" else 4
+            extra[name] = {"Mcups": wk.cells * 500 / (ms / 1e3) / 1e6, "ms_per_step": ms / 500,
+                           "hbm_frac": 8 * e * wk.cells / (ms / 500 / 1e3) / 1e9 / HBM_PEAK_GBS}
+        out["extra"] = extra
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(W.dam_break(args.n, 1) if args.solver == "Roe" else w,
+                                           args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

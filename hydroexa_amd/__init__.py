@@ -1,0 +1,20 @@
+"""hydroexa_amd: MI355X-native per-level shallow-water advance (HydroEXA's hot path).
+
+The arithmetic lives in hand-written HIP kernels for gfx950
+(``csrc/``, built into ``lib/libhydroexa.so``) behind the C-ABI of
+``include/hydroexa.h``; this package is the host-side mirror of the
+reference's solver interface.
+"""
+from .solver import (  # noqa: F401
+    Level,
+    SolverVariant,
+    context,
+    masked_average_down,
+    post_process_base_grids,
+    set_strip_rows,
+)
+from . import _lib  # noqa: F401
+from . import workloads  # noqa: F401,E402
+
+__all__ = ["Level", "SolverVariant", "masked_average_down", "post_process_base_grids",
+           "set_strip_rows", "workloads"]

@@ -1,0 +1,63 @@
+"""Build the gfx950 shared library ``hydroexa_amd/lib/libhydroexa.so`` with hipcc.
+
+The library is built in-tree so the same file is loaded by tests, smoke and
+the benchmark.  Parity FMA policy: ``-ffp-contract=off`` (the FP64 kernels
+then round exactly like the CPU restatement).  Packed FP32 instructions are
+disabled for all device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libhydroexa.so")
+SOURCES = [os.path.join(CSRC, "hx_kernels.hip")]
+HEADERS = [os.path.join(CSRC, "hx_device.hpp"), os.path.join(ROOT, "include", "hydroexa.h")]
+ARCH = os.environ.get("HYDROEXA_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP library cannot be built")
+
+
+def flags() -> list:
+    return [
+        f"--offload-arch={ARCH}",
+        "-O3",
+        "-std=c++17",
+        "-fPIC",
+        "-shared",
+        "-ffp-contract=off",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wall",
+        "-Wno-unused-command-line-argument",
+        f"-I{os.path.join(ROOT, 'include')}",
+    ]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source into the in-tree shared library (always)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc()] + flags() + SOURCES + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building libhydroexa.so")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

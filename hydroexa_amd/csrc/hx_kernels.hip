@@ -1,0 +1,877 @@
+// hx_kernels.hip -- fused per-level advance and auxiliary kernels for
+// CDNA4 (gfx950), plus the C-ABI declared in include/hydroexa.h.
+//
+// Work decomposition of the fused advance ("strip march"): one wavefront
+// owns a strip of columns and marches up a run of rows.  Each lane holds one
+// column.  A lane computes the y face above its cell in registers and keeps
+// it as the next row's lower face, so every y face is computed once; it
+// computes the x face on its left from its left neighbour's hoisted cell
+// quantities (a cross-lane shuffle) and takes its right face from the lane
+// on its right, so every x face is computed once as well.  Edge lanes only
+// feed neighbours: plain Roe updates 62 of 64 columns, the hydrostatic
+// solver (two-cell stencil) 60.  No LDS tile, no barrier in the march; the
+// rows are read once, coalesced, straight from HBM.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "../../include/hydroexa.h"
+#include "hx_device.hpp"
+
+namespace hx {
+
+constexpr int kWaves = 4;              // wavefronts per workgroup
+constexpr int kBlock = 64 * kWaves;
+constexpr int kRoeCols = 62;           // columns updated per plain-Roe strip
+constexpr int kHydroCols = 60;         // columns updated per hydrostatic strip
+
+template <typename R>
+struct StepArgs {
+    PatchView<R> p;
+    R* ho;
+    R* huo;
+    R* hvo;
+    int64_t opitch;
+    R dx, dy;
+    int rows;          // rows marched per wavefront
+    int nsx, nstrips;  // strips across x, total strips
+    double dt_host;    // used when rd == nullptr
+    double cfl;
+    CflSlot* rd;       // slot holding the CFL reduction of this step's input
+    CflSlot* acc;      // slot receiving the CFL reduction of the output
+    CflSlot* clr;      // slot cleared for the step after next
+    double* dt_hist;
+    double* vel_hist;
+    int* err;
+    int step;
+};
+
+// dt of this step: the host value, or cfl * (reduced estimate) when the
+// previous step reduced it on the device.  Returns false on dt collapse
+// (estimate < 1e-12: src/solvers/Roe.cpp:88-92), which the caller reports.
+template <typename R>
+__device__ __forceinline__ bool step_dt(const StepArgs<R>& a, R& dt)
+{
+    if (a.rd == nullptr) {
+        dt = R(a.dt_host);
+        return true;
+    }
+    if (*(volatile int*)a.err) return false;
+    const unsigned long long ab = a.rd->amax;
+    const unsigned fl = a.rd->flags;
+    const R est = slot_dt_est<R>(ab, fl);
+    if (est < R(1.e-12)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicCAS(a.err, 0, a.step + 1);
+        return false;
+    }
+    dt = est * R(a.cfl);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (a.dt_hist) a.dt_hist[a.step] = (double)dt;
+        if (a.vel_hist) a.vel_hist[a.step] = __longlong_as_double((long long)a.rd->vmax);
+        if (a.clr) { a.clr->amax = 0ull; a.clr->vmax = 0ull; a.clr->flags = 0u; }
+    }
+    return true;
+}
+
+template <typename R>
+__device__ __forceinline__ RoeCell<R> shfl_up_cell(const RoeCell<R>& c)
+{
+    RoeCell<R> o;
+    o.hd = __shfl_up(c.hd, 1, 64);
+    o.sq = __shfl_up(c.sq, 1, 64);
+    o.u = __shfl_up(c.u, 1, 64);
+    o.v = __shfl_up(c.v, 1, 64);
+    o.cg = __shfl_up(c.cg, 1, 64);
+    o.ed = __shfl_up(c.ed, 1, 64);
+    o.z = __shfl_up(c.z, 1, 64);
+    o.n = __shfl_up(c.n, 1, 64);
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// Plain Roe strip: src/solvers/Roe.cpp:327-432 fused (rebuild, x/y faces,
+// fluctuation-difference update), plus the CFL partial of the new state.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void roe_strip(const StepArgs<R>& a, int strip, R dt, CflAcc<R>& acc,
+                                          bool do_cfl)
+{
+    const PatchView<R>& p = a.p;
+    const int lane = threadIdx.x & 63;
+    const int sxi = strip % a.nsx, syi = strip / a.nsx;
+    const int c = sxi * kRoeCols - 1 + lane;
+    const int cl = min(c, p.nx);
+    const int y0 = syi * a.rows;
+    const int y1 = min(y0 + a.rows, p.ny);
+    const bool owner = (lane >= 1) && (lane <= kRoeCols) && (c < p.nx);
+    const R dtdx = dt / a.dx, dtdy = dt / a.dy;
+
+    Raw<R> rw = load_adv(p, cl, y0 - 1);
+    const RoeCell<R> cb = roe_cell(rw.hr, rw.qx, rw.qy, rw.z, rw.n);
+    Raw<R> rc = load_adv(p, cl, y0);
+    RoeCell<R> cc = roe_cell(rc.hr, rc.qx, rc.qy, rc.z, rc.n);
+    RoeFace<R> fb = roe_face<R, 1>(cb, cc, a.dy);
+
+    Raw<R> rn = load_adv(p, cl, y0 + 1);
+    for (int j = y0; j < y1; ++j) {
+        // software prefetch of the row after next
+        const Raw<R> rnn = (j + 2 <= y1) ? load_adv(p, cl, j + 2) : rn;
+        const RoeCell<R> cn = roe_cell(rn.hr, rn.qx, rn.qy, rn.z, rn.n);
+        const RoeFace<R> ft = roe_face<R, 1>(cc, cn, a.dy);
+        const RoeCell<R> cw = shfl_up_cell(cc);
+        const RoeFace<R> fx = roe_face<R, 0>(cw, cc, a.dx);
+        const R rF0 = __shfl_down(fx.F0, 1, 64);
+        const R rF1 = __shfl_down(fx.F1, 1, 64);
+        const R rF2 = __shfl_down(fx.F2, 1, 64);
+        const R rS0 = __shfl_down(fx.Sp0, 1, 64);
+        const R rS1 = __shfl_down(fx.Sp1, 1, 64);
+        const R rS2 = __shfl_down(fx.Sp2, 1, 64);
+        const R u0 = rc.hr - dtdx * (rF0 - fx.F0) - dtdy * (ft.F0 - fb.F0)
+                   - dtdx * (rS0 + fx.Sm0) - dtdy * (ft.Sp0 + fb.Sm0);
+        const R u1 = rc.qx - dtdx * (rF1 - fx.F1) - dtdy * (ft.F1 - fb.F1)
+                   - dtdx * (rS1 + fx.Sm1) - dtdy * (ft.Sp1 + fb.Sm1);
+        const R u2 = rc.qy - dtdx * (rF2 - fx.F2) - dtdy * (ft.F2 - fb.F2)
+                   - dtdx * (rS2 + fx.Sm2) - dtdy * (ft.Sp2 + fb.Sm2);
+        if (owner) {
+            const int64_t o = (int64_t)j * a.opitch + c;
+            a.ho[o] = u0;
+            a.huo[o] = u1;
+            a.hvo[o] = u2;
+            if (do_cfl) acc.add(u0, u1, u2, rc.z, a.dx, a.dy);
+        }
+        fb = ft;
+        cc = cn;
+        rc = rn;
+        rn = rnn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hydrostatic strip: src/solvers/HydrostaticRoe.cpp:487-604 fused (rebuild,
+// MUSCL van Leer face states, Audusse depths, homogeneous Roe, pressure
+// correction, update, semi-implicit Manning, dry clamp).
+// ---------------------------------------------------------------------------
+template <typename R>
+struct HRow {     // one cell of the marching window
+    R eta, u, v, z;      // reconstruction inputs
+    R h, qx, qy, n;      // stored state (for the update) and roughness
+};
+
+template <typename R>
+__device__ __forceinline__ HRow<R> hrow_load(const PatchView<R>& p, int i, int j)
+{
+    const Raw<R> r = load_adv(p, i, j);
+    HRow<R> o;
+    hydro_prims(r.hr, r.qx, r.qy, r.z, o.eta, o.u, o.v);
+    o.z = r.z;
+    o.h = r.h;
+    o.qx = r.qx;
+    o.qy = r.qy;
+    o.n = r.n;
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ void hydro_strip(const StepArgs<R>& a, int strip, R dt, CflAcc<R>& acc,
+                                            bool do_cfl)
+{
+    using k = Kc<R>;
+    const PatchView<R>& p = a.p;
+    const int lane = threadIdx.x & 63;
+    const int sxi = strip % a.nsx, syi = strip / a.nsx;
+    const int c = sxi * kHydroCols - 2 + lane;
+    const int cl = min(c, p.nx + 1);
+    const int y0 = syi * a.rows;
+    const int y1 = min(y0 + a.rows, p.ny);
+    const bool owner = (lane >= 2) && (lane < 2 + kHydroCols) && (c < p.nx);
+    const R dtdx = dt / a.dx, dtdy = dt / a.dy;
+
+    // window: rows j-1 (m), j (c), j+1 (n); y slopes of j and j+1
+    HRow<R> rm = hrow_load(p, cl, y0 - 2);
+    HRow<R> r0 = hrow_load(p, cl, y0 - 1);
+    HRow<R> r1 = hrow_load(p, cl, y0);
+    HRow<R> r2 = hrow_load(p, cl, y0 + 1);
+    // y slopes of rows y0-1 and y0, then the lower face of row y0
+    HydroCell<R> hb, hc;
+    hb.eta = r0.eta; hb.u = r0.u; hb.v = r0.v; hb.z = r0.z;
+    hb.se = vl_slope(rm.eta, r0.eta, r1.eta);
+    hb.su = vl_slope(rm.u, r0.u, r1.u);
+    hb.sv = vl_slope(rm.v, r0.v, r1.v);
+    hc.eta = r1.eta; hc.u = r1.u; hc.v = r1.v; hc.z = r1.z;
+    hc.se = vl_slope(r0.eta, r1.eta, r2.eta);
+    hc.su = vl_slope(r0.u, r1.u, r2.u);
+    hc.sv = vl_slope(r0.v, r1.v, r2.v);
+    R bF0, bF1, bF2;   // FR of the lower face of the current row
+    {
+        R f0, f1, f2, pl, pr;
+        hydro_face<R, 1>(hb, hc, f0, f1, f2, pl, pr);
+        bF0 = f0; bF1 = f1; bF2 = f2 + pr;
+    }
+    // rows: cur = r1 (slopes in hc), nxt = r2
+    HRow<R> cur = r1, nxt = r2;
+    for (int j = y0; j < y1; ++j) {
+        const HRow<R> nn = hrow_load(p, cl, j + 2);
+        HydroCell<R> hn;
+        hn.eta = nxt.eta; hn.u = nxt.u; hn.v = nxt.v; hn.z = nxt.z;
+        hn.se = vl_slope(cur.eta, nxt.eta, nn.eta);
+        hn.su = vl_slope(cur.u, nxt.u, nn.u);
+        hn.sv = vl_slope(cur.v, nxt.v, nn.v);
+        R tF0, tF1, tF2, tpl, tpr;   // upper face of the current row
+        hydro_face<R, 1>(hc, hn, tF0, tF1, tF2, tpl, tpr);
+
+        // x slopes of the current row from the neighbouring lanes
+        HydroCell<R> hx;
+        hx.eta = cur.eta; hx.u = cur.u; hx.v = cur.v; hx.z = cur.z;
+        {
+            const R em = __shfl_up(cur.eta, 1, 64), ep = __shfl_down(cur.eta, 1, 64);
+            const R um = __shfl_up(cur.u, 1, 64), up = __shfl_down(cur.u, 1, 64);
+            const R vm = __shfl_up(cur.v, 1, 64), vp = __shfl_down(cur.v, 1, 64);
+            hx.se = vl_slope(em, cur.eta, ep);
+            hx.su = vl_slope(um, cur.u, up);
+            hx.sv = vl_slope(vm, cur.v, vp);
+        }
+        HydroCell<R> hw;
+        hw.eta = __shfl_up(hx.eta, 1, 64);
+        hw.u = __shfl_up(hx.u, 1, 64);
+        hw.v = __shfl_up(hx.v, 1, 64);
+        hw.z = __shfl_up(hx.z, 1, 64);
+        hw.se = __shfl_up(hx.se, 1, 64);
+        hw.su = __shfl_up(hx.su, 1, 64);
+        hw.sv = __shfl_up(hx.sv, 1, 64);
+        R xF0, xF1, xF2, xpl, xpr;   // left x face of this lane's cell
+        hydro_face<R, 0>(hw, hx, xF0, xF1, xF2, xpl, xpr);
+        // FL of the right face comes from the lane on the right
+        const R rL0 = __shfl_down(xF0, 1, 64);
+        const R rL1 = __shfl_down(xF1 + xpl, 1, 64);
+        const R rL2 = __shfl_down(xF2, 1, 64);
+        // FR of this lane's left face
+        const R lR0 = xF0, lR1 = xF1 + xpr, lR2 = xF2;
+        // FL of the upper face, FR of the lower face
+        const R tL0 = tF0, tL1 = tF1, tL2 = tF2 + tpl;
+
+        R u0 = cur.h - dtdx * (rL0 - lR0) - dtdy * (tL0 - bF0);
+        R u1 = cur.qx - dtdx * (rL1 - lR1) - dtdy * (tL1 - bF1);
+        R u2 = cur.qy - dtdx * (rL2 - lR2) - dtdy * (tL2 - bF2);
+        if (u0 >= k::toldry) {
+            const R nsq = cur.n * cur.n;
+            const R inv_h = R(1) / u0;
+            const R uu = u1 * inv_h, vv = u2 * inv_h;
+            const R modM = sqrt(uu * uu + vv * vv);
+            if (nsq > k::tol12 && modM > k::tol12) {
+                const R h43 = u0 * cbrt(u0);
+                const R tt = dt * k::g * nsq * modM / h43;
+                const R ff = sqrt(R(1) + R(4) * tt);
+                const R fac = R(2) / (R(1) + ff);
+                u1 = u1 * fac;
+                u2 = u2 * fac;
+            }
+        } else {
+            u0 = smax(u0, R(0));
+            u1 = R(0);
+            u2 = R(0);
+        }
+        if (owner) {
+            const int64_t o = (int64_t)j * a.opitch + c;
+            a.ho[o] = u0;
+            a.huo[o] = u1;
+            a.hvo[o] = u2;
+            if (do_cfl) acc.add(u0, u1, u2, cur.z, a.dx, a.dy);
+        }
+        bF0 = tF0; bF1 = tF1; bF2 = tF2 + tpr;
+        hc = hn;
+        cur = nxt;
+        nxt = nn;
+    }
+}
+
+template <typename R, int SOLVER>
+__global__ __launch_bounds__(kBlock) void advance_kernel(StepArgs<R> a)
+{
+    R dt;
+    const bool ok = step_dt(a, dt);
+    const bool do_cfl = (a.acc != nullptr);
+    CflAcc<R> acc;
+    acc.init();
+    const int strip = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (ok && strip < a.nstrips) {
+        if (SOLVER == HX_SOLVER_ROE) roe_strip<R>(a, strip, dt, acc, do_cfl);
+        else hydro_strip<R>(a, strip, dt, acc, do_cfl);
+    }
+    if (do_cfl) cfl_commit(acc, a.acc);
+}
+
+// CFL reduction of a stored state (compute_dt_Impl's ReduceOps pass).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy, CflSlot* slot)
+{
+    CflAcc<R> acc;
+    acc.init();
+    const int64_t ncell = (int64_t)p.nx * p.ny;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ncell;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q / p.nx), i = (int)(q - (int64_t)j * p.nx);
+        const int64_t s = (int64_t)j * p.pitch + i;
+        acc.add(p.h[s], p.hu[s], p.hv[s], p.z[s], dx, dy);
+    }
+    cfl_commit(acc, slot);
+}
+
+__global__ void clear_slots_kernel(CflSlot* s, int n, int* err)
+{
+    const int t = threadIdx.x;
+    if (t < n) { s[t].amax = 0ull; s[t].vmax = 0ull; s[t].flags = 0u; s[t].pad = 0u; }
+    if (t == 0 && err) *err = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Tagging (src/solvers/Roe.cpp:121-179, HydrostaticRoe.cpp:109-169).  The
+// halo is the state's own BC image: h with the scalar rule, momenta with the
+// vector rules, terrain foextrap.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct TagCell { R h, qx, qy, z; };
+
+template <typename R>
+__device__ __forceinline__ TagCell<R> load_tag(const PatchView<R>& p, int i, int j)
+{
+    int si, ei, sj, ej;
+    bool fx, fy;
+    map_axis(i, p.nx, p.bl0, p.bh0, si, ei, fx);
+    map_axis(j, p.ny, p.bl1, p.bh1, sj, ej, fy);
+    const int64_t s = (int64_t)sj * p.pitch + si;
+    const int64_t t = (int64_t)ej * p.pitch + ei;
+    TagCell<R> c;
+    c.h = p.h[s];
+    const R qx = p.hu[s], qy = p.hv[s];
+    c.qx = fx ? -qx : qx;
+    c.qy = fy ? -qy : qy;
+    c.z = p.z[t];
+    return c;
+}
+
+template <typename R>
+__device__ __forceinline__ R umag(const TagCell<R>& c)
+{
+    if (c.h < R(1.0e-6)) return R(0);
+    const R u = c.qx / c.h, v = c.qy / c.h;
+    return sqrt(u * u + v * v);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void tag_kernel(PatchView<R> p, int solver, R dx, R dy,
+                                                  double time, R hth, R uth, R zth,
+                                                  uint8_t* tags)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= p.nx || j >= p.ny) return;
+    const TagCell<R> E = load_tag(p, i + 1, j), W = load_tag(p, i - 1, j);
+    const TagCell<R> N = load_tag(p, i, j + 1), S = load_tag(p, i, j - 1);
+    bool skip;
+    if (solver == HX_SOLVER_ROE)
+        skip = (E.z == R(-9999.0) || W.z == R(-9999.0) || N.z == R(-9999.0) || S.z == R(-9999.0));
+    else
+        skip = (fabs(E.z + R(9999.0)) < Kc<R>::tol10 || fabs(W.z + R(9999.0)) < Kc<R>::tol10 ||
+                fabs(N.z + R(9999.0)) < Kc<R>::tol10 || fabs(S.z + R(9999.0)) < Kc<R>::tol10);
+    uint8_t t = 0;
+    if (!skip) {
+        const R dhdx = (E.h - W.h) / (R(2) * dx);
+        const R dhdy = (N.h - S.h) / (R(2) * dy);
+        bool tg = sqrt(dhdx * dhdx + dhdy * dhdy) > hth;
+        const bool early = (time <= 1.0e-8);
+        if (solver == HX_SOLVER_ROE && early) {
+            const R dzdx = (E.z - W.z) / (R(2) * dx), dzdy = (N.z - S.z) / (R(2) * dy);
+            if (sqrt(dzdx * dzdx + dzdy * dzdy) > zth) tg = true;
+        }
+        if (!tg) {
+            const R dudx = (umag(E) - umag(W)) / (R(2) * dx);
+            const R dudy = (umag(N) - umag(S)) / (R(2) * dy);
+            if (sqrt(dudx * dudx + dudy * dudy) > uth) tg = true;
+            if (solver != HX_SOLVER_ROE && early) {
+                const R dzdx = (E.z - W.z) / (R(2) * dx), dzdy = (N.z - S.z) / (R(2) * dy);
+                if (sqrt(dzdx * dzdx + dzdy * dzdy) > zth) tg = true;
+            }
+        }
+        t = tg ? 1 : 0;
+    }
+    tags[(int64_t)j * p.nx + i] = t;
+}
+
+// ---------------------------------------------------------------------------
+// masked_average_down (include/utils/MaskedAverageDown.H:63-104): one thread
+// per coarse cell and component, children visited jref-major as in the
+// reference so the sums round identically.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void avgdown_kernel(const R* fine, int nxf, int nyf, int ncomp,
+                                                      int rx, int ry, R fv, R nodata, R* crse,
+                                                      int32_t* nvalid)
+{
+    const int nxc = nxf / rx, nyc = nyf / ry;
+    const int64_t ncell = (int64_t)nxc * nyc;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncell * ncomp) return;
+    const int comp = (int)(q / ncell);
+    const int64_t cc = q - (int64_t)comp * ncell;
+    const int j = (int)(cc / nxc), i = (int)(cc - (int64_t)j * nxc);
+    const R* f = fine + (int64_t)comp * nxf * nyf;
+    R cd = R(0), cv = R(0);
+    int nv = 0;
+    for (int jr = 0; jr < ry; ++jr)
+        for (int ir = 0; ir < rx; ++ir) {
+            const R val = f[(int64_t)(j * ry + jr) * nxf + (i * rx + ir)];
+            if (!isnan(val) && fabs(val - nodata) > R(1e-3)) {
+                ++nv;
+                cv += fv;
+                cd += val * fv;
+            }
+        }
+    crse[q] = (nv * 2 >= rx * ry && cv > R(0)) ? cd / cv : nodata;
+    if (nvalid) nvalid[q] = nv;
+}
+
+// ---------------------------------------------------------------------------
+// Pruning predicate (src/state/AmrMeshState_AMR.cpp:342-364): one workgroup
+// per box scans its refined footprint; flags[2b] = has valid, [2b+1] = has
+// NODATA (z <= -9998).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void prune_kernel(const int32_t* boxes, const R* z, int nxs,
+                                                    int nys, int sc, uint32_t* flags)
+{
+    const int b = blockIdx.x;
+    int ilo = boxes[4 * b] * sc, jlo = boxes[4 * b + 1] * sc;
+    int ihi = (boxes[4 * b + 2] + 1) * sc - 1, jhi = (boxes[4 * b + 3] + 1) * sc - 1;
+    ilo = max(ilo, 0); jlo = max(jlo, 0);
+    ihi = min(ihi, nxs - 1); jhi = min(jhi, nys - 1);
+    const int w = ihi - ilo + 1, hgt = jhi - jlo + 1;
+    uint32_t hv = 0, hn = 0;
+    if (w > 0 && hgt > 0) {
+        const int64_t n = (int64_t)w * hgt;
+        for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+            const int jj = jlo + (int)(q / w), ii = ilo + (int)(q % w);
+            const bool nd = z[(int64_t)jj * nxs + ii] <= R(-9998.0);
+            if (nd) hn = 1; else hv = 1;
+        }
+    }
+    hv = __any(hv) ? 1u : 0u;
+    hn = __any(hn) ? 1u : 0u;
+    if ((threadIdx.x & 63) == 0) {
+        if (hv) atomicOr(&flags[2 * b], 1u);
+        if (hn) atomicOr(&flags[2 * b + 1], 1u);
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = 0u;
+}
+
+}  // namespace hx
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+using namespace hx;
+
+struct hx_ctx {
+    int device;
+    CflSlot* slots;      // 4 slots: 3 for the fused loop, 1 for hx_level_dt
+    int* err;
+    double* hist;        // 2 * hist_cap doubles (dt, vel)
+    int hist_cap;
+    uint32_t* pflags;    // prune flags
+    int32_t* pboxes;
+    int pcap;
+    int rows[2][2];      // [solver][dtype]
+};
+
+static int hip_status(hipError_t e)
+{
+    return (e == hipSuccess) ? HX_OK : HX_E_HIP;
+}
+
+#define HX_TRY(x)                                  \
+    do {                                           \
+        hipError_t _e = (x);                       \
+        if (_e != hipSuccess) return HX_E_HIP;     \
+    } while (0)
+
+static int default_rows(int solver, int dtype)
+{
+    (void)solver;
+    (void)dtype;
+    return 64;
+}
+
+extern "C" int hx_abi_version(void) { return HX_ABI_VERSION; }
+
+extern "C" const char* hx_status_string(int s)
+{
+    switch (s) {
+        case HX_OK: return "ok";
+        case HX_E_BADARG: return "invalid argument";
+        case HX_E_DT_COLLAPSE: return "dt collapsed below 1e-12";
+        case HX_E_HIP: return "HIP runtime error";
+        case HX_E_NOMEM: return "device allocation failed";
+        case HX_E_STATE: return "context used out of order";
+        default: return "unknown status";
+    }
+}
+
+extern "C" int hx_ctx_create(hx_ctx** out, int device)
+{
+    if (!out) return HX_E_BADARG;
+    *out = nullptr;
+    int ndev = 0;
+    HX_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(device));
+    hx_ctx* c = new (std::nothrow) hx_ctx();
+    if (!c) return HX_E_NOMEM;
+    c->device = device;
+    if (hipMalloc(&c->slots, 4 * sizeof(CflSlot)) != hipSuccess ||
+        hipMalloc(&c->err, sizeof(int)) != hipSuccess) {
+        delete c;
+        return HX_E_NOMEM;
+    }
+    for (int s = 0; s < 2; ++s)
+        for (int d = 0; d < 2; ++d) c->rows[s][d] = default_rows(s, d);
+    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, 0, c->slots, 4, c->err);
+    HX_TRY(hipDeviceSynchronize());
+    *out = c;
+    return HX_OK;
+}
+
+extern "C" int hx_ctx_destroy(hx_ctx* c)
+{
+    if (!c) return HX_OK;
+    hipSetDevice(c->device);
+    hipFree(c->slots);
+    hipFree(c->err);
+    if (c->hist) hipFree(c->hist);
+    if (c->pflags) hipFree(c->pflags);
+    if (c->pboxes) hipFree(c->pboxes);
+    delete c;
+    return HX_OK;
+}
+
+extern "C" int hx_set_strip_rows(hx_ctx* c, int solver, int dtype, int rows)
+{
+    if (!c || solver < 0 || solver > 1 || dtype < 0 || dtype > 1 || rows < 0 || rows > 1 << 20)
+        return HX_E_BADARG;
+    c->rows[solver][dtype] = rows ? rows : default_rows(solver, dtype);
+    return HX_OK;
+}
+
+static int check_patch(const hx_patch* in, int need_ghost)
+{
+    if (!in || in->nx < 2 || in->ny < 2 || in->ng < 0 || in->ng > 8) return HX_E_BADARG;
+    if (in->dtype != HX_F64 && in->dtype != HX_F32) return HX_E_BADARG;
+    if (!in->h || !in->hu || !in->hv || !in->z || !in->n) return HX_E_BADARG;
+    if (!(in->dx > 0.0) || !(in->dy > 0.0)) return HX_E_BADARG;
+    for (int d = 0; d < 2; ++d) {
+        const int lo = in->bc_lo[d], hi = in->bc_hi[d];
+        if (lo < -1 || lo > 1 || hi < -1 || hi > 1) return HX_E_BADARG;
+        if ((lo == HX_BC_GHOST || hi == HX_BC_GHOST) && in->ng < need_ghost) return HX_E_BADARG;
+    }
+    const int64_t gx = (in->bc_lo[0] == HX_BC_GHOST ? in->ng : 0) + (in->bc_hi[0] == HX_BC_GHOST ? in->ng : 0);
+    if (in->pitch < (int64_t)in->nx + gx) return HX_E_BADARG;
+    return HX_OK;
+}
+
+template <typename R>
+static PatchView<R> view_of(const hx_patch* in)
+{
+    PatchView<R> v;
+    v.h = (const R*)in->h;
+    v.hu = (const R*)in->hu;
+    v.hv = (const R*)in->hv;
+    v.eta = (const R*)in->eta;
+    v.z = (const R*)in->z;
+    v.n = (const R*)in->n;
+    v.pitch = in->pitch;
+    v.nx = in->nx;
+    v.ny = in->ny;
+    v.bl0 = in->bc_lo[0];
+    v.bl1 = in->bc_lo[1];
+    v.bh0 = in->bc_hi[0];
+    v.bh1 = in->bc_hi[1];
+    return v;
+}
+
+template <typename R>
+static void fill_geometry(StepArgs<R>& a, int solver, int rows)
+{
+    const int cols = (solver == HX_SOLVER_ROE) ? kRoeCols : kHydroCols;
+    a.rows = rows;
+    a.nsx = (a.p.nx + cols - 1) / cols;
+    const int nsy = (a.p.ny + rows - 1) / rows;
+    a.nstrips = a.nsx * nsy;
+}
+
+template <typename R>
+static int launch_advance(int solver, const StepArgs<R>& a, hipStream_t st)
+{
+    const int nblk = (a.nstrips + kWaves - 1) / kWaves;
+    if (solver == HX_SOLVER_ROE)
+        hipLaunchKernelGGL((advance_kernel<R, HX_SOLVER_ROE>), dim3(nblk), dim3(kBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL((advance_kernel<R, HX_SOLVER_HYDROSTATIC_ROE>), dim3(nblk), dim3(kBlock), 0, st, a);
+    return hip_status(hipGetLastError());
+}
+
+template <typename R>
+static int advance_t(hx_ctx* c, int solver, const hx_patch* in, double dt, void* ho, void* huo,
+                     void* hvo, int64_t opitch, hipStream_t st)
+{
+    StepArgs<R> a{};
+    a.p = view_of<R>(in);
+    a.ho = (R*)ho;
+    a.huo = (R*)huo;
+    a.hvo = (R*)hvo;
+    a.opitch = opitch;
+    a.dx = (R)in->dx;
+    a.dy = (R)in->dy;
+    a.dt_host = dt;
+    fill_geometry(a, solver, c->rows[solver][in->dtype]);
+    return launch_advance(solver, a, st);
+}
+
+extern "C" int hx_advance_level(hx_ctx* c, int solver, const hx_patch* in, double dt, void* ho,
+                                void* huo, void* hvo, int64_t opitch, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    int rc = check_patch(in, solver == HX_SOLVER_ROE ? 1 : 2);
+    if (rc) return rc;
+    if (!ho || !huo || !hvo || opitch < in->nx || !(dt >= 0.0)) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    return (in->dtype == HX_F64) ? advance_t<double>(c, solver, in, dt, ho, huo, hvo, opitch, st)
+                                 : advance_t<float>(c, solver, in, dt, ho, huo, hvo, opitch, st);
+}
+
+template <typename R>
+static int launch_cfl(const hx_patch* in, CflSlot* slot, hipStream_t st)
+{
+    const int64_t ncell = (int64_t)in->nx * in->ny;
+    int64_t nblk = (ncell + kBlock - 1) / kBlock;
+    if (nblk > 4096) nblk = 4096;
+    hipLaunchKernelGGL(cfl_kernel<R>, dim3((unsigned)nblk), dim3(kBlock), 0, st, view_of<R>(in),
+                       (R)in->dx, (R)in->dy, slot);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int hx_level_dt(hx_ctx* c, int solver, const hx_patch* in, double cfl, double* dt_out,
+                           double* max_vel_out, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    int rc = check_patch(in, 0);
+    if (rc) return rc;
+    if (!dt_out || !(cfl > 0.0)) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots + 3, 1, (int*)nullptr);
+    rc = (in->dtype == HX_F64) ? launch_cfl<double>(in, c->slots + 3, st)
+                               : launch_cfl<float>(in, c->slots + 3, st);
+    if (rc) return rc;
+    CflSlot s;
+    HX_TRY(hipMemcpyAsync(&s, c->slots + 3, sizeof(s), hipMemcpyDeviceToHost, st));
+    HX_TRY(hipStreamSynchronize(st));
+    double vmax;
+    std::memcpy(&vmax, &s.vmax, sizeof(vmax));
+    double est;
+    if (in->dtype == HX_F64) {
+        est = slot_dt_est<double>(s.amax, s.flags);
+        *dt_out = est * cfl;
+    } else {
+        const float e = slot_dt_est<float>(s.amax, s.flags);
+        est = e;
+        *dt_out = (double)(e * (float)cfl);
+    }
+    if (max_vel_out) *max_vel_out = (s.flags ? vmax : -1.7976931348623157e308);
+    return (est < 1.e-12) ? HX_E_DT_COLLAPSE : HX_OK;
+}
+
+extern "C" int hx_tag_level(hx_ctx* c, int solver, const hx_patch* in, double time,
+                            const double* thresh, uint8_t* tags, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    int rc = check_patch(in, 1);
+    if (rc) return rc;
+    if (!thresh || !tags) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((in->nx + 63) / 64, (in->ny + 3) / 4);
+    if (in->dtype == HX_F64)
+        hipLaunchKernelGGL(tag_kernel<double>, grid, dim3(256), 0, st, view_of<double>(in), solver,
+                           in->dx, in->dy, time, thresh[0], thresh[1], thresh[2], tags);
+    else
+        hipLaunchKernelGGL(tag_kernel<float>, grid, dim3(256), 0, st, view_of<float>(in), solver,
+                           (float)in->dx, (float)in->dy, time, (float)thresh[0], (float)thresh[1],
+                           (float)thresh[2], tags);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int hx_masked_average_down(hx_ctx* c, int dtype, int nxf, int nyf, int ncomp,
+                                      const void* fine, int rx, int ry, double dxf, double dyf,
+                                      double nodata, void* crse, int32_t* nvalid, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if ((dtype != HX_F64 && dtype != HX_F32) || !fine || !crse || nxf < 1 || nyf < 1 ||
+        ncomp < 1 || rx < 1 || ry < 1 || nxf % rx || nyf % ry)
+        return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = (int64_t)(nxf / rx) * (nyf / ry) * ncomp;
+    const unsigned nblk = (unsigned)((n + 255) / 256);
+    if (dtype == HX_F64)
+        hipLaunchKernelGGL(avgdown_kernel<double>, dim3(nblk), dim3(256), 0, st, (const double*)fine,
+                           nxf, nyf, ncomp, rx, ry, dxf * dyf, nodata, (double*)crse, nvalid);
+    else
+        hipLaunchKernelGGL(avgdown_kernel<float>, dim3(nblk), dim3(256), 0, st, (const float*)fine,
+                           nxf, nyf, ncomp, rx, ry, (float)dxf * (float)dyf, (float)nodata,
+                           (float*)crse, nvalid);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int hx_prune_boxes(hx_ctx* c, int dtype, int nbox, const int32_t* boxes, int nxs,
+                              int nys, const void* z, int scale_log2, uint8_t* keep,
+                              uint8_t* has_nodata, int32_t* nkeep, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if ((dtype != HX_F64 && dtype != HX_F32) || nbox < 0 || (nbox > 0 && !boxes) || !z ||
+        nxs < 1 || nys < 1 || scale_log2 < 0 || scale_log2 > 20 || !keep || !has_nodata || !nkeep)
+        return HX_E_BADARG;
+    *nkeep = 0;
+    if (nbox == 0) return HX_OK;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    if (nbox > c->pcap) {
+        if (c->pflags) hipFree(c->pflags);
+        if (c->pboxes) hipFree(c->pboxes);
+        c->pflags = nullptr;
+        c->pboxes = nullptr;
+        c->pcap = 0;
+        if (hipMalloc(&c->pflags, sizeof(uint32_t) * 2 * nbox) != hipSuccess ||
+            hipMalloc(&c->pboxes, sizeof(int32_t) * 4 * nbox) != hipSuccess)
+            return HX_E_NOMEM;
+        c->pcap = nbox;
+    }
+    HX_TRY(hipMemcpyAsync(c->pboxes, boxes, sizeof(int32_t) * 4 * nbox, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbox + 255) / 256), dim3(256), 0, st, c->pflags, 2 * nbox);
+    if (dtype == HX_F64)
+        hipLaunchKernelGGL(prune_kernel<double>, dim3(nbox), dim3(256), 0, st, c->pboxes,
+                           (const double*)z, nxs, nys, 1 << scale_log2, c->pflags);
+    else
+        hipLaunchKernelGGL(prune_kernel<float>, dim3(nbox), dim3(256), 0, st, c->pboxes,
+                           (const float*)z, nxs, nys, 1 << scale_log2, c->pflags);
+    HX_TRY(hipGetLastError());
+    uint32_t* hf = (uint32_t*)std::malloc(sizeof(uint32_t) * 2 * nbox);
+    if (!hf) return HX_E_NOMEM;
+    hipError_t e = hipMemcpyAsync(hf, c->pflags, sizeof(uint32_t) * 2 * nbox, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { std::free(hf); return HX_E_HIP; }
+    int nk = 0;
+    for (int b = 0; b < nbox; ++b) {
+        keep[b] = (uint8_t)(hf[2 * b] ? 1 : 0);
+        has_nodata[b] = (uint8_t)(hf[2 * b + 1] ? 1 : 0);
+        nk += keep[b];
+    }
+    std::free(hf);
+    if (nk == 0) {   // AmrMeshState_AMR.cpp:397-401: keep the whole layout
+        for (int b = 0; b < nbox; ++b) { keep[b] = 1; has_nodata[b] = 0; }
+        nk = nbox;
+    }
+    *nkeep = nk;
+    return HX_OK;
+}
+
+template <typename R>
+static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu, void* ahv,
+                 double cfl, int nsteps, hipStream_t st)
+{
+    StepArgs<R> a{};
+    a.p = view_of<R>(in);
+    a.opitch = in->pitch;
+    a.dx = (R)in->dx;
+    a.dy = (R)in->dy;
+    a.cfl = cfl;
+    a.err = c->err;
+    a.dt_hist = c->hist;
+    a.vel_hist = c->hist + c->hist_cap;
+    fill_geometry(a, solver, c->rows[solver][in->dtype]);
+    // reset slots and the error flag, then reduce the initial state's CFL
+    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots, 3, c->err);
+    int rc = launch_cfl<R>(in, c->slots, st);
+    if (rc) return rc;
+    R* bufA[3] = {(R*)in->h, (R*)in->hu, (R*)in->hv};
+    R* bufB[3] = {(R*)ah, (R*)ahu, (R*)ahv};
+    for (int s = 0; s < nsteps; ++s) {
+        R** src = (s & 1) ? bufB : bufA;
+        R** dst = (s & 1) ? bufA : bufB;
+        a.p.h = src[0];
+        a.p.hu = src[1];
+        a.p.hv = src[2];
+        a.ho = dst[0];
+        a.huo = dst[1];
+        a.hvo = dst[2];
+        a.rd = c->slots + (s % 3);
+        a.acc = c->slots + ((s + 1) % 3);
+        a.clr = c->slots + ((s + 2) % 3);
+        a.step = s;
+        rc = launch_advance(solver, a, st);
+        if (rc) return rc;
+    }
+    return HX_OK;
+}
+
+extern "C" int hx_run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
+                            void* ahv, double cfl, int nsteps, double* dt_hist, double* vel_hist,
+                            int32_t* steps_done, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    int rc = check_patch(in, 0);
+    if (rc) return rc;
+    for (int d = 0; d < 2; ++d)
+        if (in->bc_lo[d] == HX_BC_GHOST || in->bc_hi[d] == HX_BC_GHOST) return HX_E_BADARG;
+    if (in->eta || !ah || !ahu || !ahv || nsteps < 0 || !(cfl > 0.0)) return HX_E_BADARG;
+    if (steps_done) *steps_done = 0;
+    if (nsteps == 0) return HX_OK;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    if (nsteps > c->hist_cap) {
+        if (c->hist) hipFree(c->hist);
+        c->hist = nullptr;
+        c->hist_cap = 0;
+        int cap = 1024;
+        while (cap < nsteps) cap *= 2;
+        if (hipMalloc(&c->hist, sizeof(double) * 2 * cap) != hipSuccess) return HX_E_NOMEM;
+        c->hist_cap = cap;
+    }
+    rc = (in->dtype == HX_F64) ? run_t<double>(c, solver, in, ah, ahu, ahv, cfl, nsteps, st)
+                               : run_t<float>(c, solver, in, ah, ahu, ahv, cfl, nsteps, st);
+    if (rc) return rc;
+    int err = 0;
+    HX_TRY(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (dt_hist)
+        HX_TRY(hipMemcpyAsync(dt_hist, c->hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, st));
+    if (vel_hist)
+        HX_TRY(hipMemcpyAsync(vel_hist, c->hist + c->hist_cap, sizeof(double) * nsteps,
+                              hipMemcpyDeviceToHost, st));
+    HX_TRY(hipStreamSynchronize(st));
+    if (err) {
+        if (steps_done) *steps_done = err - 1;
+        return HX_E_DT_COLLAPSE;
+    }
+    if (steps_done) *steps_done = nsteps;
+    return HX_OK;
+}

@@ -1,0 +1,172 @@
+/*
+ * hydroexa.h -- C-ABI of the MI355X-native per-level shallow-water advance.
+ *
+ * These entry points replace the per-level solver calls of HydroEXA
+ * (jkhansell/HydroEXA); each declaration names the reference interface it
+ * stands in for.  Plain pointers and sizes only: device pointers are HIP
+ * device memory, host pointers are marked as such, and a stream argument is
+ * a hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Status codes replace the reference's process-fatal errors
+ * (std::runtime_error, amrex::Error, amrex::Abort); no exception crosses
+ * this boundary.
+ */
+#ifndef HYDROEXA_H
+#define HYDROEXA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HX_ABI_VERSION 1
+
+/* Status codes. */
+#define HX_OK             0
+#define HX_E_BADARG       2  /* invalid descriptor, size or code            */
+#define HX_E_DT_COLLAPSE  3  /* dt estimate < 1e-12 (Roe.cpp:88-92,
+                                HydrostaticRoe.cpp:77-80)                   */
+#define HX_E_HIP          4  /* a HIP runtime call failed                   */
+#define HX_E_NOMEM        5  /* device allocation failed                    */
+#define HX_E_STATE        6  /* context used out of order                   */
+
+/* Solver kinds, as selected in src/state/AmrMeshState_Init.cpp:175-184. */
+#define HX_SOLVER_ROE             0
+#define HX_SOLVER_HYDROSTATIC_ROE 1
+
+/* Boundary codes per side.  0 and 1 are the input-file codes of
+ * include/boundaries/BCFill.H:49-82 (0 reflective, 1 transmissive); the
+ * library applies them while loading the halo.  HX_BC_GHOST means the
+ * caller has filled that side's ghost cells in memory (an AMReX FillPatch,
+ * a neighbouring box), and the kernels read them as stored. */
+#define HX_BC_REFLECT   0
+#define HX_BC_TRANSMIT  1
+#define HX_BC_GHOST    (-1)
+
+/* Element types. */
+#define HX_F64 0
+#define HX_F32 1
+
+/*
+ * One box of one level, structure of arrays.  Mirrors the per-FAB view the
+ * reference builds in MFIter loops (Array4 of U with comps h, hu, hv and of
+ * the terrain with comps z, n; src/solvers/Roe.cpp:327-352).
+ *
+ * Every field pointer addresses valid cell (0,0); element (i,j) is at
+ * ptr[j*pitch + i], and i,j may reach -ng .. n-1+ng on sides coded
+ * HX_BC_GHOST.  All fields share pitch and type.
+ */
+typedef struct hx_patch {
+    int32_t nx, ny;         /* valid cells                                  */
+    int32_t ng;             /* ghost layers stored on HX_BC_GHOST sides     */
+    int32_t dtype;          /* HX_F64 or HX_F32                             */
+    int64_t pitch;          /* elements between rows                        */
+    const void *h, *hu, *hv;/* state at t^n                                 */
+    const void *eta;        /* optional free surface; NULL: eta = h + z     */
+    const void *z, *n;      /* bed elevation and Manning roughness          */
+    double dx, dy;          /* cell size                                    */
+    int32_t bc_lo[2];       /* per direction: HX_BC_* at the low side       */
+    int32_t bc_hi[2];       /* per direction: HX_BC_* at the high side      */
+} hx_patch;
+
+typedef struct hx_ctx hx_ctx;
+
+/* Library context: owns the small device scratch the kernels reduce into
+ * (CFL slots, dt history).  One context per host thread; calls that share
+ * a context must be ordered on one stream.  (No reference counterpart: the
+ * reference allocates its temporaries per call, Roe.cpp:291-303.) */
+int hx_ctx_create(hx_ctx **out, int device);
+int hx_ctx_destroy(hx_ctx *ctx);
+int hx_abi_version(void);
+const char *hx_status_string(int status);
+
+/*
+ * One AdvanceLevel of one box: U_new = U_old - dt * div(fluxes).
+ * Replaces SolverVariant::compute_fluxes (include/solvers/SolverVariant.H:81-90)
+ * -> Roe::compute_fluxes_Impl (src/solvers/Roe.cpp:250-434) or
+ * HydrostaticRoe::compute_fluxes_Impl (src/solvers/HydrostaticRoe.cpp:400-606),
+ * fused into one kernel.  Output arrays (device, out_pitch) receive the new
+ * h, hu, hv of the valid cells; they must not alias the inputs.  The
+ * reference swaps U_new/U_old inside the call; here the caller swaps.
+ * Reflux face fluxes are not produced.
+ */
+int hx_advance_level(hx_ctx *ctx, int solver, const hx_patch *in, double dt,
+                     void *h_out, void *hu_out, void *hv_out, int64_t out_pitch,
+                     void *stream);
+
+/*
+ * CFL timestep of one level.  Replaces compute_dt_Impl
+ * (src/solvers/Roe.cpp:8-98, src/solvers/HydrostaticRoe.cpp:7-86):
+ * *dt_out (host) = cfl * min over cells, *max_vel_out (host) = max |u|,|v|.
+ * Synchronises the stream.  Returns HX_E_DT_COLLAPSE (and still fills the
+ * outputs) when the estimate before cfl is below 1e-12.
+ */
+int hx_level_dt(hx_ctx *ctx, int solver, const hx_patch *in, double cfl,
+                double *dt_out, double *max_vel_out, void *stream);
+
+/*
+ * Gradient tagging.  Replaces SolverVariant::tag_cells
+ * (include/solvers/SolverVariant.H:139-156) -> Roe::tag_cells_Impl
+ * (src/solvers/Roe.cpp:100-181) or HydrostaticRoe::tag_cells_Impl
+ * (src/solvers/HydrostaticRoe.cpp:88-171).  thresh (host) = {h_grad,
+ * umag_grad, z_grad} of the level (PhysicsParameters, include/utils/Params.H:103-111).
+ * tags (device, nx*ny bytes, row pitch nx) receive 1 where the reference
+ * sets TagBox::SET and 0 elsewhere.  Ghost semantics: physical sides read
+ * the halo each component's BCRec would produce; HX_BC_GHOST sides read
+ * memory.
+ */
+int hx_tag_level(hx_ctx *ctx, int solver, const hx_patch *in, double time,
+                 const double *thresh, uint8_t *tags, void *stream);
+
+/*
+ * NODATA-aware restriction.  Replaces amrex::masked_average_down
+ * (include/utils/MaskedAverageDown.H:124-127, src/utils/MaskedAverageDown.cpp:12-63).
+ * fine: ncomp planes of nyf rows x nxf (device, dense); crse: ncomp planes
+ * of (nyf/ry) x (nxf/rx); nvalid (device, optional): valid-child count per
+ * coarse cell and component.  dxf, dyf: fine cell size (the volume weight).
+ */
+int hx_masked_average_down(hx_ctx *ctx, int dtype, int nxf, int nyf, int ncomp,
+                           const void *fine, int rx, int ry, double dxf, double dyf,
+                           double nodata, void *crse, int32_t *nvalid, void *stream);
+
+/*
+ * Base-grid pruning predicate.  Replaces AmrMeshState::PostProcessBaseGrids
+ * (src/state/AmrMeshState_AMR.cpp:318-405).  boxes (host): nbox x
+ * {ilo, jlo, ihi, jhi} in level-0 cells; z (device): static bed of nys x nxs
+ * cells at 2^scale_log2 times level-0 resolution.  keep/has_nodata (host,
+ * nbox bytes) and *nkeep (host) as in the reference, including its
+ * keep-everything fallback when no box holds a valid cell.  Synchronises.
+ */
+int hx_prune_boxes(hx_ctx *ctx, int dtype, int nbox, const int32_t *boxes, int nxs,
+                   int nys, const void *z, int scale_log2, uint8_t *keep,
+                   uint8_t *has_nodata, int32_t *nkeep, void *stream);
+
+/*
+ * Step-driven single-level time loop with the CFL reduction fused into the
+ * advance: nsteps x (ComputeDt; AdvanceLevel).  Replaces the loop body of
+ * HydroEXA::Compute (src/HydroEXA.cpp:260-271) for one level with a step
+ * count instead of max_time (src/state/AmrMeshState_Physics.cpp:13-19,104-108).
+ * The dt of step k is reduced on the device by step k-1 (no host sync per
+ * step).  state: patch whose h/hu/hv hold t^0 on entry (all sides physical);
+ * alt_h/alt_hu/alt_hv: a second state buffer with the same pitch.  On
+ * return the final state is in state's buffers if nsteps is even and in the
+ * alt buffers if odd.  dt_hist / vel_hist (host, optional, nsteps each)
+ * receive dt and max|vel| per step.  Returns HX_E_DT_COLLAPSE if a step's
+ * estimate fell below 1e-12 (*steps_done, host, says how many completed).
+ */
+int hx_run_level(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
+                 void *alt_hu, void *alt_hv, double cfl, int nsteps, double *dt_hist,
+                 double *vel_hist, int32_t *steps_done, void *stream);
+
+/*
+ * Launch-shape control for the fused advance (the autotuner's knob):
+ * rows marched per wavefront.  0 restores the default.
+ */
+int hx_set_strip_rows(hx_ctx *ctx, int solver, int dtype, int rows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYDROEXA_H */

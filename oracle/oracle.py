@@ -1,0 +1,182 @@
+"""ctypes wrapper around ``libhx_oracle.so`` (the FP64 CPU restatement).
+
+Test infrastructure only.  Arrays are numpy float64, shape (ny, nx), C order,
+so element (i, j) of the reference's Array4 is ``a[j, i]``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libhx_oracle.so")
+_lib = None
+
+ROE = 0
+HYDROSTATIC_ROE = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, no FP contraction)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all", "-C", _HERE],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.hxo_roe_face.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, _dp, _dp, _dp]
+        L.hxo_hroe_face.argtypes = [C.c_double] * 6 + [C.c_double, C.c_double, _dp]
+        L.hxo_vanleer_slope.argtypes = [C.c_double] * 3
+        L.hxo_vanleer_slope.restype = C.c_double
+        L.hxo_hydro_face.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp]
+        step_args = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                     C.c_double, _ip, _ip, _dp, _dp, _dp]
+        L.hxo_roe_step.argtypes = step_args
+        L.hxo_hydro_step.argtypes = step_args
+        L.hxo_level_dt.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
+                                   C.c_double, _dp, _dp]
+        L.hxo_tag.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
+                              C.c_double, _ip, _ip, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.POINTER(C.c_uint8)]
+        L.hxo_masked_average_down.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int,
+                                              C.c_int, C.c_double, C.c_double, C.c_double,
+                                              _dp, C.POINTER(C.c_int32)]
+        L.hxo_prune_boxes.argtypes = [C.c_int, C.POINTER(C.c_int32), C.c_int, C.c_int, _dp,
+                                      C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
+        L.hxo_run.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                              C.c_double, _ip, _ip, C.c_double, C.c_int, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _bc(bc_lo, bc_hi):
+    lo = (C.c_int * 2)(*bc_lo)
+    hi = (C.c_int * 2)(*bc_hi)
+    return lo, hi
+
+
+def roe_face(L, R, dt, ds, nx, ny):
+    """Plain Roe face: returns (F, S_minus, S_plus), each length 3."""
+    Lb, Rb = _f64(L), _f64(R)
+    F, Sm, Sp = np.zeros(3), np.zeros(3), np.zeros(3)
+    lib().hxo_roe_face(_p(Lb), _p(Rb), dt, ds, nx, ny, _p(F), _p(Sm), _p(Sp))
+    return F, Sm, Sp
+
+
+def hroe_face(hi, hui, hvi, hj, huj, hvj, nx, ny):
+    F = np.zeros(3)
+    lib().hxo_hroe_face(hi, hui, hvi, hj, huj, hvj, nx, ny, _p(F))
+    return F
+
+
+def vanleer_slope(qm, q, qp):
+    return lib().hxo_vanleer_slope(qm, q, qp)
+
+
+def hydro_face(cells, nx, ny):
+    """cells: 4x4 array, rows Lm, L, R, Rp of {h, hu, hv, z}; returns (FL, FR)."""
+    c = _f64(cells).reshape(16)
+    FL, FR = np.zeros(3), np.zeros(3)
+    lib().hxo_hydro_face(_p(c), nx, ny, _p(FL), _p(FR))
+    return FL, FR
+
+
+def step(solver, h, hu, hv, z, n, dt, dx, dy, bc_lo=(1, 1), bc_hi=(1, 1)):
+    """One AdvanceLevel of the chosen solver; returns (h, hu, hv) new."""
+    h, hu, hv, z, n = (_f64(a) for a in (h, hu, hv, z, n))
+    ny, nx = h.shape
+    ho, huo, hvo = np.empty_like(h), np.empty_like(h), np.empty_like(h)
+    lo, hi = _bc(bc_lo, bc_hi)
+    fn = lib().hxo_roe_step if solver == ROE else lib().hxo_hydro_step
+    rc = fn(nx, ny, _p(h), _p(hu), _p(hv), _p(z), _p(n), dt, dx, dy, lo, hi,
+            _p(ho), _p(huo), _p(hvo))
+    if rc != 0:
+        raise ValueError(f"oracle step failed with status {rc}")
+    return ho, huo, hvo
+
+
+def level_dt(h, hu, hv, z, dx, dy):
+    """(dt_est, max_vel) before the cfl factor."""
+    h, hu, hv, z = (_f64(a) for a in (h, hu, hv, z))
+    ny, nx = h.shape
+    d, v = C.c_double(), C.c_double()
+    lib().hxo_level_dt(nx, ny, _p(h), _p(hu), _p(hv), _p(z), dx, dy, C.byref(d), C.byref(v))
+    return d.value, v.value
+
+
+def tag(solver, h, hu, hv, z, dx, dy, time, h_thresh, u_thresh, z_thresh,
+        bc_lo=(1, 1), bc_hi=(1, 1)):
+    h, hu, hv, z = (_f64(a) for a in (h, hu, hv, z))
+    ny, nx = h.shape
+    out = np.zeros((ny, nx), dtype=np.uint8)
+    lo, hi = _bc(bc_lo, bc_hi)
+    rc = lib().hxo_tag(solver, nx, ny, _p(h), _p(hu), _p(hv), _p(z), dx, dy, lo, hi, time,
+                       h_thresh, u_thresh, z_thresh,
+                       out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc != 0:
+        raise ValueError(f"oracle tag failed with status {rc}")
+    return out
+
+
+def masked_average_down(fine, rx, ry, dxf, dyf, nodata=-9999.0):
+    """fine: (ncomp, nyf, nxf); returns (crse, nvalid) of shape (ncomp, nyf/ry, nxf/rx)."""
+    f = _f64(fine)
+    if f.ndim == 2:
+        f = f[None]
+    nc, nyf, nxf = f.shape
+    crse = np.zeros((nc, nyf // ry, nxf // rx))
+    nval = np.zeros((nc, nyf // ry, nxf // rx), dtype=np.int32)
+    rc = lib().hxo_masked_average_down(nxf, nyf, nc, _p(f), rx, ry, dxf, dyf, nodata,
+                                       _p(crse), nval.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc != 0:
+        raise ValueError(f"oracle masked_average_down failed with status {rc}")
+    return crse, nval
+
+
+def prune_boxes(boxes, z_static, scale_log2):
+    """boxes: (nbox, 4) int {ilo, jlo, ihi, jhi}; returns (keep, has_nodata, nkeep)."""
+    b = np.ascontiguousarray(boxes, dtype=np.int32)
+    z = _f64(z_static)
+    nys, nxs = z.shape
+    keep = np.zeros(len(b), dtype=np.uint8)
+    nd = np.zeros(len(b), dtype=np.uint8)
+    nk = lib().hxo_prune_boxes(len(b), b.ctypes.data_as(C.POINTER(C.c_int32)), nxs, nys,
+                               _p(z), scale_log2, keep.ctypes.data_as(C.POINTER(C.c_uint8)),
+                               nd.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return keep.astype(bool), nd.astype(bool), nk
+
+
+def run(solver, h, hu, hv, z, n, dx, dy, cfl, nsteps, bc_lo=(1, 1), bc_hi=(1, 1)):
+    """Step-driven loop (ComputeDt + AdvanceLevel); returns (h, hu, hv, dt_hist, vel_hist)."""
+    h, hu, hv = (_f64(a).copy() for a in (h, hu, hv))
+    z, n = _f64(z), _f64(n)
+    ny, nx = h.shape
+    dth = np.zeros(nsteps)
+    vh = np.zeros(nsteps)
+    lo, hi = _bc(bc_lo, bc_hi)
+    rc = lib().hxo_run(solver, nx, ny, _p(h), _p(hu), _p(hv), _p(z), _p(n), dx, dy, lo, hi,
+                       cfl, nsteps, _p(dth), _p(vh))
+    if rc != 0:
+        raise RuntimeError(f"oracle run failed with status {rc}")
+    return h, hu, hv, dth, vh

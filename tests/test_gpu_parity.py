@@ -1,0 +1,274 @@
+"""GPU parity: every entry point of the C-ABI against the FP64 oracle on the
+same seeded inputs.
+
+Bars: FP64 state within 1e-12 relative L-infinity (per field, relative to
+the field's own max |value|); tag masks, valid counts and pruning results
+bit-exact; FP32 within the measured bound stated in test_fp32_bound.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from hydroexa_amd import Level, SolverVariant, masked_average_down, post_process_base_grids
+from hydroexa_amd import workloads as W
+from hydroexa_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = {"Roe": O.ROE, "HydrostaticRoe": O.HYDROSTATIC_ROE}
+
+
+def rel_linf(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+def random_state(rng, ny, nx, dry_frac=0.0, nodata=False, friction=True):
+    X, Y = np.meshgrid((np.arange(nx) + .5) / nx, (np.arange(ny) + .5) / ny)
+    z = 0.3 * np.exp(-((X - .3) ** 2 + (Y - .6) ** 2) / 0.03) + 0.02 * rng.uniform(size=(ny, nx))
+    h = np.maximum(1.0 + 0.4 * np.sin(5 * X) * np.cos(3 * Y) - z, 0.0) * rng.uniform(0.8, 1.2, (ny, nx))
+    if dry_frac:
+        h[rng.uniform(size=h.shape) < dry_frac] = 0.0
+        h[:, : nx // 5] = rng.uniform(0, 2e-6, (ny, nx // 5))          # a near-dry strip
+    hu = h * rng.normal(0, 0.5, (ny, nx))
+    hv = h * rng.normal(0, 0.5, (ny, nx))
+    hu[h == 0] = 0.0
+    hv[h == 0] = 0.0
+    n = rng.uniform(0.0, 0.05, (ny, nx)) if friction else np.zeros((ny, nx))
+    if nodata:
+        z[ny // 3: ny // 2, nx // 4: nx // 2] = -9999.0
+        n[ny // 3: ny // 2, nx // 4: nx // 2] = -9999.0
+        h[ny // 3: ny // 2, nx // 4: nx // 2] = 0.0
+        hu[z < -9000] = 0.0
+        hv[z < -9000] = 0.0
+    return h, hu, hv, z, n
+
+
+def gpu_step(model, st, dt, dx, dy, bc_lo, bc_hi, dtype=torch.float64):
+    lv = Level(*st, dx, dy, bc_lo, bc_hi, dtype=dtype)
+    SolverVariant(model).compute_fluxes(lv, dt)
+    torch.cuda.synchronize()
+    return [lv.U_new[k].double().cpu().numpy() for k in range(3)]
+
+
+CASES = [
+    # (ny, nx, dry_frac, nodata, friction, bc_lo, bc_hi)
+    (37, 130, 0.0, False, False, (1, 1), (1, 1)),
+    (64, 64, 0.0, False, True, (0, 0), (0, 0)),
+    (71, 187, 0.1, False, True, (0, 1), (1, 0)),
+    (130, 61, 0.05, True, True, (1, 0), (0, 1)),
+    (200, 250, 0.0, True, False, (0, 0), (0, 0)),
+    (3, 5, 0.0, False, True, (0, 1), (0, 1)),
+]
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_step_parity_f64(gpu, model, case):
+    ny, nx, dry, nod, fr, lo, hi = CASES[case]
+    rng = np.random.default_rng(100 + case)
+    st = random_state(rng, ny, nx, dry, nod, fr)
+    dx, dy = 1.0 / nx, 1.3 / ny
+    est, _ = O.level_dt(*st[:4], dx, dy)
+    dt = 0.45 * est
+    ref = O.step(SOLVERS[model], *st, dt, dx, dy, lo, hi)
+    out = gpu_step(model, st, dt, dx, dy, lo, hi)
+    for k in range(3):
+        err = rel_linf(out[k], ref[k])
+        assert err <= 1e-12, (model, case, k, err)
+        if not fr:   # no cbrt on the path: the FP64 kernel rounds like the oracle
+            assert np.array_equal(out[k], ref[k]), (model, case, k, err)
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_level_dt_parity(gpu, model):
+    rng = np.random.default_rng(7)
+    st = random_state(rng, 90, 77, 0.1, True, True)
+    ref = O.level_dt(*st[:4], 0.01, 0.02)
+    lv = Level(*st, 0.01, 0.02)
+    dt, vel = SolverVariant(model).compute_dt_level(lv, 0.45)
+    assert dt == 0.45 * ref[0]
+    assert vel == ref[1]
+
+
+def test_dt_collapse_is_reported(gpu):
+    """A velocity explosion (dt estimate < 1e-12) surfaces as DtCollapse
+    rather than a silent zero step."""
+    h = np.ones((8, 8))
+    hu = np.full((8, 8), 1e14)
+    zero = np.zeros((8, 8))
+    lv = Level(h, hu, zero, zero, zero, 0.1, 0.1)
+    with pytest.raises(L.DtCollapse):
+        SolverVariant("Roe").compute_dt_level(lv, 0.5)
+    with pytest.raises(L.DtCollapse):
+        SolverVariant("Roe").run(lv, 0.5, 3)
+
+
+def test_dam_break_500_steps_parity(gpu):
+    """Plain dam break, 256x256, FP64, 500 steps of (ComputeDt; AdvanceLevel)
+    with the CFL reduction fused into the advance."""
+    w = W.dam_break(256, 500)
+    ref = O.run(O.ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.cfl, w.steps, w.bc_lo, w.bc_hi)
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    dth, vh = SolverVariant("Roe").run(lv, w.cfl, w.steps)
+    out = [lv.U_new[k].cpu().numpy() for k in range(3)]
+    for k in range(3):
+        assert rel_linf(out[k], ref[k]) <= 1e-12
+    np.testing.assert_array_equal(dth, ref[3])
+    np.testing.assert_array_equal(vh, ref[4])
+    assert np.all(out[0] == out[0][0:1]) and np.all(out[2] == 0.0)   # 1-D stays 1-D
+
+
+def test_lake_at_rest_nodata_parity(gpu):
+    """Lake at rest with the NODATA corner, hydrostatic solver, 1024x1024,
+    FP64, 60 steps against the oracle."""
+    w = W.lake_at_rest(1024, 60)
+    ref = O.run(O.HYDROSTATIC_ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.cfl, w.steps,
+                w.bc_lo, w.bc_hi)
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    dth, vh = SolverVariant("HydrostaticRoe").run(lv, w.cfl, w.steps)
+    out = [lv.U_new[k].cpu().numpy() for k in range(3)]
+    for k in range(3):
+        assert rel_linf(out[k], ref[k]) <= 1e-12, k
+    np.testing.assert_array_equal(dth, ref[3])
+    nd = w.z < -9000
+    assert np.all(out[0][nd] == 0.0) and np.all(out[1][nd] == 0.0)
+
+
+def test_lake_at_rest_held_full_length(gpu):
+    """2000 steps of the 1024x1024 noisy-bump lake at rest without NODATA:
+    max|vel| stays at round-off (reference-held 3.1e-15 class)."""
+    w = W.lake_at_rest(1024, 2000, nodata_corner=0)
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    _, vh = SolverVariant("HydrostaticRoe").run(lv, w.cfl, w.steps)
+    assert vh.max() < 1e-13, vh.max()
+
+
+def test_fp32_bound(gpu):
+    """FP32 kernels against the FP64 oracle: measured bound on the 256x256
+    dam break after 200 steps (relative L-infinity per field)."""
+    w = W.dam_break(256, 200)
+    ref = O.run(O.ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.cfl, w.steps, w.bc_lo, w.bc_hi)
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi, dtype=torch.float32)
+    SolverVariant("Roe").run(lv, w.cfl, w.steps)
+    out = [lv.U_new[k].double().cpu().numpy() for k in range(3)]
+    errs = [rel_linf(out[k], ref[k]) for k in range(2)]
+    print("fp32 rel linf (h, hu):", errs)
+    assert errs[0] < 2e-5 and errs[1] < 2e-4
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_stored_ghost_mode_matches_physical_bc(gpu, model):
+    """HX_BC_GHOST sides read caller-filled halos (an AMReX FillPatch): with
+    the halo filled by the boundary rules, the result is bit-identical to the
+    inline physical-BC path."""
+    ny, nx, ng = 40, 70, 2
+    rng = np.random.default_rng(9)
+    st = random_state(rng, ny, nx, 0.0, False, False)
+    lo, hi = (0, 1), (1, 0)
+    dx, dy = 1.0 / nx, 1.0 / ny
+    dt = 0.4 * O.level_dt(*st[:4], dx, dy)[0]
+    ref = gpu_step(model, st, dt, dx, dy, lo, hi)
+
+    def pad(a, kind):
+        # eta/h: scalar rule; hu odd in x; hv odd in y; terrain foextrap
+        P = np.zeros((ny + 2 * ng, nx + 2 * ng))
+        P[ng:-ng, ng:-ng] = a
+        for g in range(1, ng + 1):
+            for side, (src_refl, src_edge, idx) in {
+                    "lo": (g - 1, 0, ng - g), "hi": (nx - g, nx - 1, ng + nx - 1 + g)}.items():
+                code = lo[0] if side == "lo" else hi[0]
+                refl = (code == 0) and kind != "t"
+                col = a[:, src_refl] * (-1 if (refl and kind == "x") else 1) if refl else a[:, src_edge]
+                P[ng:-ng, idx] = col
+        for g in range(1, ng + 1):
+            for side, (src_refl, src_edge, idx) in {
+                    "lo": (g - 1, 0, ng - g), "hi": (ny - g, ny - 1, ng + ny - 1 + g)}.items():
+                code = lo[1] if side == "lo" else hi[1]
+                refl = (code == 0) and kind != "t"
+                row = P[ng + src_refl] * (-1 if (refl and kind == "y") else 1) if refl else P[ng + src_edge]
+                P[idx] = row
+        return P
+
+    h, hu, hv, z, n = st
+    eta = pad(h + z, "s")
+    zP, nP = pad(z, "t"), pad(n, "t")
+    hP = np.zeros_like(eta)     # eta supplied, so h is only read in the valid region
+    hP[ng:-ng, ng:-ng] = h
+    fields = [torch.tensor(a, device="cuda").contiguous() for a in (hP, pad(hu, "x"), pad(hv, "y"), eta, zP, nP)]
+    es = 8
+    pitch = nx + 2 * ng
+    off = (ng * pitch + ng) * es
+    p = L.HxPatch()
+    p.nx, p.ny, p.ng, p.dtype, p.pitch = nx, ny, ng, L.HX_F64, pitch
+    p.h, p.hu, p.hv, p.eta, p.z, p.n = (t.data_ptr() + off for t in fields)
+    p.dx, p.dy = dx, dy
+    p.bc_lo[0] = p.bc_lo[1] = p.bc_hi[0] = p.bc_hi[1] = L.HX_BC_GHOST
+    out = torch.zeros((3, ny, nx), dtype=torch.float64, device="cuda")
+    import ctypes as C
+    from hydroexa_amd.solver import context, _stream, MODELS
+    plane = ny * nx * es
+    rc = L.load().hx_advance_level(context(), MODELS[model], C.byref(p), dt, out.data_ptr(),
+                                   out.data_ptr() + plane, out.data_ptr() + 2 * plane, nx, _stream())
+    L.check(rc, "hx_advance_level")
+    got = out.cpu().numpy()
+    for k in range(3):
+        np.testing.assert_array_equal(got[k], ref[k])
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_tag_parity(gpu, model):
+    rng = np.random.default_rng(11)
+    for time in (0.0, 1.0):
+        st = random_state(rng, 67, 93, 0.1, True, False)
+        dx, dy = 0.01, 0.012
+        th = (3.0, 40.0, 8.0)
+        ref = O.tag(SOLVERS[model], *st[:4], dx, dy, time, *th, (0, 1), (1, 0))
+        lv = Level(*st, dx, dy, (0, 1), (1, 0))
+        got = SolverVariant(model).tag_cells(lv, time, *th).cpu().numpy()
+        assert ref.any() and not ref.all()
+        np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_masked_average_down_parity(gpu, dtype):
+    rng = np.random.default_rng(12)
+    f = rng.uniform(-3, 3, (2, 96, 128))
+    f[rng.uniform(size=f.shape) < 0.4] = -9999.0
+    f[0, 5, 7] = np.nan
+    crse, nv = masked_average_down(torch.tensor(f, dtype=dtype, device="cuda"), (4, 2), 0.1, 0.2)
+    rc, rn = O.masked_average_down(f.astype(np.float32).astype(np.float64) if dtype == torch.float32 else f,
+                                   4, 2, 0.1, 0.2)
+    np.testing.assert_array_equal(nv.cpu().numpy(), rn)
+    if dtype == torch.float64:
+        np.testing.assert_array_equal(crse.cpu().numpy(), rc)
+    else:
+        np.testing.assert_allclose(crse.cpu().numpy(), rc, rtol=2e-6)
+
+
+def test_prune_parity(gpu):
+    """The lake-at-rest layout: 32x32 boxes over 1024x1024, NODATA corner."""
+    w = W.lake_at_rest(1024, 1)
+    boxes = W.boxes_of(1024, 1024, 32)
+    kept, pure, boundary, keep, nod = post_process_base_grids(boxes, torch.tensor(w.z, device="cuda"), 0)
+    rk, rn, nk = O.prune_boxes(boxes, w.z, 0)
+    np.testing.assert_array_equal(keep, rk)
+    np.testing.assert_array_equal(nod, rn)
+    assert len(kept) == nk == 1024 - 81
+    assert len(boundary) == 19
+
+
+def test_full_size_dam_break_parity(gpu):
+    """The 4096x4096 FP64 dam break (the throughput workload) for 3 steps
+    against the oracle; y-invariance holds bit-exactly."""
+    w = W.dam_break(4096, 3)
+    ref = O.run(O.ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.cfl, w.steps, w.bc_lo, w.bc_hi)
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    dth, _ = SolverVariant("Roe").run(lv, w.cfl, w.steps)
+    out = [lv.U_new[k].cpu().numpy() for k in range(3)]
+    for k in range(3):
+        assert rel_linf(out[k], ref[k]) <= 1e-12
+    np.testing.assert_array_equal(dth, ref[3])
+    assert np.all(out[0] == out[0][0:1])
