@@ -65,11 +65,11 @@ def cpu_baseline(w, budget_s):
 
 def time_gpu(w, dtype, steps, warmup, rows):
     import torch
-    from hydroexa_amd import Level, SolverVariant, set_strip_rows
+    from hydroexa_amd import Level, SolverVariant, set_tuning
 
     tdt = torch.float64 if dtype == "f64" else torch.float32
     if rows:
-        set_strip_rows(w.solver, tdt, rows)
+        set_tuning(w.solver, tdt, rows, 0)
     lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi, dtype=tdt)
     sv = SolverVariant(w.solver)
     sv.run(lv, w.cfl, warmup)

@@ -11,10 +11,11 @@ from .solver import (  # noqa: F401
     context,
     masked_average_down,
     post_process_base_grids,
-    set_strip_rows,
+    set_arith,
+    set_tuning,
 )
 from . import _lib  # noqa: F401
 from . import workloads  # noqa: F401,E402
 
 __all__ = ["Level", "SolverVariant", "masked_average_down", "post_process_base_grids",
-           "set_strip_rows", "workloads"]
+           "set_tuning", "set_arith", "workloads"]

@@ -29,12 +29,14 @@ HX_BC_TRANSMIT = 1
 HX_BC_GHOST = -1
 HX_F64 = 0
 HX_F32 = 1
+HX_ARITH_EXACT = 0
+HX_ARITH_FAST = 1
 
 # Every symbol include/hydroexa.h declares.
 EXPORTS = (
     "hx_ctx_create", "hx_ctx_destroy", "hx_abi_version", "hx_status_string",
     "hx_advance_level", "hx_level_dt", "hx_tag_level", "hx_masked_average_down",
-    "hx_prune_boxes", "hx_run_level", "hx_set_strip_rows",
+    "hx_prune_boxes", "hx_run_level", "hx_set_tuning", "hx_set_arith",
 )
 
 
@@ -88,7 +90,8 @@ def load() -> C.CDLL:
     L.hx_prune_boxes.argtypes = [vp, C.c_int, C.c_int, i32p, C.c_int, C.c_int, vp, C.c_int,
                                  u8p, u8p, i32p, vp]
     L.hx_run_level.argtypes = [vp, C.c_int, pp, vp, vp, vp, C.c_double, C.c_int, dp, dp, i32p, vp]
-    L.hx_set_strip_rows.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.hx_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.hx_set_arith.argtypes = [vp, C.c_int]
     _lib = L
     return L
 

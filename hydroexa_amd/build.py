@@ -17,8 +17,9 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libhydroexa.so")
-SOURCES = [os.path.join(CSRC, "hx_kernels.hip")]
-HEADERS = [os.path.join(CSRC, "hx_device.hpp"), os.path.join(ROOT, "include", "hydroexa.h")]
+# (source, FP contraction policy): the exact build must not contract.
+SOURCES = [(os.path.join(CSRC, "hx_kernels.hip"), "off"),
+           (os.path.join(CSRC, "hx_fast.hip"), "fast")]
 ARCH = os.environ.get("HYDROEXA_ARCH", "gfx950")
 
 
@@ -35,8 +36,6 @@ def flags() -> list:
         "-O3",
         "-std=c++17",
         "-fPIC",
-        "-shared",
-        "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wall",
         "-Wno-unused-command-line-argument",
@@ -45,16 +44,25 @@ def flags() -> list:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source into the in-tree shared library (always)."""
+    """Compile every HIP source and link the in-tree shared library (always)."""
     os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    for src, contract in SOURCES:
+        obj = os.path.join(LIBDIR, os.path.basename(src) + ".o")
+        cmd = [hipcc()] + flags() + [f"-ffp-contract={contract}", "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed on {os.path.basename(src)}")
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + flags() + SOURCES + ["-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libhydroexa.so")
+        raise RuntimeError("linking libhydroexa.so failed")
     os.replace(tmp, LIB)
     return LIB
 

@@ -3,24 +3,27 @@
 // plain Roe face with split sources, the hydrostatic-reconstruction face, and
 // the CFL accumulator.
 //
-// Arithmetic follows the reference operation by operation so that a build
-// with -ffp-contract=off reproduces the FP64 CPU restatement bit for bit
-// (friction's cbrt aside).  Where this file departs from the literal
-// reference expression it is only by
-//   * hoisting a per-cell value that the reference recomputes per face
-//     (same operands, same operations, same rounding),
-//   * dropping a term that the reference multiplies by an exact 0.0 or 1.0
-//     (the face-normal rotation with axis normals, the zero contact-wave
-//     source), which can change at most the sign of a zero,
-//   * evaluating the positivity-guard timing test only when its depth test
-//     passes (no side effects, so the outcome is unchanged).
-#pragma once
+// Included once per arithmetic build, inside namespace hx::HX_NS, with
+// HX_FAST set:
+//   HX_FAST 0 (exact): every division is a correctly rounded '/', compiled
+//     with -ffp-contract=off.  Arithmetic follows the reference operation by
+//     operation, so the FP64 result equals the FP64 CPU restatement bit for
+//     bit (friction's cbrt aside).  It departs from the literal reference
+//     expression only by
+//       * hoisting a per-cell value that the reference recomputes per face
+//         (same operands, same operations, same rounding),
+//       * dropping a term the reference multiplies by an exact 0.0 or 1.0
+//         (axis-normal rotation, the zero contact-wave source), which can
+//         change at most the sign of a zero,
+//       * evaluating the positivity guard's timing test only when its depth
+//         test passes (no side effects, so the outcome is unchanged),
+//       * multiplying by 1/dx when dx is a power of two (an exact scaling).
+//   HX_FAST 1 (fast): the same expressions with divisions that share a
+//     denominator turned into one Newton-refined reciprocal, and FMA
+//     contraction allowed.  Each quotient is within ~2 ulp; the parity tests
+//     hold this build to 1e-12 relative L-infinity against the oracle.
 
-#include <hip/hip_runtime.h>
-#include <stdint.h>
-#include <cstring>
-
-namespace hx {
+namespace HX_NS {
 
 template <typename R>
 struct Kc {
@@ -34,19 +37,66 @@ struct Kc {
 template <typename R> __device__ __forceinline__ R smax(R a, R b) { return (a < b) ? b : a; }
 template <typename R> __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
 
-// Device view of one box (hx_patch with a typed element).
+#if HX_FAST
+// Reciprocal from the hardware estimate and Newton steps r <- r + r(1 - b r)
+// (two for FP64, one for FP32).  Callers only pass finite, non-zero b.
+__device__ __forceinline__ double rcp(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp(float b)
+{
+    float r = __builtin_amdgcn_rcpf(b);
+    const float e = fmaf(-b, r, 1.0f);
+    return fmaf(r, e, r);
+}
 template <typename R>
-struct PatchView {
-    const R* h;
-    const R* hu;
-    const R* hv;
-    const R* eta;   // may be null: eta = h + z
-    const R* z;
-    const R* n;
-    int64_t pitch;
-    int nx, ny;
-    int bl0, bl1, bh0, bh1;   // per side: 0 reflect, 1 transmit, -1 stored ghosts
+struct Div {
+    R r;
+    __device__ __forceinline__ explicit Div(R b) : r(rcp(b)) {}
+    __device__ __forceinline__ R operator()(R a) const { return a * r; }
 };
+
+// sqrt(x) and 1/sqrt(x) together for finite x > 0: hardware estimate, then
+// Goldschmidt steps g <- g + g e, h <- h + h e with e = 1/2 - g h (two for
+// FP64, one for FP32), g -> sqrt(x), 2h -> 1/sqrt(x).
+__device__ __forceinline__ void sqrt_rsq(double x, double& sq, double& rs)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    sq = g;
+    rs = 2.0 * h;
+}
+__device__ __forceinline__ void sqrt_rsq(float x, float& sq, float& rs)
+{
+    const float y = __builtin_amdgcn_rsqf(x);
+    float g = x * y, h = 0.5f * y;
+    const float e = fmaf(-g, h, 0.5f);
+    sq = fmaf(g, e, g);
+    rs = 2.0f * fmaf(h, e, h);
+}
+
+template <typename R> struct SqrtG;
+template <> struct SqrtG<double> { static constexpr double v = 3.132091952673165; };
+template <> struct SqrtG<float> { static constexpr float v = 3.132092f; };
+#else
+template <typename R>
+struct Div {
+    R b;
+    __device__ __forceinline__ explicit Div(R b_) : b(b_) {}
+    __device__ __forceinline__ R operator()(R a) const { return a / b; }
+};
+#endif
 
 // Cell as the advance sees it: hr is the rebuilt depth max(0, eta - z)
 // (src/solvers/Roe.cpp:244-247, src/solvers/HydrostaticRoe.cpp:495-499),
@@ -75,6 +125,21 @@ __device__ __forceinline__ void map_axis(int i, int n, int blo, int bhi, int& sr
 // momenta with their vector rules, and h is rebuilt on this cell's own
 // (edge) bed -- the reference's FillPatchEta + FillPatch + rebuild sequence
 // (src/solvers/Roe.cpp:315-339, src/solvers/HydrostaticRoe.cpp:477-500).
+// Load of a cell known to lie inside the patch (no boundary mapping).
+template <typename R>
+__device__ __forceinline__ Raw<R> load_in(const PatchView<R>& p, int64_t s)
+{
+    Raw<R> c;
+    c.h = p.h[s];
+    c.z = p.z[s];
+    c.n = p.n[s];
+    const R e = p.eta ? p.eta[s] : (c.h + c.z);
+    c.hr = smax(R(0), e - c.z);
+    c.qx = p.hu[s];
+    c.qy = p.hv[s];
+    return c;
+}
+
 template <typename R>
 __device__ __forceinline__ Raw<R> load_adv(const PatchView<R>& p, int i, int j)
 {
@@ -121,10 +186,22 @@ __device__ __forceinline__ RoeCell<R> roe_cell(R hr, R qx, R qy, R z, R n)
     const R hd = (hr >= Kc<R>::toldry) ? hr : R(0);
     const bool wet = hd > R(0);
     c.hd = hd;
+#if HX_FAST
+    // one reciprocal square root gives sqrt(h), 1/h and sqrt(g h)
+    R s, r;
+    sqrt_rsq(wet ? hd : R(1), s, r);
+    const R ih = r * r;
+    c.sq = wet ? s : R(0);
+    c.u = wet ? qx * ih : R(0);
+    c.v = wet ? qy * ih : R(0);
+    c.cg = wet ? SqrtG<R>::v * s : R(0);
+#else
     c.sq = wet ? sqrt(hd) : R(0);
-    c.u = wet ? qx / hd : R(0);
-    c.v = wet ? qy / hd : R(0);
+    const Div<R> ih(wet ? hd : R(1));
+    c.u = wet ? ih(qx) : R(0);
+    c.v = wet ? ih(qy) : R(0);
     c.cg = sqrt(Kc<R>::g * hd);
+#endif
     c.ed = hd + z;
     c.z = z;
     c.n = n;
@@ -134,6 +211,7 @@ __device__ __forceinline__ RoeCell<R> roe_cell(R hr, R qx, R qy, R z, R n)
 template <typename R>
 struct RoeFace {
     R F0, F1, F2, Sm0, Sm1, Sm2, Sp0, Sp1, Sp2;
+    bool src;   // wave-uniform: some lane has a non-zero source share
 };
 
 // DIR 0: x face, normal (1,0); DIR 1: y face, normal (0,1).  A is the
@@ -150,69 +228,101 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
 
     const R hmean = R(0.5) * (hl + hr);
     const R ht = (hmean > k::toldry) ? hmean : k::toldry;
+#if HX_FAST
+    // ct and 1/ct from one reciprocal square root; 1/(2 ct + TOL12) from
+    // 1/ct to first order in TOL12/(2 ct) (relative error below 1e-19)
+    R ct, ict;
+    sqrt_rsq(k::g * ht, ct, ict);
+    const R qh = R(0.5) * ict;
+    const R rden = qh * fma(-k::tol12, qh, R(1));
+#else
     const R ct = sqrt(k::g * ht);
-    const R dsq = A.sq + B.sq + k::tol12;
-    const R ut = (unl * A.sq + unr * B.sq) / dsq;
-    const R vt = (utl * A.sq + utr * B.sq) / dsq;
+#endif
+    const Div<R> idsq(A.sq + B.sq + k::tol12);
+    const R ut = idsq(unl * A.sq + unr * B.sq);
+    const R vt = idsq(utl * A.sq + utr * B.sq);
 
     R l0 = ut - ct;
     const R l1 = ut;
     R l2 = ut + ct;
     {
         R el = unl - A.cg, er = unr - B.cg;
-        if (el < R(0) && er > R(0)) l0 = el * (er - l0) / (er - el);
+        if (el < R(0) && er > R(0)) l0 = Div<R>(er - el)(el * (er - l0));
         el = unl + A.cg;
         er = unr + B.cg;
-        if (el < R(0) && er > R(0)) l2 = er * (l2 - el) / (er - el);
+        if (el < R(0) && er > R(0)) l2 = Div<R>(er - el)(er * (l2 - el));
     }
 
-    const R den = R(2) * ct + k::tol12;
+#if HX_FAST
+    const auto iden = [rden](R a) { return a * rden; };
+    const auto ictd = [ict](R a) { return a * ict; };
+#else
+    const Div<R> iden(R(2) * ct + k::tol12);
+    const Div<R> ictd(ct);
+#endif
     const R dh = hr - hl;
     const R dhu = hr * unr - hl * unl;
     const R dhv = hr * utr - hl * utl;
-    const R a0 = (l2 * dh - dhu) / den;
-    const R a1 = (-vt * dh + dhv) / ct;
-    const R a2 = (-l0 * dh + dhu) / den;
+    const R a0 = iden(l2 * dh - dhu);
+    const R a1 = ictd(-vt * dh + dhv);
+    const R a2 = iden(-l0 * dh + dhu);
 
-    // Bed thrust (Roe.H:151-176) and Manning friction (:182-188).  With
-    // zero roughness the reference's tau is a signed zero, so it is skipped.
+    // Bed thrust (Roe.H:151-176) and Manning friction (:182-188).  On a face
+    // with a flat bed and zero roughness the reference's thrust is
+    // -g (h - 0) * 0 and its tau a signed zero, so the source share is an
+    // exact zero: when that holds on every lane of the wavefront the
+    // evaluation is skipped (a wave-uniform branch).  With zero roughness
+    // on a lane, tau is likewise skipped.
     const R dz = B.z - A.z;
-    const R dd = B.ed - A.ed;
-    const R thrust_a = -k::g * ht * dz;
-    R dzt;
-    if ((dz >= R(0)) && (A.ed < B.z)) dzt = hl;
-    else if ((dz < R(0)) && (B.ed < A.z)) dzt = hr;
-    else dzt = dz;
-    const R hup = (dz >= R(0)) ? hl : hr;
-    const R thrust_b = -k::g * (hup - R(0.5) * fabs(dzt)) * dzt;
-    const R thrust = ((dz * dd >= R(0)) && (ut * dz > R(0))) ? smax(thrust_a, thrust_b) : thrust_b;
     const R nt = R(0.5) * (A.n + B.n);
-    R tau = R(0);
-    if (nt != R(0)) {
-        const R sf = (nt * nt * sqrt(ut * ut + vt * vt) * ut) / smax(k::toldry, ht * cbrt(ht));
-        tau = k::g * ht * sf * ds;
+    const bool flat = __all((dz == R(0)) && (nt == R(0)));
+    R b0 = R(0);
+    if (!flat) {
+        const R dd = B.ed - A.ed;
+        const R thrust_a = -k::g * ht * dz;
+        R dzt;
+        if ((dz >= R(0)) && (A.ed < B.z)) dzt = hl;
+        else if ((dz < R(0)) && (B.ed < A.z)) dzt = hr;
+        else dzt = dz;
+        const R hup = (dz >= R(0)) ? hl : hr;
+        const R thrust_b = -k::g * (hup - R(0.5) * fabs(dzt)) * dzt;
+        const R thrust = ((dz * dd >= R(0)) && (ut * dz > R(0))) ? smax(thrust_a, thrust_b) : thrust_b;
+        R tau = R(0);
+        if (nt != R(0)) {
+            const R sf = Div<R>(smax(k::toldry, ht * cbrt(ht)))(nt * nt * sqrt(ut * ut + vt * vt) * ut);
+            tau = k::g * ht * sf * ds;
+        }
+        b0 = iden(thrust - tau);
     }
-    R b0 = (thrust - tau) / den;
     R b2 = -b0;
 
-    {   // positivity guard, Roe.H:200-247
+    {   // positivity guard, Roe.H:200-247 (with b0 = 0, b/lambda is 0)
         const R l0s = (fabs(l0) > k::tol12) ? l0 : (l0 >= R(0) ? k::tol12 : -k::tol12);
         const R l2s = (fabs(l2) > k::tol12) ? l2 : (l2 >= R(0) ? k::tol12 : -k::tol12);
-        const R h1s = hl + a0 - (b0 / l0s);
-        const R h3s = hr - a2 + (b2 / l2s);
+        R h1s = hl + a0, h3s = hr - a2;
+        if (!flat) {
+#if HX_FAST
+            const R il = rcp(l0s * l2s);   // |l0s|, |l2s| >= TOL12: no underflow
+            h1s = h1s - b0 * (l2s * il);
+            h3s = h3s + b2 * (l0s * il);
+#else
+            h1s = h1s - Div<R>(l0s)(b0);
+            h3s = h3s + Div<R>(l2s)(b2);
+#endif
+        }
         const bool c1 = (h1s < -k::toldry) && (h3s > k::toldry);
         const bool c3 = (h1s > k::toldry) && (h3s < -k::toldry);
         if (c1 || c3) {
             R mlam = fabs(l0);
             mlam = smax(mlam, fabs(l1));
             mlam = smax(mlam, fabs(l2));
-            const R loc_dt = ds / (mlam + k::tol12);
+            const R loc_dt = Div<R>(mlam + k::tol12)(ds);
             const R m1 = -(hl + a0) * fabs(l0);
             const R m3 = -(hr - a2) * l2;
             if (c1) {
                 const R d1 = hl - h1s;
                 const R d1s = (fabs(d1) > k::tol12) ? d1 : (d1 >= R(0) ? k::tol12 : -k::tol12);
-                const R dt1 = (ds * R(0.5) / l0s) * (hl / d1s);
+                const R dt1 = Div<R>(l0s)(ds * R(0.5)) * Div<R>(d1s)(hl);
                 if (dt1 < loc_dt) {
                     if (-m1 >= m3) b0 = m1;
                     b2 = -b0;
@@ -220,7 +330,7 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
             } else {
                 const R d3 = hr - h3s;
                 const R d3s = (fabs(d3) > k::tol12) ? d3 : (d3 >= R(0) ? k::tol12 : -k::tol12);
-                const R dt3 = (ds * R(0.5) / l2s) * (hr / d3s);
+                const R dt3 = Div<R>(l2s)(ds * R(0.5)) * Div<R>(d3s)(hr);
                 if (dt3 < loc_dt) {
                     if (-m3 >= m1) b2 = m3;
                     b0 = -b2;
@@ -242,14 +352,24 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
     F2 -= w1 * ct;
     F0 -= w2; F1 -= w2 * l2; F2 -= w2 * vt;
 
-    // Upwind source split (Roe.H:281-289); the contact wave carries none.
-    R Sm0 = R(0), Sm1 = R(0), Sm2 = R(0), Sp0 = R(0), Sp1 = R(0), Sp2 = R(0);
-    if (l0 > R(0)) { Sm0 += b0; Sm1 += b0 * l0; Sm2 += b0 * vt; }
-    else if (l0 < R(0)) { Sp0 += b0; Sp1 += b0 * l0; Sp2 += b0 * vt; }
-    if (l2 > R(0)) { Sm0 += b2; Sm1 += b2 * l2; Sm2 += b2 * vt; }
-    else if (l2 < R(0)) { Sp0 += b2; Sp1 += b2 * l2; Sp2 += b2 * vt; }
-
     RoeFace<R> o;
+    // Upwind source split (Roe.H:281-289), without branches: each acoustic
+    // wave's share goes to S- when its speed is positive and to S+ when
+    // negative; the contact wave carries none.  Adding a selected 0 in place
+    // of a skipped term leaves every sum as the reference's.
+    o.src = __any(b0 != R(0));
+    R Sm0 = R(0), Sm1 = R(0), Sm2 = R(0), Sp0 = R(0), Sp1 = R(0), Sp2 = R(0);
+    if (o.src) {
+        const R k0b = b0 * l0, k0c = b0 * vt, k2b = b2 * l2, k2c = b2 * vt;
+        const bool p0 = l0 > R(0), n0 = l0 < R(0), p2 = l2 > R(0), n2 = l2 < R(0);
+        Sm0 = (p0 ? b0 : R(0)) + (p2 ? b2 : R(0));
+        Sm1 = (p0 ? k0b : R(0)) + (p2 ? k2b : R(0));
+        Sm2 = (p0 ? k0c : R(0)) + (p2 ? k2c : R(0));
+        Sp0 = (n0 ? b0 : R(0)) + (n2 ? b2 : R(0));
+        Sp1 = (n0 ? k0b : R(0)) + (n2 ? k2b : R(0));
+        Sp2 = (n0 ? k0c : R(0)) + (n2 ? k2c : R(0));
+    }
+
     o.F0 = F0; o.Sm0 = Sm0; o.Sp0 = Sp0;
     if (DIR == 0) {
         o.F1 = F1; o.F2 = F2; o.Sm1 = Sm1; o.Sm2 = Sm2; o.Sp1 = Sp1; o.Sp2 = Sp2;
@@ -265,14 +385,21 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
 // ---------------------------------------------------------------------------
 
 // van Leer slope (HydrostaticRoe.cpp:173-195); |r| on the floating value.
+// The fast build uses the equivalent harmonic-mean form 2 dm dp / (dm + dp)
+// for dm dp > 0 (one reciprocal instead of two quotients).
 template <typename R>
 __device__ __forceinline__ R vl_slope(R qm, R q, R qp)
 {
     const R dm = q - qm;
     const R dp = qp - q;
     if (fabs(dp) <= Kc<R>::tol12) return R(0);
+#if HX_FAST
+    const R pr = dm * dp;
+    return (pr > R(0)) ? R(2) * pr * rcp(dm + dp) : R(0);
+#else
     const R r = dm / dp;
     return ((r + fabs(r)) / (R(1) + fabs(r))) * dp;
+#endif
 }
 
 // Per-cell reconstruction inputs: eta = h_ext + z_ext, u and v guarded by
@@ -288,8 +415,9 @@ __device__ __forceinline__ void hydro_prims(R hr, R qx, R qy, R z, R& eta, R& u,
 {
     eta = hr + z;
     const bool ok = hr > Kc<R>::tol12;
-    u = ok ? qx / hr : R(0);
-    v = ok ? qy / hr : R(0);
+    const Div<R> ih(ok ? hr : R(1));
+    u = ok ? ih(qx) : R(0);
+    v = ok ? ih(qy) : R(0);
 }
 
 // Homogeneous Roe flux on face states (kernels/HydrostaticRoe.H:43-159).
@@ -300,40 +428,71 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
     hi = (hi >= k::toldry) ? hi : R(0);
     hj = (hj >= k::toldry) ? hj : R(0);
     const bool wl = hi > R(0), wr = hj > R(0);
+#if HX_FAST
+    R sl, rl, sr, rr;
+    sqrt_rsq(wl ? hi : R(1), sl, rl);
+    sqrt_rsq(wr ? hj : R(1), sr, rr);
+    const R sql = wl ? sl : R(0);
+    const R sqr = wr ? sr : R(0);
+    const R il = rl * rl, ir = rr * rr;
+    const R ul = wl ? hui * il : R(0);
+    const R vl = wl ? hvi * il : R(0);
+    const R ur = wr ? huj * ir : R(0);
+    const R vr = wr ? hvj * ir : R(0);
+#else
     const R sql = wl ? sqrt(hi) : R(0);
     const R sqr = wr ? sqrt(hj) : R(0);
-    const R ul = wl ? hui / hi : R(0);
-    const R vl = wl ? hvi / hi : R(0);
-    const R ur = wr ? huj / hj : R(0);
-    const R vr = wr ? hvj / hj : R(0);
+    const Div<R> il(wl ? hi : R(1)), ir(wr ? hj : R(1));
+    const R ul = wl ? il(hui) : R(0);
+    const R vl = wl ? il(hvi) : R(0);
+    const R ur = wr ? ir(huj) : R(0);
+    const R vr = wr ? ir(hvj) : R(0);
+#endif
     const R unl = (DIR == 0) ? ul : vl;
     const R utl = (DIR == 0) ? vl : -ul;
     const R unr = (DIR == 0) ? ur : vr;
     const R utr = (DIR == 0) ? vr : -ur;
     const R hmean = R(0.5) * (hi + hj);
     const R ht = (hmean > k::toldry) ? hmean : k::toldry;
+#if HX_FAST
+    R ct, ict;
+    sqrt_rsq(k::g * ht, ct, ict);
+    const R qh = R(0.5) * ict;
+    const R rden = qh * fma(-k::tol12, qh, R(1));
+#else
     const R ct = sqrt(k::g * ht);
-    const R dsq = sql + sqr + k::tol12;
-    const R ut = (unl * sql + unr * sqr) / dsq;
-    const R vt = (utl * sql + utr * sqr) / dsq;
+#endif
+    const Div<R> idsq(sql + sqr + k::tol12);
+    const R ut = idsq(unl * sql + unr * sqr);
+    const R vt = idsq(utl * sql + utr * sqr);
     R l0 = ut - ct;
     const R l1 = ut;
     R l2 = ut + ct;
     {
+#if HX_FAST
+        const R cl = SqrtG<R>::v * sql, cr = SqrtG<R>::v * sqr;
+#else
         const R cl = sqrt(k::g * hi), cr = sqrt(k::g * hj);
+#endif
         R el = unl - cl, er = unr - cr;
-        if (el < R(0) && er > R(0)) l0 = el * (er - l0) / (er - el);
+        if (el < R(0) && er > R(0)) l0 = Div<R>(er - el)(el * (er - l0));
         el = unl + cl;
         er = unr + cr;
-        if (el < R(0) && er > R(0)) l2 = er * (l2 - el) / (er - el);
+        if (el < R(0) && er > R(0)) l2 = Div<R>(er - el)(er * (l2 - el));
     }
-    const R den = R(2) * ct + k::tol12;
+#if HX_FAST
+    const auto iden = [rden](R a) { return a * rden; };
+    const auto ictd = [ict](R a) { return a * ict; };
+#else
+    const Div<R> iden(R(2) * ct + k::tol12);
+    const Div<R> ictd(ct);
+#endif
     const R dh = hj - hi;
     const R dhu = hj * unr - hi * unl;
     const R dhv = hj * utr - hi * utl;
-    const R a0 = (l2 * dh - dhu) / den;
-    const R a1 = (-vt * dh + dhv) / ct;
-    const R a2 = (-l0 * dh + dhu) / den;
+    const R a0 = iden(l2 * dh - dhu);
+    const R a1 = ictd(-vt * dh + dhv);
+    const R a2 = iden(-l0 * dh + dhu);
     R f0 = R(0.5) * (hi * unl + hj * unr);
     R f1 = R(0.5) * (hi * unl * unl + R(0.5) * k::g * hi * hi +
                      hj * unr * unr + R(0.5) * k::g * hj * hj);
@@ -351,7 +510,7 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
 
 // One face: NODATA wall, face bed, MUSCL face states, Audusse depths,
 // homogeneous Roe, hydrostatic pressure correction (HydrostaticRoe.cpp:234-396).
-// Only the normal component of FL/FR differs from the Roe flux.
+// Only the normal component of FL/FR differs from the Roe flux, by pL/pR.
 template <typename R, int DIR>
 __device__ __forceinline__ void hydro_face(const HydroCell<R>& L, const HydroCell<R>& Rc,
                                            R& F0, R& F1, R& F2, R& pL, R& pR)
@@ -383,33 +542,31 @@ __device__ __forceinline__ void hydro_face(const HydroCell<R>& L, const HydroCel
 
 // ---------------------------------------------------------------------------
 // CFL accumulation of one cell (src/solvers/Roe.cpp:38-74 and
-// HydrostaticRoe.cpp:37-63, identical cell functions).  Instead of
-// min over cells of 1/a, the maximum a is kept: correctly rounded division
-// is monotone, so fl(1/max a) = min fl(1/a) exactly.  Dry cells contribute
-// the 1e6 and NODATA cells the 1e30 of the reference through flags.
+// HydrostaticRoe.cpp:37-63, identical cell functions).  The maximum of
+// a = (|u|+c)/dx + (|v|+c)/dy is kept; see slot_dt_est.
 // ---------------------------------------------------------------------------
-enum : unsigned { CFL_WET = 1u, CFL_DRY = 2u, CFL_NODATA = 4u };
-
-struct CflSlot {
-    unsigned long long amax;   // bits of a non-negative double
-    unsigned long long vmax;   // bits of a non-negative double
-    unsigned int flags;
-    unsigned int pad;
-};
-
 template <typename R>
 struct CflAcc {
     R amax, vmax;
     unsigned flags;
     __device__ __forceinline__ void init() { amax = R(0); vmax = R(0); flags = 0u; }
-    __device__ __forceinline__ void add(R h, R qx, R qy, R z, R dx, R dy)
+    __device__ __forceinline__ void add(R h, R qx, R qy, R z, R dx, R dy, R rdx, R rdy, bool mul)
     {
         if (fabs(z - R(-9999)) < Kc<R>::tol10) { flags |= CFL_NODATA; return; }
         if (h > R(1e-12)) {
-            const R u = qx / h, v = qy / h;
+#if HX_FAST
+            R s, r;
+            sqrt_rsq(h, s, r);
+            const R ih = r * r;
+            const R u = qx * ih, v = qy * ih;
+            const R c = SqrtG<R>::v * s;
+#else
+            const Div<R> ih(h);
+            const R u = ih(qx), v = ih(qy);
             const R c = sqrt(Kc<R>::g * h);
+#endif
             const R sx = fabs(u) + c, sy = fabs(v) + c;
-            const R a = sx / dx + sy / dy;
+            const R a = mul ? (sx * rdx + sy * rdy) : (sx / dx + sy / dy);
             amax = smax(amax, a);
             vmax = smax(vmax, smax(fabs(u), fabs(v)));
             flags |= CFL_WET;
@@ -418,19 +575,6 @@ struct CflAcc {
         }
     }
 };
-
-// dt estimate (before cfl) from a reduced slot.
-template <typename R>
-__host__ __device__ inline R slot_dt_est(unsigned long long abits, unsigned flags)
-{
-    double a;
-    memcpy(&a, &abits, sizeof(a));
-    R est = (sizeof(R) == 8) ? R(1.7976931348623157e308) : R(3.40282347e38f);
-    if (flags & CFL_WET) est = R(1) / R(a);
-    if ((flags & CFL_DRY) && R(1.e6) < est) est = R(1.e6);
-    if ((flags & CFL_NODATA) && R(1.e30) < est) est = R(1.e30);
-    return est;
-}
 
 __device__ __forceinline__ unsigned long long dbits(double x)
 {
@@ -471,4 +615,4 @@ __device__ __forceinline__ void cfl_commit(CflAcc<R> acc, CflSlot* slot)
     }
 }
 
-}  // namespace hx
+}  // namespace HX_NS

@@ -1,311 +1,32 @@
-// hx_kernels.hip -- fused per-level advance and auxiliary kernels for
-// CDNA4 (gfx950), plus the C-ABI declared in include/hydroexa.h.
-//
-// Work decomposition of the fused advance ("strip march"): one wavefront
-// owns a strip of columns and marches up a run of rows.  Each lane holds one
-// column.  A lane computes the y face above its cell in registers and keeps
-// it as the next row's lower face, so every y face is computed once; it
-// computes the x face on its left from its left neighbour's hoisted cell
-// quantities (a cross-lane shuffle) and takes its right face from the lane
-// on its right, so every x face is computed once as well.  Edge lanes only
-// feed neighbours: plain Roe updates 62 of 64 columns, the hydrostatic
-// solver (two-cell stencil) 60.  No LDS tile, no barrier in the march; the
-// rows are read once, coalesced, straight from HBM.
+// hx_kernels.hip -- exact arithmetic build of the fused advance, the
+// auxiliary kernels (CFL reduction, tagging, masked_average_down, pruning)
+// and the C-ABI declared in include/hydroexa.h.  Compiled with
+// -ffp-contract=off; the fast build of the advance lives in hx_fast.hip.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <new>
 
 #include "../../include/hydroexa.h"
-#include "hx_device.hpp"
+#include "hx_types.hpp"
 
 namespace hx {
+#define HX_NS exact
+#define HX_FAST 0
+#include "hx_device.hpp"
+#include "hx_advance.inc"
+#undef HX_NS
+#undef HX_FAST
 
-constexpr int kWaves = 4;              // wavefronts per workgroup
-constexpr int kBlock = 64 * kWaves;
-constexpr int kRoeCols = 62;           // columns updated per plain-Roe strip
-constexpr int kHydroCols = 60;         // columns updated per hydrostatic strip
-
-template <typename R>
-struct StepArgs {
-    PatchView<R> p;
-    R* ho;
-    R* huo;
-    R* hvo;
-    int64_t opitch;
-    R dx, dy;
-    int rows;          // rows marched per wavefront
-    int nsx, nstrips;  // strips across x, total strips
-    double dt_host;    // used when rd == nullptr
-    double cfl;
-    CflSlot* rd;       // slot holding the CFL reduction of this step's input
-    CflSlot* acc;      // slot receiving the CFL reduction of the output
-    CflSlot* clr;      // slot cleared for the step after next
-    double* dt_hist;
-    double* vel_hist;
-    int* err;
-    int step;
-};
-
-// dt of this step: the host value, or cfl * (reduced estimate) when the
-// previous step reduced it on the device.  Returns false on dt collapse
-// (estimate < 1e-12: src/solvers/Roe.cpp:88-92), which the caller reports.
-template <typename R>
-__device__ __forceinline__ bool step_dt(const StepArgs<R>& a, R& dt)
-{
-    if (a.rd == nullptr) {
-        dt = R(a.dt_host);
-        return true;
-    }
-    if (*(volatile int*)a.err) return false;
-    const unsigned long long ab = a.rd->amax;
-    const unsigned fl = a.rd->flags;
-    const R est = slot_dt_est<R>(ab, fl);
-    if (est < R(1.e-12)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicCAS(a.err, 0, a.step + 1);
-        return false;
-    }
-    dt = est * R(a.cfl);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (a.dt_hist) a.dt_hist[a.step] = (double)dt;
-        if (a.vel_hist) a.vel_hist[a.step] = __longlong_as_double((long long)a.rd->vmax);
-        if (a.clr) { a.clr->amax = 0ull; a.clr->vmax = 0ull; a.clr->flags = 0u; }
-    }
-    return true;
-}
-
-template <typename R>
-__device__ __forceinline__ RoeCell<R> shfl_up_cell(const RoeCell<R>& c)
-{
-    RoeCell<R> o;
-    o.hd = __shfl_up(c.hd, 1, 64);
-    o.sq = __shfl_up(c.sq, 1, 64);
-    o.u = __shfl_up(c.u, 1, 64);
-    o.v = __shfl_up(c.v, 1, 64);
-    o.cg = __shfl_up(c.cg, 1, 64);
-    o.ed = __shfl_up(c.ed, 1, 64);
-    o.z = __shfl_up(c.z, 1, 64);
-    o.n = __shfl_up(c.n, 1, 64);
-    return o;
-}
-
-// ---------------------------------------------------------------------------
-// Plain Roe strip: src/solvers/Roe.cpp:327-432 fused (rebuild, x/y faces,
-// fluctuation-difference update), plus the CFL partial of the new state.
-// ---------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ void roe_strip(const StepArgs<R>& a, int strip, R dt, CflAcc<R>& acc,
-                                          bool do_cfl)
-{
-    const PatchView<R>& p = a.p;
-    const int lane = threadIdx.x & 63;
-    const int sxi = strip % a.nsx, syi = strip / a.nsx;
-    const int c = sxi * kRoeCols - 1 + lane;
-    const int cl = min(c, p.nx);
-    const int y0 = syi * a.rows;
-    const int y1 = min(y0 + a.rows, p.ny);
-    const bool owner = (lane >= 1) && (lane <= kRoeCols) && (c < p.nx);
-    const R dtdx = dt / a.dx, dtdy = dt / a.dy;
-
-    Raw<R> rw = load_adv(p, cl, y0 - 1);
-    const RoeCell<R> cb = roe_cell(rw.hr, rw.qx, rw.qy, rw.z, rw.n);
-    Raw<R> rc = load_adv(p, cl, y0);
-    RoeCell<R> cc = roe_cell(rc.hr, rc.qx, rc.qy, rc.z, rc.n);
-    RoeFace<R> fb = roe_face<R, 1>(cb, cc, a.dy);
-
-    Raw<R> rn = load_adv(p, cl, y0 + 1);
-    for (int j = y0; j < y1; ++j) {
-        // software prefetch of the row after next
-        const Raw<R> rnn = (j + 2 <= y1) ? load_adv(p, cl, j + 2) : rn;
-        const RoeCell<R> cn = roe_cell(rn.hr, rn.qx, rn.qy, rn.z, rn.n);
-        const RoeFace<R> ft = roe_face<R, 1>(cc, cn, a.dy);
-        const RoeCell<R> cw = shfl_up_cell(cc);
-        const RoeFace<R> fx = roe_face<R, 0>(cw, cc, a.dx);
-        const R rF0 = __shfl_down(fx.F0, 1, 64);
-        const R rF1 = __shfl_down(fx.F1, 1, 64);
-        const R rF2 = __shfl_down(fx.F2, 1, 64);
-        const R rS0 = __shfl_down(fx.Sp0, 1, 64);
-        const R rS1 = __shfl_down(fx.Sp1, 1, 64);
-        const R rS2 = __shfl_down(fx.Sp2, 1, 64);
-        const R u0 = rc.hr - dtdx * (rF0 - fx.F0) - dtdy * (ft.F0 - fb.F0)
-                   - dtdx * (rS0 + fx.Sm0) - dtdy * (ft.Sp0 + fb.Sm0);
-        const R u1 = rc.qx - dtdx * (rF1 - fx.F1) - dtdy * (ft.F1 - fb.F1)
-                   - dtdx * (rS1 + fx.Sm1) - dtdy * (ft.Sp1 + fb.Sm1);
-        const R u2 = rc.qy - dtdx * (rF2 - fx.F2) - dtdy * (ft.F2 - fb.F2)
-                   - dtdx * (rS2 + fx.Sm2) - dtdy * (ft.Sp2 + fb.Sm2);
-        if (owner) {
-            const int64_t o = (int64_t)j * a.opitch + c;
-            a.ho[o] = u0;
-            a.huo[o] = u1;
-            a.hvo[o] = u2;
-            if (do_cfl) acc.add(u0, u1, u2, rc.z, a.dx, a.dy);
-        }
-        fb = ft;
-        cc = cn;
-        rc = rn;
-        rn = rnn;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Hydrostatic strip: src/solvers/HydrostaticRoe.cpp:487-604 fused (rebuild,
-// MUSCL van Leer face states, Audusse depths, homogeneous Roe, pressure
-// correction, update, semi-implicit Manning, dry clamp).
-// ---------------------------------------------------------------------------
-template <typename R>
-struct HRow {     // one cell of the marching window
-    R eta, u, v, z;      // reconstruction inputs
-    R h, qx, qy, n;      // stored state (for the update) and roughness
-};
-
-template <typename R>
-__device__ __forceinline__ HRow<R> hrow_load(const PatchView<R>& p, int i, int j)
-{
-    const Raw<R> r = load_adv(p, i, j);
-    HRow<R> o;
-    hydro_prims(r.hr, r.qx, r.qy, r.z, o.eta, o.u, o.v);
-    o.z = r.z;
-    o.h = r.h;
-    o.qx = r.qx;
-    o.qy = r.qy;
-    o.n = r.n;
-    return o;
-}
-
-template <typename R>
-__device__ __forceinline__ void hydro_strip(const StepArgs<R>& a, int strip, R dt, CflAcc<R>& acc,
-                                            bool do_cfl)
-{
-    using k = Kc<R>;
-    const PatchView<R>& p = a.p;
-    const int lane = threadIdx.x & 63;
-    const int sxi = strip % a.nsx, syi = strip / a.nsx;
-    const int c = sxi * kHydroCols - 2 + lane;
-    const int cl = min(c, p.nx + 1);
-    const int y0 = syi * a.rows;
-    const int y1 = min(y0 + a.rows, p.ny);
-    const bool owner = (lane >= 2) && (lane < 2 + kHydroCols) && (c < p.nx);
-    const R dtdx = dt / a.dx, dtdy = dt / a.dy;
-
-    // window: rows j-1 (m), j (c), j+1 (n); y slopes of j and j+1
-    HRow<R> rm = hrow_load(p, cl, y0 - 2);
-    HRow<R> r0 = hrow_load(p, cl, y0 - 1);
-    HRow<R> r1 = hrow_load(p, cl, y0);
-    HRow<R> r2 = hrow_load(p, cl, y0 + 1);
-    // y slopes of rows y0-1 and y0, then the lower face of row y0
-    HydroCell<R> hb, hc;
-    hb.eta = r0.eta; hb.u = r0.u; hb.v = r0.v; hb.z = r0.z;
-    hb.se = vl_slope(rm.eta, r0.eta, r1.eta);
-    hb.su = vl_slope(rm.u, r0.u, r1.u);
-    hb.sv = vl_slope(rm.v, r0.v, r1.v);
-    hc.eta = r1.eta; hc.u = r1.u; hc.v = r1.v; hc.z = r1.z;
-    hc.se = vl_slope(r0.eta, r1.eta, r2.eta);
-    hc.su = vl_slope(r0.u, r1.u, r2.u);
-    hc.sv = vl_slope(r0.v, r1.v, r2.v);
-    R bF0, bF1, bF2;   // FR of the lower face of the current row
-    {
-        R f0, f1, f2, pl, pr;
-        hydro_face<R, 1>(hb, hc, f0, f1, f2, pl, pr);
-        bF0 = f0; bF1 = f1; bF2 = f2 + pr;
-    }
-    // rows: cur = r1 (slopes in hc), nxt = r2
-    HRow<R> cur = r1, nxt = r2;
-    for (int j = y0; j < y1; ++j) {
-        const HRow<R> nn = hrow_load(p, cl, j + 2);
-        HydroCell<R> hn;
-        hn.eta = nxt.eta; hn.u = nxt.u; hn.v = nxt.v; hn.z = nxt.z;
-        hn.se = vl_slope(cur.eta, nxt.eta, nn.eta);
-        hn.su = vl_slope(cur.u, nxt.u, nn.u);
-        hn.sv = vl_slope(cur.v, nxt.v, nn.v);
-        R tF0, tF1, tF2, tpl, tpr;   // upper face of the current row
-        hydro_face<R, 1>(hc, hn, tF0, tF1, tF2, tpl, tpr);
-
-        // x slopes of the current row from the neighbouring lanes
-        HydroCell<R> hx;
-        hx.eta = cur.eta; hx.u = cur.u; hx.v = cur.v; hx.z = cur.z;
-        {
-            const R em = __shfl_up(cur.eta, 1, 64), ep = __shfl_down(cur.eta, 1, 64);
-            const R um = __shfl_up(cur.u, 1, 64), up = __shfl_down(cur.u, 1, 64);
-            const R vm = __shfl_up(cur.v, 1, 64), vp = __shfl_down(cur.v, 1, 64);
-            hx.se = vl_slope(em, cur.eta, ep);
-            hx.su = vl_slope(um, cur.u, up);
-            hx.sv = vl_slope(vm, cur.v, vp);
-        }
-        HydroCell<R> hw;
-        hw.eta = __shfl_up(hx.eta, 1, 64);
-        hw.u = __shfl_up(hx.u, 1, 64);
-        hw.v = __shfl_up(hx.v, 1, 64);
-        hw.z = __shfl_up(hx.z, 1, 64);
-        hw.se = __shfl_up(hx.se, 1, 64);
-        hw.su = __shfl_up(hx.su, 1, 64);
-        hw.sv = __shfl_up(hx.sv, 1, 64);
-        R xF0, xF1, xF2, xpl, xpr;   // left x face of this lane's cell
-        hydro_face<R, 0>(hw, hx, xF0, xF1, xF2, xpl, xpr);
-        // FL of the right face comes from the lane on the right
-        const R rL0 = __shfl_down(xF0, 1, 64);
-        const R rL1 = __shfl_down(xF1 + xpl, 1, 64);
-        const R rL2 = __shfl_down(xF2, 1, 64);
-        // FR of this lane's left face
-        const R lR0 = xF0, lR1 = xF1 + xpr, lR2 = xF2;
-        // FL of the upper face, FR of the lower face
-        const R tL0 = tF0, tL1 = tF1, tL2 = tF2 + tpl;
-
-        R u0 = cur.h - dtdx * (rL0 - lR0) - dtdy * (tL0 - bF0);
-        R u1 = cur.qx - dtdx * (rL1 - lR1) - dtdy * (tL1 - bF1);
-        R u2 = cur.qy - dtdx * (rL2 - lR2) - dtdy * (tL2 - bF2);
-        if (u0 >= k::toldry) {
-            const R nsq = cur.n * cur.n;
-            const R inv_h = R(1) / u0;
-            const R uu = u1 * inv_h, vv = u2 * inv_h;
-            const R modM = sqrt(uu * uu + vv * vv);
-            if (nsq > k::tol12 && modM > k::tol12) {
-                const R h43 = u0 * cbrt(u0);
-                const R tt = dt * k::g * nsq * modM / h43;
-                const R ff = sqrt(R(1) + R(4) * tt);
-                const R fac = R(2) / (R(1) + ff);
-                u1 = u1 * fac;
-                u2 = u2 * fac;
-            }
-        } else {
-            u0 = smax(u0, R(0));
-            u1 = R(0);
-            u2 = R(0);
-        }
-        if (owner) {
-            const int64_t o = (int64_t)j * a.opitch + c;
-            a.ho[o] = u0;
-            a.huo[o] = u1;
-            a.hvo[o] = u2;
-            if (do_cfl) acc.add(u0, u1, u2, cur.z, a.dx, a.dy);
-        }
-        bF0 = tF0; bF1 = tF1; bF2 = tF2 + tpr;
-        hc = hn;
-        cur = nxt;
-        nxt = nn;
-    }
-}
-
-template <typename R, int SOLVER>
-__global__ __launch_bounds__(kBlock) void advance_kernel(StepArgs<R> a)
-{
-    R dt;
-    const bool ok = step_dt(a, dt);
-    const bool do_cfl = (a.acc != nullptr);
-    CflAcc<R> acc;
-    acc.init();
-    const int strip = blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (ok && strip < a.nstrips) {
-        if (SOLVER == HX_SOLVER_ROE) roe_strip<R>(a, strip, dt, acc, do_cfl);
-        else hydro_strip<R>(a, strip, dt, acc, do_cfl);
-    }
-    if (do_cfl) cfl_commit(acc, a.acc);
-}
+using namespace exact;
 
 // CFL reduction of a stored state (compute_dt_Impl's ReduceOps pass).
 template <typename R>
-__global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy, CflSlot* slot)
+__global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy, R rdx, R rdy,
+                                                     int mul, CflSlot* slot)
 {
     CflAcc<R> acc;
     acc.init();
@@ -314,7 +35,7 @@ __global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy,
          q += (int64_t)gridDim.x * blockDim.x) {
         const int j = (int)(q / p.nx), i = (int)(q - (int64_t)j * p.nx);
         const int64_t s = (int64_t)j * p.pitch + i;
-        acc.add(p.h[s], p.hu[s], p.hv[s], p.z[s], dx, dy);
+        acc.add(p.h[s], p.hu[s], p.hv[s], p.z[s], dx, dy, rdx, rdy, mul != 0);
     }
     cfl_commit(acc, slot);
 }
@@ -488,6 +209,8 @@ struct hx_ctx {
     int32_t* pboxes;
     int pcap;
     int rows[2][2];      // [solver][dtype]
+    int occ[2][2];       // [solver][dtype] minimum waves per SIMD requested
+    int arith;           // HX_ARITH_EXACT or HX_ARITH_FAST
 };
 
 static int hip_status(hipError_t e)
@@ -506,6 +229,20 @@ static int default_rows(int solver, int dtype)
     (void)solver;
     (void)dtype;
     return 64;
+}
+
+static int default_occ(int solver, int dtype)
+{
+    (void)solver;
+    (void)dtype;
+    return 1;
+}
+
+// 1/d is exact when d is a power of two (mantissa bits all zero).
+static bool is_pow2(double d)
+{
+    int e;
+    return d > 0.0 && std::frexp(d, &e) == 0.5;
 }
 
 extern "C" int hx_abi_version(void) { return HX_ABI_VERSION; }
@@ -540,7 +277,11 @@ extern "C" int hx_ctx_create(hx_ctx** out, int device)
         return HX_E_NOMEM;
     }
     for (int s = 0; s < 2; ++s)
-        for (int d = 0; d < 2; ++d) c->rows[s][d] = default_rows(s, d);
+        for (int d = 0; d < 2; ++d) {
+            c->rows[s][d] = default_rows(s, d);
+            c->occ[s][d] = default_occ(s, d);
+        }
+    c->arith = HX_ARITH_EXACT;
     hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, 0, c->slots, 4, c->err);
     HX_TRY(hipDeviceSynchronize());
     *out = c;
@@ -560,11 +301,23 @@ extern "C" int hx_ctx_destroy(hx_ctx* c)
     return HX_OK;
 }
 
-extern "C" int hx_set_strip_rows(hx_ctx* c, int solver, int dtype, int rows)
+extern "C" int hx_set_tuning(hx_ctx* c, int solver, int dtype, int rows, int occupancy)
 {
     if (!c || solver < 0 || solver > 1 || dtype < 0 || dtype > 1 || rows < 0 || rows > 1 << 20)
         return HX_E_BADARG;
+    bool known = (occupancy == 0);
+    for (int v = 0; v < kOccVariants; ++v) known = known || (occupancy == kOcc[v]);
+    if (!known) return HX_E_BADARG;
     c->rows[solver][dtype] = rows ? rows : default_rows(solver, dtype);
+    c->occ[solver][dtype] = occupancy ? occupancy : default_occ(solver, dtype);
+    return HX_OK;
+}
+
+extern "C" int hx_set_arith(hx_ctx* c, int mode)
+{
+    if (!c) return HX_E_STATE;
+    if (mode != HX_ARITH_EXACT && mode != HX_ARITH_FAST) return HX_E_BADARG;
+    c->arith = mode;
     return HX_OK;
 }
 
@@ -605,8 +358,13 @@ static PatchView<R> view_of(const hx_patch* in)
 }
 
 template <typename R>
-static void fill_geometry(StepArgs<R>& a, int solver, int rows)
+static void fill_geometry(StepArgs<R>& a, int solver, int rows, double dx, double dy)
 {
+    a.dx = (R)dx;
+    a.dy = (R)dy;
+    a.rdx = (R)(1.0 / dx);
+    a.rdy = (R)(1.0 / dy);
+    a.dx_pow2 = is_pow2(dx) && is_pow2(dy) ? 1 : 0;
     const int cols = (solver == HX_SOLVER_ROE) ? kRoeCols : kHydroCols;
     a.rows = rows;
     a.nsx = (a.p.nx + cols - 1) / cols;
@@ -615,14 +373,12 @@ static void fill_geometry(StepArgs<R>& a, int solver, int rows)
 }
 
 template <typename R>
-static int launch_advance(int solver, const StepArgs<R>& a, hipStream_t st)
+static int launch(const hx_ctx* c, int solver, int dtype, const StepArgs<R>& a, hipStream_t st)
 {
-    const int nblk = (a.nstrips + kWaves - 1) / kWaves;
-    if (solver == HX_SOLVER_ROE)
-        hipLaunchKernelGGL((advance_kernel<R, HX_SOLVER_ROE>), dim3(nblk), dim3(kBlock), 0, st, a);
-    else
-        hipLaunchKernelGGL((advance_kernel<R, HX_SOLVER_HYDROSTATIC_ROE>), dim3(nblk), dim3(kBlock), 0, st, a);
-    return hip_status(hipGetLastError());
+    const int occ = c->occ[solver][dtype];
+    const int rc = (c->arith == HX_ARITH_FAST) ? fast::launch_advance<R>(solver, occ, a, st)
+                                               : exact::launch_advance<R>(solver, occ, a, st);
+    return rc ? HX_E_HIP : HX_OK;
 }
 
 template <typename R>
@@ -635,11 +391,9 @@ static int advance_t(hx_ctx* c, int solver, const hx_patch* in, double dt, void*
     a.huo = (R*)huo;
     a.hvo = (R*)hvo;
     a.opitch = opitch;
-    a.dx = (R)in->dx;
-    a.dy = (R)in->dy;
     a.dt_host = dt;
-    fill_geometry(a, solver, c->rows[solver][in->dtype]);
-    return launch_advance(solver, a, st);
+    fill_geometry(a, solver, c->rows[solver][in->dtype], in->dx, in->dy);
+    return launch(c, solver, in->dtype, a, st);
 }
 
 extern "C" int hx_advance_level(hx_ctx* c, int solver, const hx_patch* in, double dt, void* ho,
@@ -662,8 +416,9 @@ static int launch_cfl(const hx_patch* in, CflSlot* slot, hipStream_t st)
     const int64_t ncell = (int64_t)in->nx * in->ny;
     int64_t nblk = (ncell + kBlock - 1) / kBlock;
     if (nblk > 4096) nblk = 4096;
+    const int mul = is_pow2(in->dx) && is_pow2(in->dy) ? 1 : 0;
     hipLaunchKernelGGL(cfl_kernel<R>, dim3((unsigned)nblk), dim3(kBlock), 0, st, view_of<R>(in),
-                       (R)in->dx, (R)in->dy, slot);
+                       (R)in->dx, (R)in->dy, (R)(1.0 / in->dx), (R)(1.0 / in->dy), mul, slot);
     return hip_status(hipGetLastError());
 }
 
@@ -801,13 +556,11 @@ static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
     StepArgs<R> a{};
     a.p = view_of<R>(in);
     a.opitch = in->pitch;
-    a.dx = (R)in->dx;
-    a.dy = (R)in->dy;
     a.cfl = cfl;
     a.err = c->err;
     a.dt_hist = c->hist;
     a.vel_hist = c->hist + c->hist_cap;
-    fill_geometry(a, solver, c->rows[solver][in->dtype]);
+    fill_geometry(a, solver, c->rows[solver][in->dtype], in->dx, in->dy);
     // reset slots and the error flag, then reduce the initial state's CFL
     hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots, 3, c->err);
     int rc = launch_cfl<R>(in, c->slots, st);
@@ -827,7 +580,7 @@ static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
         a.acc = c->slots + ((s + 1) % 3);
         a.clr = c->slots + ((s + 2) % 3);
         a.step = s;
-        rc = launch_advance(solver, a, st);
+        rc = launch(c, solver, in->dtype, a, st);
         if (rc) return rc;
     }
     return HX_OK;

@@ -1,0 +1,96 @@
+// hx_types.hpp -- types shared by the host C-ABI and both arithmetic builds of
+// the kernels (exact: correctly rounded divisions and no FMA contraction,
+// bit-identical to the FP64 CPU restatement; fast: shared Newton-refined
+// reciprocals and contraction, held to 1e-12 relative by the parity tests).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstring>
+
+namespace hx {
+
+constexpr int kWaves = 4;              // wavefronts per workgroup
+constexpr int kBlock = 64 * kWaves;
+constexpr int kRoeCols = 62;           // columns updated per plain-Roe strip
+constexpr int kHydroCols = 60;         // columns updated per hydrostatic strip
+
+// Device view of one box (hx_patch with a typed element).
+template <typename R>
+struct PatchView {
+    const R* h;
+    const R* hu;
+    const R* hv;
+    const R* eta;   // may be null: eta = h + z
+    const R* z;
+    const R* n;
+    int64_t pitch;
+    int nx, ny;
+    int bl0, bl1, bh0, bh1;   // per side: 0 reflect, 1 transmit, -1 stored ghosts
+};
+
+enum : unsigned { CFL_WET = 1u, CFL_DRY = 2u, CFL_NODATA = 4u };
+
+struct CflSlot {
+    unsigned long long amax;   // bits of a non-negative double
+    unsigned long long vmax;   // bits of a non-negative double
+    unsigned int flags;
+    unsigned int pad;
+};
+
+// dt estimate (before cfl) from a reduced slot: fl(1/max a) equals the
+// reference's min over cells of fl(1/a) because correctly rounded division
+// is monotone; dry cells contribute 1e6 and NODATA cells 1e30
+// (src/solvers/Roe.cpp:42-50).
+template <typename R>
+__host__ __device__ inline R slot_dt_est(unsigned long long abits, unsigned flags)
+{
+    double a;
+    memcpy(&a, &abits, sizeof(a));
+    R est = (sizeof(R) == 8) ? R(1.7976931348623157e308) : R(3.40282347e38f);
+    if (flags & CFL_WET) est = R(1) / R(a);
+    if ((flags & CFL_DRY) && R(1.e6) < est) est = R(1.e6);
+    if ((flags & CFL_NODATA) && R(1.e30) < est) est = R(1.e30);
+    return est;
+}
+
+template <typename R>
+struct StepArgs {
+    PatchView<R> p;
+    R* ho;
+    R* huo;
+    R* hvo;
+    int64_t opitch;
+    R dx, dy;
+    R rdx, rdy;        // 1/dx, 1/dy
+    int dx_pow2;       // 1/dx and 1/dy are exact (dx, dy powers of two)
+    int rows;          // rows marched per wavefront
+    int nsx, nstrips;  // strips across x, total strips
+    double dt_host;    // used when rd == nullptr
+    double cfl;
+    CflSlot* rd;       // slot holding the CFL reduction of this step's input
+    CflSlot* acc;      // slot receiving the CFL reduction of the output
+    CflSlot* clr;      // slot cleared for the step after next
+    double* dt_hist;
+    double* vel_hist;
+    int* err;
+    int step;
+};
+
+// Occupancy variants compiled for the advance kernel (minimum waves per
+// SIMD requested from the register allocator).
+constexpr int kOccVariants = 3;
+constexpr int kOcc[kOccVariants] = {1, 3, 4};
+
+// Host launchers of the two arithmetic builds.
+namespace exact {
+template <typename R>
+int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
+}
+namespace fast {
+template <typename R>
+int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
+}
+
+}  // namespace hx

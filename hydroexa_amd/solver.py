@@ -195,9 +195,18 @@ class SolverVariant:
         return dth[:nsteps], vh[:nsteps]
 
 
-def set_strip_rows(model: str, dtype: torch.dtype, rows: int) -> None:
-    L.check(L.load().hx_set_strip_rows(context(), MODELS[model], _DT[dtype], int(rows)),
-            "hx_set_strip_rows")
+def set_tuning(model: str, dtype: torch.dtype, rows: int = 0, occupancy: int = 0) -> None:
+    """Launch shape of the fused advance: rows marched per wavefront and the
+    minimum waves per SIMD (0 restores the defaults)."""
+    L.check(L.load().hx_set_tuning(context(), MODELS[model], _DT[dtype], int(rows), int(occupancy)),
+            "hx_set_tuning")
+
+
+def set_arith(mode: str) -> None:
+    """'exact' (default; bit-identical to the reference's operation order in
+    FP64) or 'fast' (shared reciprocals and FMA contraction)."""
+    code = {"exact": L.HX_ARITH_EXACT, "fast": L.HX_ARITH_FAST}[mode]
+    L.check(L.load().hx_set_arith(context(), code), "hx_set_arith")
 
 
 def masked_average_down(fine: torch.Tensor, ratio: Sequence[int], dx_fine: float,

@@ -48,6 +48,13 @@ extern "C" {
 #define HX_F64 0
 #define HX_F32 1
 
+/* Arithmetic builds of the fused advance.  EXACT: correctly rounded
+ * divisions, no FMA contraction; FP64 results equal the reference's
+ * operation order bit for bit.  FAST: shared Newton-refined reciprocals and
+ * FMA contraction, each quotient within ~2 ulp. */
+#define HX_ARITH_EXACT 0
+#define HX_ARITH_FAST  1
+
 /*
  * One box of one level, structure of arrays.  Mirrors the per-FAB view the
  * reference builds in MFIter loops (Array4 of U with comps h, hu, hv and of
@@ -160,10 +167,19 @@ int hx_run_level(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
                  double *vel_hist, int32_t *steps_done, void *stream);
 
 /*
- * Launch-shape control for the fused advance (the autotuner's knob):
- * rows marched per wavefront.  0 restores the default.
+ * Launch-shape control for the fused advance (the autotuner's knobs): rows
+ * marched per wavefront (0 = default) and the minimum waves per SIMD the
+ * register allocator must leave room for (0 = default; 1, 3 or 4).  No
+ * reference counterpart (AMReX picks launch shapes itself).
  */
-int hx_set_strip_rows(hx_ctx *ctx, int solver, int dtype, int rows);
+int hx_set_tuning(hx_ctx *ctx, int solver, int dtype, int rows, int occupancy);
+
+/*
+ * Select the arithmetic build of the fused advance (HX_ARITH_EXACT, the
+ * default, or HX_ARITH_FAST).  No reference counterpart; the reference's
+ * arithmetic is what HX_ARITH_EXACT reproduces.
+ */
+int hx_set_arith(hx_ctx *ctx, int mode);
 
 #ifdef __cplusplus
 }
