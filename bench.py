@@ -9,6 +9,11 @@ Prints one JSON line: cell-updates/s, the HBM-roofline fraction of the
 advance kernel (algorithmic 64 B per cell-update over its average duration,
 HIP events on the kernel's stream) and the oracle's CPU throughput on a
 bounded sample of the same workload.
+
+The headline uses the fast arithmetic build (shared reciprocals, FMA
+contraction); the exact build (bit-identical to the FP64 oracle) is timed on
+the same workload, and the relative L-infinity distance between the two
+final states after all timed steps is reported beside the numbers.
 """
 from __future__ import annotations
 
@@ -39,6 +44,7 @@ def parse():
                     help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time the other workloads")
+    ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
     return ap.parse_args()
 
 
@@ -63,13 +69,13 @@ def cpu_baseline(w, budget_s):
                       f"{el:.1f} s, OMP threads {threads}"}
 
 
-def time_gpu(w, dtype, steps, warmup, rows):
+def time_gpu(w, dtype, steps, warmup, rows, arith="fast", keep=False):
     import torch
-    from hydroexa_amd import Level, SolverVariant, set_tuning
+    from hydroexa_amd import Level, SolverVariant, set_arith, set_tuning
 
     tdt = torch.float64 if dtype == "f64" else torch.float32
-    if rows:
-        set_tuning(w.solver, tdt, rows, 0)
+    set_tuning(w.solver, tdt, rows, 0)
+    set_arith(arith)
     lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi, dtype=tdt)
     sv = SolverVariant(w.solver)
     sv.run(lv, w.cfl, warmup)
@@ -83,8 +89,23 @@ def time_gpu(w, dtype, steps, warmup, rows):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev_ms = e0.elapsed_time(e1)
+    state = lv.U_new.clone() if keep else None
     del lv
-    return ev_ms, wall, dth
+    set_arith("exact")
+    return ev_ms, wall, dth, state
+
+
+def traffic_from_profile(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (calibrated against a copy of known size), when a summary for
+    this kernel and workload is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    e = d.get(name)
+    return e.get("bytes_per_launch") if isinstance(e, dict) else None
 
 
 def main():
@@ -102,11 +123,20 @@ def main():
     else:
         w = W.lake_at_rest(args.n, args.steps)
     es = 8 if args.dtype == "f64" else 4
-    ev_ms, wall, dth = time_gpu(w, args.dtype, args.steps, args.warmup, args.rows)
+    ev_ms, wall, dth, fin = time_gpu(w, args.dtype, args.steps, args.warmup, args.rows,
+                                     args.arith, keep=True)
     ms_per_step = ev_ms / args.steps
     cups = w.cells * args.steps / (ev_ms / 1e3)
     bytes_per_launch = 8 * es * w.cells          # read h,hu,hv,z,n; write h,hu,hv
     achieved = bytes_per_launch / (ms_per_step / 1e3) / 1e9
+    # the other arithmetic build on the same workload, and the distance
+    # between the two final states
+    other = "exact" if args.arith == "fast" else "fast"
+    ev2, _, _, fin2 = time_gpu(w, args.dtype, args.steps, args.warmup, args.rows, other, keep=True)
+    linf = [float((fin[k] - fin2[k]).abs().max() / fin2[k].abs().max().clamp_min(1e-300))
+            for k in range(3)]
+    kname = f"advance_{w.solver.lower()}_{args.dtype}_{args.arith}_{args.n}"
+    traffic = traffic_from_profile(kname)
     out = {
         "metric": "Mcell-updates/s/GPU + %HBM roofline",
         "value": cups / 1e6,
@@ -121,24 +151,38 @@ def main():
         "dtype": args.dtype,
         "data": "synthetic",
         "config": {"workload": w.name if args.dtype == "f64" else w.name.replace("fp64", "fp32"),
-                   "cells": w.cells, "solver": w.solver, "cfl": w.cfl,
+                   "cells": w.cells, "solver": w.solver, "cfl": w.cfl, "arith": args.arith,
                    "step": "fused advance with device CFL reduction", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kname,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": ms_per_step},
+        "arith_check": {"other": other, "other_ms_per_step": ev2 / args.steps,
+                        "other_Mcups": w.cells * args.steps / (ev2 / 1e3) / 1e6,
+                        "rel_linf_h_hu_hv": linf},
         "wall_s": wall,
         "final_dt": float(dth[-1]) if len(dth) else None,
     }
     if args.extra:
         extra = {}
-        for name, wk, dt_ in (("lake_at_rest_1024_f64", W.lake_at_rest(1024, 500), "f64"),
-                              ("lake_at_rest_1024_f32", W.lake_at_rest(1024, 500), "f32"),
-                              ("dam_break_4096_f32", W.dam_break(4096, 500), "f32")):
-            ms, _, _ = time_gpu(wk, dt_, 500, 20, 0)
+        bumpy = W.dam_break(4096, 1)
+        rng = np.random.default_rng(5)
+        bumpy.z = 0.05 * rng.uniform(size=bumpy.h.shape)
+        bumpy.h = np.maximum(np.where(bumpy.h > 0.75, 1.0, 0.5) - bumpy.z, 0.0)
+        bumpy.n = np.full_like(bumpy.h, 0.03)
+        cases = (("dam_break_4096_bumpy_rough_f64_fast", bumpy, "f64", "fast", 0),
+                 ("dam_break_4096_bumpy_rough_f64_exact", bumpy, "f64", "exact", 0),
+                 ("dam_break_4096_f32_fast", W.dam_break(4096, 1), "f32", "fast", 0),
+                 ("lake_at_rest_1024_f64_fast", W.lake_at_rest(1024, 1), "f64", "fast", 0),
+                 ("lake_at_rest_1024_f64_exact", W.lake_at_rest(1024, 1), "f64", "exact", 0),
+                 ("lake_at_rest_2048_f64_fast", W.lake_at_rest(2048, 1), "f64", "fast", 0),
+                 ("lake_at_rest_1024_f32_fast", W.lake_at_rest(1024, 1), "f32", "fast", 0))
+        for name, wk, dt_, ar, rows in cases:
+            ms, _, _, _ = time_gpu(wk, dt_, 200, 10, rows, ar)
             e = 8 if dt_ == "f64" else 4
-            extra[name] = {"Mcups": wk.cells * 500 / (ms / 1e3) / 1e6, "ms_per_step": ms / 500,
-                           "hbm_frac": 8 * e * wk.cells / (ms / 500 / 1e3) / 1e9 / HBM_PEAK_GBS}
+            extra[name] = {"Mcups": wk.cells * 200 / (ms / 1e3) / 1e6, "ms_per_step": ms / 200,
+                           "hbm_frac": 8 * e * wk.cells / (ms / 200 / 1e3) / 1e9 / HBM_PEAK_GBS}
         out["extra"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W.dam_break(args.n, 1) if args.solver == "Roe" else w,

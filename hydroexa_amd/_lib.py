@@ -13,7 +13,9 @@ import os
 import torch  # noqa: F401  (loads the HIP runtime the library binds to)
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "lib", "libhydroexa.so")
+# HYDROEXA_LIB points at another build of the same library (launch-shape and
+# code-generation experiments); the default is the in-tree build.
+LIB_PATH = os.environ.get("HYDROEXA_LIB") or os.path.join(_PKG, "lib", "libhydroexa.so")
 
 HX_OK = 0
 HX_E_BADARG = 2

@@ -43,13 +43,16 @@ def flags() -> list:
     ]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source and link the in-tree shared library (always)."""
-    os.makedirs(LIBDIR, exist_ok=True)
+def build(verbose: bool = False, libdir: str = LIBDIR, defines=()) -> str:
+    """Compile every HIP source and link the shared library (always).  The
+    default target is the in-tree build; libdir/defines build variants."""
+    os.makedirs(libdir, exist_ok=True)
+    lib = os.path.join(libdir, "libhydroexa.so")
     objs = []
     for src, contract in SOURCES:
-        obj = os.path.join(LIBDIR, os.path.basename(src) + ".o")
-        cmd = [hipcc()] + flags() + [f"-ffp-contract={contract}", "-c", src, "-o", obj]
+        obj = os.path.join(libdir, os.path.basename(src) + ".o")
+        cmd = [hipcc()] + flags() + [f"-D{d}" for d in defines] + \
+              [f"-ffp-contract={contract}", "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         res = subprocess.run(cmd, capture_output=True, text=True)
@@ -57,14 +60,14 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(res.stdout + res.stderr)
             raise RuntimeError(f"hipcc failed on {os.path.basename(src)}")
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("linking libhydroexa.so failed")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -224,18 +224,20 @@ static int hip_status(hipError_t e)
         if (_e != hipSuccess) return HX_E_HIP;     \
     } while (0)
 
+// Launch-shape defaults from the strip-height / occupancy sweep on MI355X
+// (4096^2 plain Roe and 2048^2 hydrostatic workloads): short strips keep
+// enough wavefronts in flight; the FP64 kernels run best when the register
+// allocator leaves room for 3 waves per SIMD.
 static int default_rows(int solver, int dtype)
 {
-    (void)solver;
-    (void)dtype;
-    return 64;
+    if (dtype == HX_F32) return 16;
+    return (solver == HX_SOLVER_ROE) ? 32 : 16;
 }
 
 static int default_occ(int solver, int dtype)
 {
     (void)solver;
-    (void)dtype;
-    return 1;
+    return (dtype == HX_F64) ? 3 : 1;
 }
 
 // 1/d is exact when d is a power of two (mantissa bits all zero).
