@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time the other workloads")
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--ensemble-steps", type=int, default=120,
+                    help="timed steps of the batched ensemble and of the per-scenario loop (0: skip)")
     return ap.parse_args()
 
 
@@ -106,6 +108,65 @@ def traffic_from_profile(name):
         return None
     e = d.get(name)
     return e.get("bytes_per_launch") if isinstance(e, dict) else None
+
+
+def time_ensemble(steps, arith="fast"):
+    """256 dam-break scenarios of uneven size (FP32): the batched ensemble
+    (one launch per step for all scenarios, 6-step HIP graph replayed,
+    autotuned launch shape) against the plain loop that runs each scenario
+    on its own, from the same initial states and for the same steps."""
+    import torch
+    from hydroexa_amd import Ensemble, Level, SolverVariant, set_arith
+    from hydroexa_amd.autotune import tune_ensemble
+    from hydroexa_amd import workloads as W
+
+    set_arith(arith)
+    ws = W.ensemble(256)
+    cells = sum(w.cells for w in ws)
+
+    def make():
+        return [Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi,
+                      dtype=torch.float32) for w in ws]
+
+    levels = make()
+    ens = Ensemble(levels, "Roe")
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        best, _ = tune_ensemble(ens, steps=6, rows=(8, 16, 32), occs=(1,), use_cache=False, stream=st)
+        ens.run(0.45, 6, graph=True, stream=st)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        ens.run(0.45, steps, graph=True, stream=st)
+        e1.record(st)
+    torch.cuda.synchronize()
+    batched_ms = e0.elapsed_time(e1)
+    ens.close()
+    del levels, ens
+    torch.cuda.empty_cache()
+    plain = make()
+    sv = SolverVariant("Roe")
+    for lv in plain[:8]:
+        sv.run(lv, 0.45, 2)             # warm-up of the single-level path
+    plain = make()
+    torch.cuda.synchronize()
+    cur = torch.cuda.current_stream()
+    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    p0.record(cur)
+    for lv in plain:
+        sv.run(lv, 0.45, steps)
+    p1.record(cur)
+    torch.cuda.synchronize()
+    plain_ms = p0.elapsed_time(p1)
+    set_arith("exact")
+    del plain
+    torch.cuda.empty_cache()
+    return {"workload": "ensemble of 256 dam breaks, sides U{128..1024} (seeded), FP32, plain Roe",
+            "cells": cells, "steps": steps, "arith": arith,
+            "tuning": {"rows": best[0], "occupancy": best[1], "persistent": best[2]},
+            "batched_Mcups": cells * steps / (batched_ms / 1e3) / 1e6,
+            "plain_loop_Mcups": cells * steps / (plain_ms / 1e3) / 1e6,
+            "speedup": plain_ms / batched_ms,
+            "batched_ms_per_step": batched_ms / steps, "plain_ms_per_step": plain_ms / steps}
 
 
 def main():
@@ -184,6 +245,8 @@ def main():
             extra[name] = {"Mcups": wk.cells * 200 / (ms / 1e3) / 1e6, "ms_per_step": ms / 200,
                            "hbm_frac": 8 * e * wk.cells / (ms / 200 / 1e3) / 1e9 / HBM_PEAK_GBS}
         out["extra"] = extra
+    if args.ensemble_steps > 0:
+        out["ensemble"] = time_ensemble(args.ensemble_steps, args.arith)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W.dam_break(args.n, 1) if args.solver == "Roe" else w,
                                            args.cpu_seconds)

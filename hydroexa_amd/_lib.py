@@ -39,6 +39,7 @@ EXPORTS = (
     "hx_ctx_create", "hx_ctx_destroy", "hx_abi_version", "hx_status_string",
     "hx_advance_level", "hx_level_dt", "hx_tag_level", "hx_masked_average_down",
     "hx_prune_boxes", "hx_run_level", "hx_set_tuning", "hx_set_arith",
+    "hx_ensemble_create", "hx_ensemble_destroy", "hx_ensemble_set_tuning", "hx_ensemble_run",
 )
 
 
@@ -94,6 +95,10 @@ def load() -> C.CDLL:
     L.hx_run_level.argtypes = [vp, C.c_int, pp, vp, vp, vp, C.c_double, C.c_int, dp, dp, i32p, vp]
     L.hx_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.hx_set_arith.argtypes = [vp, C.c_int]
+    L.hx_ensemble_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, pp, C.POINTER(vp), C.POINTER(vp)]
+    L.hx_ensemble_destroy.argtypes = [vp]
+    L.hx_ensemble_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.hx_ensemble_run.argtypes = [vp, C.c_double, C.c_int, C.c_int, dp, i32p, vp]
     _lib = L
     return L
 

@@ -6,9 +6,11 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "../../include/hydroexa.h"
 #include "hx_types.hpp"
@@ -190,6 +192,43 @@ __global__ void zero_u32_kernel(uint32_t* p, int n)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) p[q] = 0u;
+}
+
+// Ensemble set-up: clear every scenario's CFL ring and error flag, the work
+// queue ring and the step counter (kernels, not memsets, so the same code is
+// safe inside graph capture).
+__global__ void ens_reset_kernel(CflSlot* slots, int nslots, int* err, int nscen,
+                                 unsigned* queue, int* step_base)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nslots) { slots[q].amax = 0ull; slots[q].vmax = 0ull; slots[q].flags = 0u; slots[q].pad = 0u; }
+    if (q < nscen) err[q] = 0;
+    if (q < 3) queue[q] = 0u;
+    if (q == 0) *step_base = 0;
+}
+
+__global__ void ens_tick_kernel(int* step_base, int inc)
+{
+    if (threadIdx.x == 0) *step_base += inc;
+}
+
+// Initial CFL reduction of every scenario (blockIdx.y = scenario) into
+// slot 0 of its ring, read from buffer A.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void ens_cfl_kernel(const ScenDesc<R>* desc)
+{
+    const ScenDesc<R>& d = desc[blockIdx.y];
+    const PatchView<R>& p = d.p;
+    CflAcc<R> acc;
+    acc.init();
+    const int64_t ncell = (int64_t)p.nx * p.ny;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ncell;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q / p.nx), i = (int)(q - (int64_t)j * p.nx);
+        const int64_t s = (int64_t)j * p.pitch + i;
+        acc.add(p.h[s], p.hu[s], p.hv[s], p.z[s], d.dx, d.dy, d.rdx, d.rdy, d.dx_pow2 != 0);
+    }
+    cfl_commit(acc, d.slots);
 }
 
 }  // namespace hx
@@ -629,4 +668,259 @@ extern "C" int hx_run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah,
     }
     if (steps_done) *steps_done = nsteps;
     return HX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensemble
+// ---------------------------------------------------------------------------
+struct hx_ensemble {
+    hx_ctx* ctx;
+    int solver, dtype, nscen;
+    int rows, occ, persistent;
+    int total;
+    std::vector<hx_patch> patches;
+    std::vector<void*> alt;      // 3 per scenario: buffer B
+    void* desc;                  // device ScenDesc<R>[nscen]
+    int* prefix;                 // device [nscen + 1]
+    CflSlot* slots;              // device [3 * nscen]
+    int* err;                    // device [nscen]
+    double* hist;                // device [2 * nscen * cap]
+    int cap;
+    unsigned* queue;             // device [3]
+    int* step_base;              // device [1]
+    int ncu;
+};
+
+template <typename R>
+static int ens_upload(hx_ensemble* e)
+{
+    const int cols = (e->solver == HX_SOLVER_ROE) ? kRoeCols : kHydroCols;
+    std::vector<ScenDesc<R>> d(e->nscen);
+    std::vector<int> pre(e->nscen + 1, 0);
+    // table order: largest scenarios first, so the long strips of big levels
+    // start early and small levels fill the tail of the launch
+    std::vector<int> order(e->nscen);
+    for (int s = 0; s < e->nscen; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (int64_t)e->patches[a].nx * e->patches[a].ny > (int64_t)e->patches[b].nx * e->patches[b].ny;
+    });
+    int total = 0;
+    for (int t = 0; t < e->nscen; ++t) {
+        const int s = order[t];
+        const hx_patch* in = &e->patches[s];
+        ScenDesc<R>& x = d[t];
+        std::memset(&x, 0, sizeof(x));
+        x.p = view_of<R>(in);
+        x.bh = (R*)e->alt[3 * s];
+        x.bhu = (R*)e->alt[3 * s + 1];
+        x.bhv = (R*)e->alt[3 * s + 2];
+        x.dx = (R)in->dx;
+        x.dy = (R)in->dy;
+        x.rdx = (R)(1.0 / in->dx);
+        x.rdy = (R)(1.0 / in->dy);
+        x.dx_pow2 = is_pow2(in->dx) && is_pow2(in->dy) ? 1 : 0;
+        x.nsx = (in->nx + cols - 1) / cols;
+        x.nstrips = x.nsx * ((in->ny + e->rows - 1) / e->rows);
+        x.strip0 = total;
+        x.slots = e->slots + 3 * s;
+        x.err = e->err + s;
+        x.dt_hist = e->hist + (size_t)s * e->cap;
+        x.vel_hist = e->hist + (size_t)e->nscen * e->cap + (size_t)s * e->cap;
+        pre[t] = total;
+        total += x.nstrips;
+    }
+    pre[e->nscen] = total;
+    e->total = total;
+    HX_TRY(hipMemcpy(e->desc, d.data(), sizeof(ScenDesc<R>) * e->nscen, hipMemcpyHostToDevice));
+    HX_TRY(hipMemcpy(e->prefix, pre.data(), sizeof(int) * (e->nscen + 1), hipMemcpyHostToDevice));
+    return HX_OK;
+}
+
+static int ens_upload_any(hx_ensemble* e)
+{
+    return (e->dtype == HX_F64) ? ens_upload<double>(e) : ens_upload<float>(e);
+}
+
+static void ens_free(hx_ensemble* e)
+{
+    if (!e) return;
+    hipFree(e->desc);
+    hipFree(e->prefix);
+    hipFree(e->slots);
+    hipFree(e->err);
+    if (e->hist) hipFree(e->hist);
+    hipFree(e->queue);
+    hipFree(e->step_base);
+    delete e;
+}
+
+extern "C" int hx_ensemble_create(hx_ctx* c, int solver, int dtype, int nscen, const hx_patch* scen,
+                                  void* const* alt, hx_ensemble** out)
+{
+    if (!c) return HX_E_STATE;
+    if (!out || !scen || !alt || nscen < 1) return HX_E_BADARG;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    if (dtype != HX_F64 && dtype != HX_F32) return HX_E_BADARG;
+    *out = nullptr;
+    for (int s = 0; s < nscen; ++s) {
+        const hx_patch* in = &scen[s];
+        int rc = check_patch(in, 0);
+        if (rc) return rc;
+        if (in->dtype != dtype || in->eta) return HX_E_BADARG;
+        for (int d = 0; d < 2; ++d)
+            if (in->bc_lo[d] == HX_BC_GHOST || in->bc_hi[d] == HX_BC_GHOST) return HX_E_BADARG;
+        if (!alt[3 * s] || !alt[3 * s + 1] || !alt[3 * s + 2]) return HX_E_BADARG;
+    }
+    HX_TRY(hipSetDevice(c->device));
+    hx_ensemble* e = new (std::nothrow) hx_ensemble();
+    if (!e) return HX_E_NOMEM;
+    e->ctx = c;
+    e->solver = solver;
+    e->dtype = dtype;
+    e->nscen = nscen;
+    e->rows = c->rows[solver][dtype];
+    e->occ = c->occ[solver][dtype];
+    e->persistent = 0;
+    e->patches.assign(scen, scen + nscen);
+    e->alt.assign(alt, alt + 3 * nscen);
+    const size_t dsz = (dtype == HX_F64) ? sizeof(ScenDesc<double>) : sizeof(ScenDesc<float>);
+    e->cap = 1024;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+        ncu = 256;
+    e->ncu = ncu;
+    if (hipMalloc(&e->desc, dsz * nscen) != hipSuccess ||
+        hipMalloc(&e->prefix, sizeof(int) * (nscen + 1)) != hipSuccess ||
+        hipMalloc(&e->slots, sizeof(CflSlot) * 3 * nscen) != hipSuccess ||
+        hipMalloc(&e->err, sizeof(int) * nscen) != hipSuccess ||
+        hipMalloc(&e->hist, sizeof(double) * 2 * nscen * (size_t)e->cap) != hipSuccess ||
+        hipMalloc(&e->queue, sizeof(unsigned) * 3) != hipSuccess ||
+        hipMalloc(&e->step_base, sizeof(int)) != hipSuccess) {
+        ens_free(e);
+        return HX_E_NOMEM;
+    }
+    int rc = ens_upload_any(e);
+    if (rc) { ens_free(e); return rc; }
+    *out = e;
+    return HX_OK;
+}
+
+extern "C" int hx_ensemble_destroy(hx_ensemble* e)
+{
+    if (!e) return HX_OK;
+    hipSetDevice(e->ctx->device);
+    ens_free(e);
+    return HX_OK;
+}
+
+extern "C" int hx_ensemble_set_tuning(hx_ensemble* e, int rows, int occupancy, int persistent)
+{
+    if (!e) return HX_E_STATE;
+    bool known = (occupancy == 0);
+    for (int v = 0; v < kOccVariants; ++v) known = known || (occupancy == kOcc[v]);
+    if (!known || rows < 0 || rows > 1 << 20 || (persistent != 0 && persistent != 1))
+        return HX_E_BADARG;
+    HX_TRY(hipSetDevice(e->ctx->device));
+    e->rows = rows ? rows : default_rows(e->solver, e->dtype);
+    e->occ = occupancy ? occupancy : default_occ(e->solver, e->dtype);
+    e->persistent = persistent;
+    return ens_upload_any(e);
+}
+
+template <typename R>
+static int ens_step(hx_ensemble* e, double cfl, int parity, int ring, int step_off, hipStream_t st)
+{
+    EnsArgs<R> a{};
+    a.desc = (const ScenDesc<R>*)e->desc;
+    a.strip_start = e->prefix;
+    a.nscen = e->nscen;
+    a.total = e->total;
+    a.rows = e->rows;
+    a.parity = parity;
+    a.ring = ring;
+    a.step_base = e->step_base;
+    a.step_off = step_off;
+    a.cfl = cfl;
+    a.queue = e->queue;
+    a.persistent = e->persistent;
+    const int grid = e->persistent ? std::max(1, e->ncu * e->occ) : (e->total + kWaves - 1) / kWaves;
+    const int rc = (e->ctx->arith == HX_ARITH_FAST)
+                       ? fast::launch_ensemble<R>(e->solver, e->occ, a, grid, st)
+                       : exact::launch_ensemble<R>(e->solver, e->occ, a, grid, st);
+    return rc ? HX_E_HIP : HX_OK;
+}
+
+template <typename R>
+static int ens_run_t(hx_ensemble* e, double cfl, int nsteps, int use_graph, hipStream_t st)
+{
+    const int nsl = 3 * e->nscen;
+    const int nthr = std::max(nsl, std::max(e->nscen, 3));
+    hipLaunchKernelGGL(ens_reset_kernel, dim3((nthr + 255) / 256), dim3(256), 0, st, e->slots, nsl,
+                       e->err, e->nscen, e->queue, e->step_base);
+    hipLaunchKernelGGL(ens_cfl_kernel<R>, dim3(64, e->nscen), dim3(kBlock), 0, st,
+                       (const ScenDesc<R>*)e->desc);
+    HX_TRY(hipGetLastError());
+    int done = 0;
+    const int cycles = (use_graph && st != nullptr) ? nsteps / 6 : 0;
+    if (cycles > 0) {
+        // one 6-step cycle (2 state buffers x 3 CFL slots) captured once and
+        // replayed; the tick node advances the device step counter
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HX_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = HX_OK;
+        for (int k = 0; k < 6 && rc == HX_OK; ++k) rc = ens_step<R>(e, cfl, k & 1, k % 3, k, st);
+        if (rc == HX_OK) hipLaunchKernelGGL(ens_tick_kernel, dim3(1), dim3(64), 0, st, e->step_base, 6);
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (rc != HX_OK || ce != hipSuccess) { if (g) hipGraphDestroy(g); return HX_E_HIP; }
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) { hipGraphDestroy(g); return HX_E_HIP; }
+        hipError_t le = hipSuccess;
+        for (int c = 0; c < cycles && le == hipSuccess; ++c) le = hipGraphLaunch(ge, st);
+        hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        if (le != hipSuccess) return HX_E_HIP;
+        done = 6 * cycles;
+    }
+    for (int j = 0; done + j < nsteps; ++j) {
+        const int rc = ens_step<R>(e, cfl, j & 1, j % 3, (cycles > 0) ? j : done + j, st);
+        if (rc) return rc;
+    }
+    return HX_OK;
+}
+
+extern "C" int hx_ensemble_run(hx_ensemble* e, double cfl, int nsteps, int use_graph,
+                               double* dt_hist, int32_t* steps_done, void* stream)
+{
+    if (!e) return HX_E_STATE;
+    if (nsteps < 0 || !(cfl > 0.0)) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(e->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    if (nsteps > e->cap) {
+        int cap = e->cap;
+        while (cap < nsteps) cap *= 2;
+        HX_TRY(hipStreamSynchronize(st));
+        hipFree(e->hist);
+        e->hist = nullptr;
+        if (hipMalloc(&e->hist, sizeof(double) * 2 * e->nscen * (size_t)cap) != hipSuccess) return HX_E_NOMEM;
+        e->cap = cap;
+        int rc = ens_upload_any(e);
+        if (rc) return rc;
+    }
+    if (nsteps == 0) return HX_OK;
+    int rc = (e->dtype == HX_F64) ? ens_run_t<double>(e, cfl, nsteps, use_graph, st)
+                                  : ens_run_t<float>(e, cfl, nsteps, use_graph, st);
+    if (rc) return rc;
+    std::vector<int> err(e->nscen);
+    HX_TRY(hipMemcpyAsync(err.data(), e->err, sizeof(int) * e->nscen, hipMemcpyDeviceToHost, st));
+    if (dt_hist)
+        for (int s = 0; s < e->nscen; ++s)
+            HX_TRY(hipMemcpyAsync(dt_hist + (size_t)s * nsteps, e->hist + (size_t)s * e->cap,
+                                  sizeof(double) * nsteps, hipMemcpyDeviceToHost, st));
+    HX_TRY(hipStreamSynchronize(st));
+    bool any = false;
+    for (int s = 0; s < e->nscen; ++s) {
+        if (steps_done) steps_done[s] = err[s] ? err[s] - 1 : nsteps;
+        any = any || err[s];
+    }
+    return any ? HX_E_DT_COLLAPSE : HX_OK;
 }

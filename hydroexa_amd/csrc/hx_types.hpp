@@ -94,3 +94,51 @@ int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
 }
 
 }  // namespace hx
+
+namespace hx {
+
+// One scenario of a batched ensemble (device-resident descriptor table).
+// Buffers A and B alternate as input and output: even steps read A.
+template <typename R>
+struct ScenDesc {
+    PatchView<R> p;        // geometry, BCs, terrain, and buffer A in h/hu/hv
+    R* bh;                 // buffer B
+    R* bhu;
+    R* bhv;
+    R dx, dy, rdx, rdy;
+    int dx_pow2;
+    int nsx;               // strips across x
+    int strip0;            // first global strip index of this scenario
+    int nstrips;
+    CflSlot* slots;        // 3-slot ring of this scenario
+    int* err;              // dt-collapse flag (step + 1), 0 if none
+    double* dt_hist;       // [hist_cap]
+    double* vel_hist;      // [hist_cap]
+};
+
+template <typename R>
+struct EnsArgs {
+    const ScenDesc<R>* desc;
+    const int* strip_start;   // [nscen + 1] prefix of nstrips
+    int nscen;
+    int total;                // total strips
+    int rows;
+    int parity;               // 0: A -> B, 1: B -> A
+    int ring;                 // step index modulo 3 (slot ring position)
+    const int* step_base;     // device step counter (graph cycles advance it)
+    int step_off;             // step = *step_base + step_off
+    double cfl;
+    unsigned* queue;          // 3-counter ring of the persistent work queue
+    int persistent;           // 0: one wavefront per strip; 1: persistent waves pull strips
+};
+
+namespace exact {
+template <typename R>
+int launch_ensemble(int solver, int occ, const EnsArgs<R>& e, int grid, hipStream_t st);
+}
+namespace fast {
+template <typename R>
+int launch_ensemble(int solver, int occ, const EnsArgs<R>& e, int grid, hipStream_t st);
+}
+
+}  // namespace hx

@@ -118,3 +118,31 @@ def boxes_of(nx: int, ny: int, box: int) -> np.ndarray:
         for i0 in range(0, nx, box):
             out.append((i0, j0, min(i0 + box, nx) - 1, min(j0 + box, ny) - 1))
     return np.array(out, dtype=np.int32)
+
+
+def ensemble(nscen: int = 256, side_min: int = 128, side_max: int = 1024, seed: int = SEED,
+             steps: int = 1000, cfl: float = 0.45, solver: str = "Roe"):
+    """Batched dam-break ensemble: side ~ U{side_min..side_max} per scenario
+    (seeded), dam at x_d ~ U[0.25, 0.75], h_L = 1, h_R = r h_L with
+    r ~ U[0.1, 0.9], bed A exp(-|x - c|^2 / (2 s^2)) with A ~ U[0, 0.2],
+    s ~ U[0.05, 0.15], c ~ U[0.3, 0.7]^2, no friction, transmissive sides.
+    The free surface is set, h = max(eta - z, 0)."""
+    u = splitmix64_uniform(seed + 7, 8 * nscen).reshape(nscen, 8)
+    out = []
+    for s in range(nscen):
+        n = side_min + int(u[s, 0] * (side_max - side_min + 1))
+        n = min(n, side_max)
+        xd = 0.25 + 0.5 * u[s, 1]
+        r = 0.1 + 0.8 * u[s, 2]
+        A = 0.2 * u[s, 3]
+        sg = 0.05 + 0.1 * u[s, 4]
+        cx, cy = 0.3 + 0.4 * u[s, 5], 0.3 + 0.4 * u[s, 6]
+        X, Y, dx, dy = _centres(n, n)
+        z = A * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * sg * sg))
+        eta = np.where(X < xd, 1.0, r)
+        h = np.maximum(eta - z, 0.0)
+        zero = np.zeros_like(h)
+        out.append(Workload(f"ensemble_{s}_{n}x{n}", solver, h, zero.copy(), zero.copy(), z,
+                            zero.copy(), dx, dy, (1, 1), (1, 1), cfl, steps,
+                            meta={"dam": xd, "ratio": r, "bump": (A, sg, cx, cy)}))
+    return out

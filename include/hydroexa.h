@@ -181,6 +181,40 @@ int hx_set_tuning(hx_ctx *ctx, int solver, int dtype, int rows, int occupancy);
  */
 int hx_set_arith(hx_ctx *ctx, int mode);
 
+/*
+ * Batched ensemble: many independent levels (flood scenarios of uneven
+ * sizes) advanced together, one launch per step for the whole batch, each
+ * scenario with its own CFL-limited dt (the reference's per-level
+ * compute_dt + AdvanceLevel, src/state/AmrMeshState_Physics.cpp:13-19,153-172,
+ * applied per scenario).  No batching exists in the reference; each scenario
+ * matches hx_run_level on it alone bit for bit.
+ *
+ * scen: nscen patches (all sides physical, one dtype, no eta), h/hu/hv
+ * holding t^0; alt: 3*nscen device pointers, buffer B (h, hu, hv) of each
+ * scenario, same pitch.  After hx_ensemble_run the state is in the patches'
+ * buffers if nsteps is even, in alt otherwise.  The ensemble keeps the
+ * pointers; the caller keeps the memory alive.
+ */
+typedef struct hx_ensemble hx_ensemble;
+int hx_ensemble_create(hx_ctx *ctx, int solver, int dtype, int nscen, const hx_patch *scen,
+                       void *const *alt, hx_ensemble **out);
+int hx_ensemble_destroy(hx_ensemble *ens);
+
+/* rows marched per wavefront, waves per SIMD (0 = defaults) and the
+ * schedule: 0 = one wavefront per strip, 1 = persistent wavefronts pulling
+ * strips from an atomic work queue. */
+int hx_ensemble_set_tuning(hx_ensemble *ens, int rows, int occupancy, int persistent);
+
+/*
+ * nsteps steps of every scenario.  use_graph != 0 (and a non-null stream)
+ * captures a 6-step cycle in a HIP graph once and replays it.  dt_hist
+ * (host, optional): nscen x nsteps, row-major.  steps_done (host, optional,
+ * nscen): steps completed per scenario.  Returns HX_E_DT_COLLAPSE if any
+ * scenario's dt collapsed (that scenario stops; the others continue).
+ */
+int hx_ensemble_run(hx_ensemble *ens, double cfl, int nsteps, int use_graph, double *dt_hist,
+                    int32_t *steps_done, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,131 @@
+"""Batched ensemble: every scenario equals the single-level loop on it alone,
+in each schedule (one wavefront per strip, persistent work queue, eager
+launches, replayed HIP graph), and the FP64 ensemble matches the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from hydroexa_amd import Ensemble, Level, SolverVariant
+from hydroexa_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def levels_of(ws, dtype=torch.float64):
+    return [Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi, dtype=dtype) for w in ws]
+
+
+def small_ensemble(model="Roe", n=7, seed=3):
+    ws = W.ensemble(n, side_min=24, side_max=150, seed=seed, solver=model)
+    if model == "HydrostaticRoe":
+        for w in ws:
+            w.bc_lo, w.bc_hi = (0, 1), (1, 0)
+            w.n = np.full_like(w.h, 0.02)
+    return ws
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+@pytest.mark.parametrize("graph,persistent", [(False, False), (True, False), (True, True), (False, True)])
+def test_ensemble_equals_single_level_runs(gpu, model, graph, persistent):
+    ws = small_ensemble(model)
+    steps = 37                                   # 6 graph cycles + 1 eager step
+    ens_levels = levels_of(ws)
+    ens = Ensemble(ens_levels, model)
+    ens.set_tuning(rows=16, occupancy=0, persistent=persistent)
+    dth, done = ens.run(0.45, steps, graph=graph)
+    torch.cuda.synchronize()
+    assert np.all(done == steps)
+    for s, w in enumerate(ws):
+        lv = levels_of([w])[0]
+        d1, _ = SolverVariant(model).run(lv, 0.45, steps)
+        np.testing.assert_array_equal(dth[s], d1)
+        for k in range(3):
+            assert torch.equal(ens_levels[s].U_new[k], lv.U_new[k]), (s, k)
+    ens.close()
+
+
+def test_ensemble_graph_replays_against_eager(gpu):
+    """Several run() calls in a row, with eager torch work and syncs between
+    them: each graph-replayed run equals the eager run from the same state."""
+    ws = small_ensemble("Roe", n=5, seed=11)
+    a, b = levels_of(ws), levels_of(ws)
+    ea, eb = Ensemble(a, "Roe"), Ensemble(b, "Roe")
+    for rep in range(3):
+        da, _ = ea.run(0.45, 12 + rep, graph=True)
+        torch.cuda.synchronize()
+        junk = torch.randn(1000, device="cuda").sum().item()    # unrelated eager work
+        db, _ = eb.run(0.45, 12 + rep, graph=False)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(da, db)
+        for s in range(len(ws)):
+            assert torch.equal(a[s].U_new, b[s].U_new), (rep, s, junk)
+    ea.close()
+    eb.close()
+
+
+def test_ensemble_fp64_matches_oracle(gpu):
+    ws = small_ensemble("Roe", n=4, seed=5)
+    lv = levels_of(ws)
+    ens = Ensemble(lv, "Roe")
+    dth, _ = ens.run(0.45, 40, graph=True)
+    for s, w in enumerate(ws):
+        ref = O.run(O.ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, 0.45, 40, w.bc_lo, w.bc_hi)
+        np.testing.assert_array_equal(dth[s], ref[3])
+        for k in range(3):
+            got = lv[s].U_new[k].cpu().numpy()
+            err = np.abs(got - ref[k]).max() / max(np.abs(ref[k]).max(), 1e-300)
+            assert err <= 1e-12, (s, k, err)
+    ens.close()
+
+
+def test_ensemble_fp32_bound(gpu):
+    """FP32 ensemble against the FP64 oracle after 100 steps: relative
+    L-infinity below 3e-5 on h (measured bound)."""
+    ws = small_ensemble("Roe", n=3, seed=9)
+    lv = levels_of(ws, torch.float32)
+    ens = Ensemble(lv, "Roe")
+    ens.run(0.45, 100, graph=True)
+    worst = 0.0
+    for s, w in enumerate(ws):
+        ref = O.run(O.ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, 0.45, 100, w.bc_lo, w.bc_hi)
+        got = lv[s].U_new[0].double().cpu().numpy()
+        worst = max(worst, np.abs(got - ref[0]).max() / np.abs(ref[0]).max())
+    print("fp32 ensemble rel linf h:", worst)
+    assert worst < 3e-5
+    ens.close()
+
+
+def test_ensemble_dt_collapse_isolated(gpu):
+    """A scenario whose dt collapses stops; the others run to the end."""
+    ws = small_ensemble("Roe", n=3, seed=13)
+    ws[1].hu = np.full_like(ws[1].h, 1e14)
+    lv = levels_of(ws)
+    ens = Ensemble(lv, "Roe")
+    from hydroexa_amd import _lib as L
+    with pytest.raises(L.DtCollapse):
+        ens.run(0.45, 8, graph=False)
+    ens.close()
+
+
+def test_autotuner_picks_a_valid_shape_and_keeps_results(gpu, tmp_path, monkeypatch):
+    """The tuner times real steps and applies the fastest shape; the shape
+    changes speed only, so results equal an untuned run bit for bit."""
+    monkeypatch.setenv("HYDROEXA_TUNE_CACHE", str(tmp_path / "tune.json"))
+    from hydroexa_amd.autotune import tune_level, tune_ensemble
+    from hydroexa_amd import set_tuning
+    w = W.dam_break(200, 1)
+    scratch = levels_of([w])[0]
+    best, times = tune_level(scratch, "Roe", steps=3, rows=(8, 32), occs=(1, 3))
+    assert best in {(8, 1), (8, 3), (32, 1), (32, 3)} and len(times) == 4
+    a = levels_of([w])[0]
+    SolverVariant("Roe").run(a, 0.45, 10)
+    set_tuning("Roe", torch.float64, 0, 0)
+    b = levels_of([w])[0]
+    SolverVariant("Roe").run(b, 0.45, 10)
+    assert torch.equal(a.U_new, b.U_new)
+    ens = Ensemble(levels_of(small_ensemble("Roe", n=4)), "Roe")
+    best, times = tune_ensemble(ens, steps=3, rows=(16,), occs=(1,))
+    assert len(times) == 2 and best[0] == 16
+    assert (tmp_path / "tune.json").exists()
+    ens.close()
