@@ -19,7 +19,8 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libhydroexa.so")
 # (source, FP contraction policy): the exact build must not contract.
 SOURCES = [(os.path.join(CSRC, "hx_kernels.hip"), "off"),
-           (os.path.join(CSRC, "hx_fast.hip"), "fast")]
+           (os.path.join(CSRC, "hx_fast.hip"), "fast"),
+           (os.path.join(CSRC, "hx_adjoint.hip"), "off")]
 ARCH = os.environ.get("HYDROEXA_ARCH", "gfx950")
 
 

@@ -1,0 +1,754 @@
+// hx_adjoint.hip -- adjoint (reverse-mode) of the per-level step with
+// checkpointing, FP64.  No reference counterpart: HydroEXA has no
+// derivatives.  The forward trajectory is the library's own fused advance;
+// the adjoint linearises each step around the state the forward produced,
+// with the dt schedule of that forward held fixed.
+//
+// One adjoint step, lambda^{n+1} -> lambda^n (plus the roughness and bed
+// gradients accumulated), is three kernels, all deterministic (no atomics):
+//   post    (hydrostatic only) per cell: re-evaluate the cell's four faces,
+//           the flux update, and the reverse of Manning friction and the
+//           dry clamp -> lambda_u (adjoint of the pre-friction state);
+//   face    per face: the face's vector-Jacobian product with respect to
+//           its stencil cells (2 for plain Roe, 4 for the hydrostatic
+//           face) -> one record per face;
+//   gather  per cell: sums the records of every face whose stencil holds
+//           the cell or a ghost filled from it, then applies the ghost-fill
+//           and rebuild chain rules (eta = h + z, h_rebuilt =
+//           max(0, eta - z), mirror / edge sources, odd momenta).
+// Compiled with -ffp-contract=off like the exact advance.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "../../include/hydroexa.h"
+#include "hx_adjoint.hpp"
+#include "hx_host.hpp"
+#include "hx_types.hpp"
+
+namespace hx {
+namespace adj {
+
+constexpr int kThreads = 256;
+
+struct Geo {
+    int nx, ny, bl0, bl1, bh0, bh1;
+};
+
+// One state (dense planes, row pitch nx) with its terrain.
+struct State {
+    const double* h;
+    const double* hu;
+    const double* hv;
+    const double* z;
+    const double* n;
+};
+
+// Physical-BC source of a padded index (include/boundaries/BCFill.H:49-82,
+// terrain foextrap per src/state/AmrMeshState_Init.cpp:62-70): src for the
+// state, edge for the terrain, flip for odd (normal) momenta.
+__device__ __forceinline__ void map1(int i, int n, int blo, int bhi, int& src, int& edge,
+                                     bool& flip)
+{
+    src = i;
+    edge = i;
+    flip = false;
+    if (i < 0) {
+        edge = 0;
+        flip = (blo == 0);
+        src = flip ? (-1 - i) : 0;
+    } else if (i >= n) {
+        edge = n - 1;
+        flip = (bhi == 0);
+        src = flip ? (2 * n - 1 - i) : n - 1;
+    }
+}
+
+// Padded cell as the forward step sees it; x = eta - z before the max.
+struct PC {
+    double he, hu, hv, z, n, x;
+};
+
+__device__ __forceinline__ PC pcell(const State& s, const Geo& g, int i, int j)
+{
+    int si, ei, sj, ej;
+    bool fx, fy;
+    map1(i, g.nx, g.bl0, g.bh0, si, ei, fx);
+    map1(j, g.ny, g.bl1, g.bh1, sj, ej, fy);
+    const int64_t a = (int64_t)sj * g.nx + si, b = (int64_t)ej * g.nx + ei;
+    PC c;
+    c.z = s.z[b];
+    c.n = s.n[b];
+    const double eta = s.h[a] + s.z[a];
+    c.x = eta - c.z;
+    c.he = (0.0 < c.x) ? c.x : 0.0;
+    const double qx = s.hu[a], qy = s.hv[a];
+    c.hu = fx ? -qx : qx;
+    c.hv = fy ? -qy : qy;
+    return c;
+}
+
+__device__ __forceinline__ void put4(double* c, const PC& p)
+{
+    c[0] = p.he;
+    c[1] = p.hu;
+    c[2] = p.hv;
+    c[3] = p.z;
+}
+
+// ---------------------------------------------------------------------------
+// Hydrostatic step (src/solvers/HydrostaticRoe.cpp:400-606)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void hpost_kernel(State s, Geo g, double dt, double dx,
+                                                         double dy, const double* __restrict__ lam,
+                                                         double* __restrict__ lun,
+                                                         double* __restrict__ gn)
+{
+    const int64_t N = (int64_t)g.nx * g.ny;
+    const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (c >= N) return;
+    const int j = (int)(c / g.nx), i = (int)(c - (int64_t)j * g.nx);
+    double st[20];
+    double FRx[3], FLx[3], FRy[3], FLy[3], dummy[3];
+    for (int q = 0; q < 5; ++q) put4(st + 4 * q, pcell(s, g, i - 2 + q, j));
+    hydro_face_fwd(st, 1.0, 0.0, dummy, FRx);        // face i: FR enters cell i
+    hydro_face_fwd(st + 4, 1.0, 0.0, FLx, dummy);    // face i+1: FL enters cell i
+    for (int q = 0; q < 5; ++q) put4(st + 4 * q, pcell(s, g, i, j - 2 + q));
+    hydro_face_fwd(st, 0.0, 1.0, dummy, FRy);
+    hydro_face_fwd(st + 4, 0.0, 1.0, FLy, dummy);
+    const double uo[3] = {s.h[c], s.hu[c], s.hv[c]};
+    double un[3], ob[3], unb[3];
+    for (int k = 0; k < 3; ++k) {
+        un[k] = uo[k] - (dt / dx) * (FLx[k] - FRx[k]) - (dt / dy) * (FLy[k] - FRy[k]);
+        ob[k] = lam[k * N + c];
+    }
+    const double dn = hydro_post_vjp(un, s.n[c], dt, ob, unb);
+    for (int k = 0; k < 3; ++k) lun[k * N + c] = unb[k];
+    gn[c] += dn;
+}
+
+// Face records: 16 planes (stencil slot q = 0..3 for Lm, L, R, Rp; field
+// m = h, hu, hv, z), plane q*4+m, one value per face.
+template <int DIR>
+__global__ __launch_bounds__(kThreads) void hface_kernel(State s, Geo g, double dt, double ds,
+                                                         const double* __restrict__ lun,
+                                                         double* __restrict__ fb)
+{
+    const int64_t N = (int64_t)g.nx * g.ny;
+    const int64_t nf = DIR == 0 ? (int64_t)(g.nx + 1) * g.ny : (int64_t)g.nx * (g.ny + 1);
+    const int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (idx >= nf) return;
+    int f, i, j, n;
+    if (DIR == 0) {
+        j = (int)(idx / (g.nx + 1));
+        f = (int)(idx - (int64_t)j * (g.nx + 1));
+        i = f;
+        n = g.nx;
+    } else {
+        f = (int)(idx / g.nx);
+        i = (int)(idx - (int64_t)f * g.nx);
+        j = f;
+        n = g.ny;
+    }
+    double c[16], cb[16];
+    for (int q = 0; q < 4; ++q)
+        put4(c + 4 * q, DIR == 0 ? pcell(s, g, i - 2 + q, j) : pcell(s, g, i, j - 2 + q));
+    const double r = dt / ds;
+    double mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0};
+    const int64_t cl = DIR == 0 ? (int64_t)j * g.nx + (f - 1) : (int64_t)(f - 1) * g.nx + i;
+    const int64_t cr = DIR == 0 ? (int64_t)j * g.nx + f : (int64_t)f * g.nx + i;
+    if (f >= 1)
+        for (int k = 0; k < 3; ++k) mL[k] = -r * lun[k * N + cl];
+    if (f < n)
+        for (int k = 0; k < 3; ++k) mR[k] = r * lun[k * N + cr];
+    hydro_face_vjp(c, DIR == 0 ? 1.0 : 0.0, DIR == 0 ? 0.0 : 1.0, mL, mR, cb);
+    for (int m = 0; m < 16; ++m) fb[m * nf + idx] = cb[m];
+}
+
+// Sum of the records that land on padded position (px, j) from x faces.
+__device__ __forceinline__ double hsum_x(const double* fb, const Geo& g, int px, int j, int m)
+{
+    const int64_t nf = (int64_t)(g.nx + 1) * g.ny;
+    double acc = 0.0;
+    for (int f = px + 2; f >= px - 1; --f) {
+        if (f < 0 || f > g.nx) continue;
+        const int q = px - f + 2;
+        acc += fb[(q * 4 + m) * nf + (int64_t)j * (g.nx + 1) + f];
+    }
+    return acc;
+}
+
+__device__ __forceinline__ double hsum_y(const double* fb, const Geo& g, int i, int py, int m)
+{
+    const int64_t nf = (int64_t)g.nx * (g.ny + 1);
+    double acc = 0.0;
+    for (int f = py + 2; f >= py - 1; --f) {
+        if (f < 0 || f > g.ny) continue;
+        const int q = py - f + 2;
+        acc += fb[(q * 4 + m) * nf + (int64_t)f * g.nx + i];
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(kThreads) void hgather_kernel(State s, Geo g,
+                                                           const double* __restrict__ lun,
+                                                           const double* __restrict__ fbx,
+                                                           const double* __restrict__ fby,
+                                                           double* __restrict__ lam_new,
+                                                           double* __restrict__ gz)
+{
+    const int64_t N = (int64_t)g.nx * g.ny;
+    const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (c >= N) return;
+    const int j = (int)(c / g.nx), i = (int)(c - (int64_t)j * g.nx);
+    // the update starts from the stored state
+    double l0 = lun[c], l1 = lun[N + c], l2 = lun[2 * N + c];
+    double eb = 0.0, zb = 0.0;
+    {
+        const PC p = pcell(s, g, i, j);
+        const double he = hsum_x(fbx, g, i, j, 0) + hsum_y(fby, g, i, j, 0);
+        l1 += hsum_x(fbx, g, i, j, 1) + hsum_y(fby, g, i, j, 1);
+        l2 += hsum_x(fbx, g, i, j, 2) + hsum_y(fby, g, i, j, 2);
+        zb += hsum_x(fbx, g, i, j, 3) + hsum_y(fby, g, i, j, 3);
+        if (0.0 < p.x) { eb += he; zb -= he; }
+    }
+    if (i <= 1 || i >= g.nx - 2) {
+        const int gx[4] = {-2, -1, g.nx, g.nx + 1};
+        for (int t = 0; t < 4; ++t) {
+            int src, edge;
+            bool flip;
+            map1(gx[t], g.nx, g.bl0, g.bh0, src, edge, flip);
+            if (src != i && edge != i) continue;
+            const PC p = pcell(s, g, gx[t], j);
+            const double he = hsum_x(fbx, g, gx[t], j, 0);
+            if (src == i) {
+                const double hu = hsum_x(fbx, g, gx[t], j, 1);
+                l1 += flip ? -hu : hu;
+                l2 += hsum_x(fbx, g, gx[t], j, 2);
+                if (0.0 < p.x) eb += he;
+            }
+            if (edge == i) {
+                zb += hsum_x(fbx, g, gx[t], j, 3);
+                if (0.0 < p.x) zb -= he;
+            }
+        }
+    }
+    if (j <= 1 || j >= g.ny - 2) {
+        const int gy[4] = {-2, -1, g.ny, g.ny + 1};
+        for (int t = 0; t < 4; ++t) {
+            int src, edge;
+            bool flip;
+            map1(gy[t], g.ny, g.bl1, g.bh1, src, edge, flip);
+            if (src != j && edge != j) continue;
+            const PC p = pcell(s, g, i, gy[t]);
+            const double he = hsum_y(fby, g, i, gy[t], 0);
+            if (src == j) {
+                l1 += hsum_y(fby, g, i, gy[t], 1);
+                const double hv = hsum_y(fby, g, i, gy[t], 2);
+                l2 += flip ? -hv : hv;
+                if (0.0 < p.x) eb += he;
+            }
+            if (edge == j) {
+                zb += hsum_y(fby, g, i, gy[t], 3);
+                if (0.0 < p.x) zb -= he;
+            }
+        }
+    }
+    lam_new[c] = l0 + eb;         // eta = h + z
+    lam_new[N + c] = l1;
+    lam_new[2 * N + c] = l2;
+    gz[c] += eb + zb;
+}
+
+// ---------------------------------------------------------------------------
+// Plain Roe step (src/solvers/Roe.cpp:250-434)
+// Face records: 10 planes, slot q = 0 (left cell), 1 (right cell), field
+// m = h, hu, hv, z, n: plane q*5+m.
+// ---------------------------------------------------------------------------
+template <int DIR>
+__global__ __launch_bounds__(kThreads) void rface_kernel(State s, Geo g, double dt, double ds,
+                                                         const double* __restrict__ lam,
+                                                         double* __restrict__ fb)
+{
+    const int64_t N = (int64_t)g.nx * g.ny;
+    const int64_t nf = DIR == 0 ? (int64_t)(g.nx + 1) * g.ny : (int64_t)g.nx * (g.ny + 1);
+    const int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (idx >= nf) return;
+    int f, i, j, n;
+    if (DIR == 0) {
+        j = (int)(idx / (g.nx + 1));
+        f = (int)(idx - (int64_t)j * (g.nx + 1));
+        i = f;
+        n = g.nx;
+    } else {
+        f = (int)(idx / g.nx);
+        i = (int)(idx - (int64_t)f * g.nx);
+        j = f;
+        n = g.ny;
+    }
+    const PC a = DIR == 0 ? pcell(s, g, f - 1, j) : pcell(s, g, i, f - 1);
+    const PC b = DIR == 0 ? pcell(s, g, f, j) : pcell(s, g, i, f);
+    const double L[5] = {a.he, a.hu, a.hv, a.z, a.n};
+    const double R[5] = {b.he, b.hu, b.hv, b.z, b.n};
+    const double r = dt / ds;
+    const int64_t cl = DIR == 0 ? (int64_t)j * g.nx + (f - 1) : (int64_t)(f - 1) * g.nx + i;
+    const int64_t cr = DIR == 0 ? (int64_t)j * g.nx + f : (int64_t)f * g.nx + i;
+    double lL[3] = {0.0, 0.0, 0.0}, lR[3] = {0.0, 0.0, 0.0};
+    if (f >= 1)
+        for (int k = 0; k < 3; ++k) lL[k] = lam[k * N + cl];
+    if (f < n)
+        for (int k = 0; k < 3; ++k) lR[k] = lam[k * N + cr];
+    // un(c) = uo(c) - r (F[xr] - F[xl]) - r (Sp[xr] + Sm[xl])
+    double Fb[3], Smb[3], Spb[3], Lb[5], Rb[5];
+    for (int k = 0; k < 3; ++k) {
+        Fb[k] = r * lR[k] - r * lL[k];
+        Spb[k] = -r * lL[k];
+        Smb[k] = -r * lR[k];
+    }
+    roe_face_vjp(L, R, ds, DIR == 0 ? 1.0 : 0.0, DIR == 0 ? 0.0 : 1.0, Fb, Smb, Spb, Lb, Rb);
+    for (int m = 0; m < 5; ++m) {
+        fb[m * nf + idx] = Lb[m];
+        fb[(5 + m) * nf + idx] = Rb[m];
+    }
+}
+
+__device__ __forceinline__ double rsum_x(const double* fb, const Geo& g, int px, int j, int m)
+{
+    const int64_t nf = (int64_t)(g.nx + 1) * g.ny;
+    double acc = 0.0;
+    for (int f = px + 1; f >= px; --f) {
+        if (f < 0 || f > g.nx) continue;
+        const int q = px - f + 1;
+        acc += fb[(q * 5 + m) * nf + (int64_t)j * (g.nx + 1) + f];
+    }
+    return acc;
+}
+
+__device__ __forceinline__ double rsum_y(const double* fb, const Geo& g, int i, int py, int m)
+{
+    const int64_t nf = (int64_t)g.nx * (g.ny + 1);
+    double acc = 0.0;
+    for (int f = py + 1; f >= py; --f) {
+        if (f < 0 || f > g.ny) continue;
+        const int q = py - f + 1;
+        acc += fb[(q * 5 + m) * nf + (int64_t)f * g.nx + i];
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(kThreads) void rgather_kernel(State s, Geo g,
+                                                           const double* __restrict__ lam,
+                                                           const double* __restrict__ fbx,
+                                                           const double* __restrict__ fby,
+                                                           double* __restrict__ lam_new,
+                                                           double* __restrict__ gz,
+                                                           double* __restrict__ gn)
+{
+    const int64_t N = (int64_t)g.nx * g.ny;
+    const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (c >= N) return;
+    const int j = (int)(c / g.nx), i = (int)(c - (int64_t)j * g.nx);
+    // the update starts from the rebuilt depth and the stored momenta
+    double l1 = lam[N + c], l2 = lam[2 * N + c];
+    double eb = 0.0, zb = 0.0, nb = 0.0;
+    {
+        const PC p = pcell(s, g, i, j);
+        const double he = lam[c] + rsum_x(fbx, g, i, j, 0) + rsum_y(fby, g, i, j, 0);
+        l1 += rsum_x(fbx, g, i, j, 1) + rsum_y(fby, g, i, j, 1);
+        l2 += rsum_x(fbx, g, i, j, 2) + rsum_y(fby, g, i, j, 2);
+        zb += rsum_x(fbx, g, i, j, 3) + rsum_y(fby, g, i, j, 3);
+        nb += rsum_x(fbx, g, i, j, 4) + rsum_y(fby, g, i, j, 4);
+        if (0.0 < p.x) { eb += he; zb -= he; }
+    }
+    if (i == 0 || i == g.nx - 1) {
+        const int gx[2] = {-1, g.nx};
+        for (int t = 0; t < 2; ++t) {
+            int src, edge;
+            bool flip;
+            map1(gx[t], g.nx, g.bl0, g.bh0, src, edge, flip);
+            if (src != i && edge != i) continue;
+            const PC p = pcell(s, g, gx[t], j);
+            const double he = rsum_x(fbx, g, gx[t], j, 0);
+            if (src == i) {
+                const double hu = rsum_x(fbx, g, gx[t], j, 1);
+                l1 += flip ? -hu : hu;
+                l2 += rsum_x(fbx, g, gx[t], j, 2);
+                if (0.0 < p.x) eb += he;
+            }
+            if (edge == i) {
+                zb += rsum_x(fbx, g, gx[t], j, 3);
+                nb += rsum_x(fbx, g, gx[t], j, 4);
+                if (0.0 < p.x) zb -= he;
+            }
+        }
+    }
+    if (j == 0 || j == g.ny - 1) {
+        const int gy[2] = {-1, g.ny};
+        for (int t = 0; t < 2; ++t) {
+            int src, edge;
+            bool flip;
+            map1(gy[t], g.ny, g.bl1, g.bh1, src, edge, flip);
+            if (src != j && edge != j) continue;
+            const PC p = pcell(s, g, i, gy[t]);
+            const double he = rsum_y(fby, g, i, gy[t], 0);
+            if (src == j) {
+                l1 += rsum_y(fby, g, i, gy[t], 1);
+                const double hv = rsum_y(fby, g, i, gy[t], 2);
+                l2 += flip ? -hv : hv;
+                if (0.0 < p.x) eb += he;
+            }
+            if (edge == j) {
+                zb += rsum_y(fby, g, i, gy[t], 3);
+                nb += rsum_y(fby, g, i, gy[t], 4);
+                if (0.0 < p.x) zb -= he;
+            }
+        }
+    }
+    lam_new[c] = eb;
+    lam_new[N + c] = l1;
+    lam_new[2 * N + c] = l2;
+    gz[c] += eb + zb;
+    gn[c] += nb;
+}
+
+// ---------------------------------------------------------------------------
+// Small kernels
+// ---------------------------------------------------------------------------
+__global__ void fill_kernel(double* p, int64_t n, double v)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = v;
+}
+
+__global__ void sample_kernel(const double* h, const int64_t* gauges, double* out, int ng)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ng) out[g] = h[gauges[g]];
+}
+
+// Gauge adjoints into lambda_h, in gauge order (repeated gauges add up).
+__global__ void seed_kernel(double* lam_h, const int64_t* gauges, const double* gs, int ng)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int g = 0; g < ng; ++g) lam_h[gauges[g]] += gs[g];
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+}  // namespace adj
+}  // namespace hx
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+using namespace hx;
+using namespace hx::adj;
+
+struct hx_adjoint {
+    hx_ctx* ctx;
+    int solver, nx, ny;
+    double dx, dy;
+    int bc_lo[2], bc_hi[2];
+    int nsteps, every, ncp, nseg, ngauge;
+    int64_t N;
+    int64_t* gauges;   // device, ngauge flat cell indices
+    double* ckpt;      // ncp states (3N each): the state at steps 0, every, 2 every, ...
+    double* seg;       // nseg states: recomputed states inside a segment
+    double* lam;       // 2 states: adjoint ping-pong
+    double* lun;       // 1 state: adjoint before friction (hydrostatic)
+    double* fbx;       // x-face records
+    double* fby;       // y-face records
+    std::vector<double> dts;
+    const double* z;
+    const double* n;
+    int ready;         // a forward completed and the dt schedule is known
+};
+
+static void adj_free(hx_adjoint* a)
+{
+    double* bufs[] = {a->ckpt, a->seg, a->lam, a->lun, a->fbx, a->fby};
+    for (double* b : bufs)
+        if (b) hipFree(b);
+    if (a->gauges) hipFree(a->gauges);
+}
+
+extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, int nsteps,
+                                 int ckpt_every, int ngauge, const int64_t* gauges,
+                                 hx_adjoint** out)
+{
+    if (!c) return HX_E_STATE;
+    if (!out) return HX_E_BADARG;
+    *out = nullptr;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    if (!geom || geom->nx < 2 || geom->ny < 2 || geom->dtype != HX_F64) return HX_E_BADARG;
+    if (!(geom->dx > 0.0) || !(geom->dy > 0.0)) return HX_E_BADARG;
+    for (int d = 0; d < 2; ++d)
+        if ((geom->bc_lo[d] != 0 && geom->bc_lo[d] != 1) ||
+            (geom->bc_hi[d] != 0 && geom->bc_hi[d] != 1))
+            return HX_E_BADARG;
+    if (nsteps < 1 || ckpt_every < 1 || ngauge < 0 || (ngauge > 0 && !gauges)) return HX_E_BADARG;
+    const int64_t N = (int64_t)geom->nx * geom->ny;
+    for (int g = 0; g < ngauge; ++g)
+        if (gauges[g] < 0 || gauges[g] >= N) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hx_adjoint* a = new (std::nothrow) hx_adjoint();
+    if (!a) return HX_E_NOMEM;
+    a->ctx = c;
+    a->solver = solver;
+    a->nx = geom->nx;
+    a->ny = geom->ny;
+    a->dx = geom->dx;
+    a->dy = geom->dy;
+    for (int d = 0; d < 2; ++d) {
+        a->bc_lo[d] = geom->bc_lo[d];
+        a->bc_hi[d] = geom->bc_hi[d];
+    }
+    a->nsteps = nsteps;
+    a->every = std::min(ckpt_every, nsteps);
+    a->ncp = (nsteps + a->every - 1) / a->every;
+    a->nseg = std::max(a->every - 1, 2);
+    a->ngauge = ngauge;
+    a->N = N;
+    const size_t S = sizeof(double) * 3 * (size_t)N;
+    const int nrec = (solver == HX_SOLVER_ROE) ? 10 : 16;
+    const size_t nfx = (size_t)(a->nx + 1) * a->ny, nfy = (size_t)a->nx * (a->ny + 1);
+    bool ok = hipMalloc(&a->ckpt, S * a->ncp) == hipSuccess &&
+              hipMalloc(&a->seg, S * a->nseg) == hipSuccess &&
+              hipMalloc(&a->lam, 2 * S) == hipSuccess &&
+              hipMalloc(&a->fbx, sizeof(double) * nrec * nfx) == hipSuccess &&
+              hipMalloc(&a->fby, sizeof(double) * nrec * nfy) == hipSuccess;
+    if (ok && solver == HX_SOLVER_HYDROSTATIC_ROE) ok = hipMalloc(&a->lun, S) == hipSuccess;
+    if (ok && ngauge > 0) {
+        ok = hipMalloc(&a->gauges, sizeof(int64_t) * ngauge) == hipSuccess &&
+             hipMemcpy(a->gauges, gauges, sizeof(int64_t) * ngauge, hipMemcpyHostToDevice) ==
+                 hipSuccess;
+    }
+    if (!ok) {
+        adj_free(a);
+        delete a;
+        return HX_E_NOMEM;
+    }
+    a->dts.assign(nsteps, 0.0);
+    *out = a;
+    return HX_OK;
+}
+
+extern "C" int hx_adjoint_destroy(hx_adjoint* a)
+{
+    if (!a) return HX_OK;
+    hipSetDevice(a->ctx->device);
+    adj_free(a);
+    delete a;
+    return HX_OK;
+}
+
+static double* state_ptr(hx_adjoint* a, double* base, int k) { return base + (size_t)k * 3 * a->N; }
+
+static void set_io(StepArgs<double>& s, const double* in, int64_t N, double* ho, double* huo,
+                   double* hvo)
+{
+    s.p.h = in;
+    s.p.hu = in + N;
+    s.p.hv = in + 2 * N;
+    s.ho = ho;
+    s.huo = huo;
+    s.hvo = hvo;
+}
+
+extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl, double* dt_sched,
+                                  double* series, void* ho, void* huo, void* hvo,
+                                  int32_t* steps_done, void* stream)
+{
+    if (!a) return HX_E_STATE;
+    a->ready = 0;
+    if (steps_done) *steps_done = 0;
+    int rc = host::check_patch(st, 0);
+    if (rc) return rc;
+    if (st->dtype != HX_F64 || st->nx != a->nx || st->ny != a->ny || st->pitch != a->nx ||
+        st->eta || st->dx != a->dx || st->dy != a->dy)
+        return HX_E_BADARG;
+    for (int d = 0; d < 2; ++d)
+        if (st->bc_lo[d] != a->bc_lo[d] || st->bc_hi[d] != a->bc_hi[d]) return HX_E_BADARG;
+    if (!dt_sched || !ho || !huo || !hvo || (a->ngauge > 0 && !series)) return HX_E_BADARG;
+    const bool use_cfl = cfl > 0.0;
+    if (!use_cfl)
+        for (int s = 0; s < a->nsteps; ++s)
+            if (!(dt_sched[s] >= 0.0)) return HX_E_BADARG;
+    hx_ctx* c = a->ctx;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t hs = (hipStream_t)stream;
+    const int64_t N = a->N;
+    a->z = (const double*)st->z;
+    a->n = (const double*)st->n;
+    double* c0 = a->ckpt;
+    HX_TRY(hipMemcpyAsync(c0, st->h, sizeof(double) * N, hipMemcpyDeviceToDevice, hs));
+    HX_TRY(hipMemcpyAsync(c0 + N, st->hu, sizeof(double) * N, hipMemcpyDeviceToDevice, hs));
+    HX_TRY(hipMemcpyAsync(c0 + 2 * N, st->hv, sizeof(double) * N, hipMemcpyDeviceToDevice, hs));
+    StepArgs<double> s{};
+    s.p = host::view_of<double>(st);
+    s.opitch = a->nx;
+    host::fill_geometry(s, a->solver, c->rows[a->solver][HX_F64], a->dx, a->dy);
+    if (use_cfl) {
+        rc = host::ensure_hist(c, a->nsteps);
+        if (rc) return rc;
+        rc = host::clear_slots(c->slots, 3, c->err, hs);
+        if (rc) return rc;
+        hx_patch p0 = *st;
+        p0.h = c0;
+        p0.hu = c0 + N;
+        p0.hv = c0 + 2 * N;
+        rc = host::reduce_cfl_f64(&p0, c->slots, hs);
+        if (rc) return rc;
+        s.cfl = cfl;
+        s.err = c->err;
+        s.dt_hist = c->hist;
+        s.vel_hist = c->hist + c->hist_cap;
+    }
+    const double* in = c0;
+    for (int k = 0; k < a->nsteps; ++k) {
+        double *oh, *ohu, *ohv;
+        if (k == a->nsteps - 1) {
+            oh = (double*)ho;
+            ohu = (double*)huo;
+            ohv = (double*)hvo;
+        } else {
+            double* o = ((k + 1) % a->every == 0) ? state_ptr(a, a->ckpt, (k + 1) / a->every)
+                                                  : state_ptr(a, a->seg, (k + 1) & 1);
+            oh = o;
+            ohu = o + N;
+            ohv = o + 2 * N;
+        }
+        set_io(s, in, N, oh, ohu, ohv);
+        if (use_cfl) {
+            s.rd = c->slots + (k % 3);
+            s.acc = c->slots + ((k + 1) % 3);
+            s.clr = c->slots + ((k + 2) % 3);
+            s.step = k;
+        } else {
+            s.rd = nullptr;
+            s.dt_host = dt_sched[k];
+        }
+        rc = host::launch_step_f64(c, a->solver, s, hs);
+        if (rc) return rc;
+        if (a->ngauge > 0) {
+            hipLaunchKernelGGL(sample_kernel, dim3((a->ngauge + 63) / 64), dim3(64), 0, hs, oh,
+                               a->gauges, series + (size_t)k * a->ngauge, a->ngauge);
+            HX_TRY(hipGetLastError());
+        }
+        in = oh;   // oh, ohu, ohv are planes of one state except for the last step
+    }
+    if (use_cfl) {
+        int err = 0;
+        HX_TRY(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, hs));
+        HX_TRY(hipMemcpyAsync(a->dts.data(), c->hist, sizeof(double) * a->nsteps,
+                              hipMemcpyDeviceToHost, hs));
+        HX_TRY(hipStreamSynchronize(hs));
+        if (err) {
+            if (steps_done) *steps_done = err - 1;
+            return HX_E_DT_COLLAPSE;
+        }
+        std::copy(a->dts.begin(), a->dts.end(), dt_sched);
+    } else {
+        std::copy(dt_sched, dt_sched + a->nsteps, a->dts.begin());
+    }
+    if (steps_done) *steps_done = a->nsteps;
+    a->ready = 1;
+    return HX_OK;
+}
+
+// One adjoint step: lam (adjoint of the step's output) -> lam_new (adjoint
+// of its input U), accumulating the bed and roughness gradients.
+static int adj_step(hx_adjoint* a, const double* U, double dt, const double* lam, double* lam_new,
+                    double* gz, double* gn, hipStream_t hs)
+{
+    const int64_t N = a->N;
+    State s{U, U + N, U + 2 * N, a->z, a->n};
+    Geo g{a->nx, a->ny, a->bc_lo[0], a->bc_lo[1], a->bc_hi[0], a->bc_hi[1]};
+    const int64_t nfx = (int64_t)(a->nx + 1) * a->ny, nfy = (int64_t)a->nx * (a->ny + 1);
+    if (a->solver == HX_SOLVER_HYDROSTATIC_ROE) {
+        hipLaunchKernelGGL(hpost_kernel, dim3(nblocks(N)), dim3(kThreads), 0, hs, s, g, dt, a->dx,
+                           a->dy, lam, a->lun, gn);
+        hipLaunchKernelGGL(hface_kernel<0>, dim3(nblocks(nfx)), dim3(kThreads), 0, hs, s, g, dt,
+                           a->dx, (const double*)a->lun, a->fbx);
+        hipLaunchKernelGGL(hface_kernel<1>, dim3(nblocks(nfy)), dim3(kThreads), 0, hs, s, g, dt,
+                           a->dy, (const double*)a->lun, a->fby);
+        hipLaunchKernelGGL(hgather_kernel, dim3(nblocks(N)), dim3(kThreads), 0, hs, s, g,
+                           (const double*)a->lun, (const double*)a->fbx, (const double*)a->fby,
+                           lam_new, gz);
+    } else {
+        hipLaunchKernelGGL(rface_kernel<0>, dim3(nblocks(nfx)), dim3(kThreads), 0, hs, s, g, dt,
+                           a->dx, lam, a->fbx);
+        hipLaunchKernelGGL(rface_kernel<1>, dim3(nblocks(nfy)), dim3(kThreads), 0, hs, s, g, dt,
+                           a->dy, lam, a->fby);
+        hipLaunchKernelGGL(rgather_kernel, dim3(nblocks(N)), dim3(kThreads), 0, hs, s, g, lam,
+                           (const double*)a->fbx, (const double*)a->fby, lam_new, gz, gn);
+    }
+    return host::hip_status(hipGetLastError());
+}
+
+extern "C" int hx_adjoint_backward(hx_adjoint* a, const double* g_series, const double* g_final,
+                                   double* g_state, double* g_z, double* g_n, void* stream)
+{
+    if (!a) return HX_E_STATE;
+    if (!a->ready) return HX_E_STATE;
+    if (!g_state || !g_z || !g_n) return HX_E_BADARG;
+    hx_ctx* c = a->ctx;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t hs = (hipStream_t)stream;
+    const int64_t N = a->N;
+    double* lam = a->lam;
+    double* lam2 = a->lam + 3 * N;
+    const unsigned fb = (unsigned)std::min<int64_t>(nblocks(3 * N), 8192);
+    if (g_final)
+        HX_TRY(hipMemcpyAsync(lam, g_final, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, hs));
+    else
+        hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(kThreads), 0, hs, lam, 3 * N, 0.0);
+    hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(kThreads), 0, hs, g_z, N, 0.0);
+    hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(kThreads), 0, hs, g_n, N, 0.0);
+    HX_TRY(hipGetLastError());
+    StepArgs<double> s{};
+    hx_patch geom{};
+    geom.nx = a->nx;
+    geom.ny = a->ny;
+    geom.dtype = HX_F64;
+    geom.pitch = a->nx;
+    geom.z = a->z;
+    geom.n = a->n;
+    geom.dx = a->dx;
+    geom.dy = a->dy;
+    for (int d = 0; d < 2; ++d) {
+        geom.bc_lo[d] = a->bc_lo[d];
+        geom.bc_hi[d] = a->bc_hi[d];
+    }
+    s.p = host::view_of<double>(&geom);
+    s.opitch = a->nx;
+    host::fill_geometry(s, a->solver, c->rows[a->solver][HX_F64], a->dx, a->dy);
+    for (int cp = a->ncp - 1; cp >= 0; --cp) {
+        const int s0 = cp * a->every, s1 = std::min(s0 + a->every, a->nsteps);
+        auto state = [&](int t) -> double* {
+            return (t == s0) ? state_ptr(a, a->ckpt, cp) : state_ptr(a, a->seg, t - s0 - 1);
+        };
+        // recompute the segment's states from its checkpoint
+        for (int t = s0; t < s1 - 1; ++t) {
+            double* o = state(t + 1);
+            set_io(s, state(t), N, o, o + N, o + 2 * N);
+            s.dt_host = a->dts[t];
+            int rc = host::launch_step_f64(c, a->solver, s, hs);
+            if (rc) return rc;
+        }
+        for (int t = s1 - 1; t >= s0; --t) {
+            if (g_series && a->ngauge > 0) {
+                hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(64), 0, hs, lam, a->gauges,
+                                   g_series + (size_t)t * a->ngauge, a->ngauge);
+            }
+            int rc = adj_step(a, state(t), a->dts[t], lam, lam2, g_z, g_n, hs);
+            if (rc) return rc;
+            std::swap(lam, lam2);
+        }
+    }
+    HX_TRY(hipMemcpyAsync(g_state, lam, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, hs));
+    return HX_OK;
+}

@@ -1,0 +1,88 @@
+// hx_host.hpp -- host-side internals shared by the library's translation
+// units: the context behind the opaque hx_ctx handle, patch views, launch
+// geometry and the fused-advance launcher (defined in hx_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../include/hydroexa.h"
+#include "hx_types.hpp"
+
+struct hx_ctx {
+    int device;
+    hx::CflSlot* slots;  // 4 slots: 3 for the fused loop, 1 for hx_level_dt
+    int* err;
+    double* hist;        // 2 * hist_cap doubles (dt, vel)
+    int hist_cap;
+    uint32_t* pflags;    // prune flags
+    int32_t* pboxes;
+    int pcap;
+    int rows[2][2];      // [solver][dtype]
+    int occ[2][2];       // [solver][dtype] minimum waves per SIMD requested
+    int arith;           // HX_ARITH_EXACT or HX_ARITH_FAST
+};
+
+#define HX_TRY(x)                                  \
+    do {                                           \
+        hipError_t _e = (x);                       \
+        if (_e != hipSuccess) return HX_E_HIP;     \
+    } while (0)
+
+namespace hx {
+namespace host {
+
+inline int hip_status(hipError_t e) { return (e == hipSuccess) ? HX_OK : HX_E_HIP; }
+
+// 1/d is exact when d is a power of two (mantissa bits all zero).
+inline bool is_pow2(double d)
+{
+    int e;
+    return d > 0.0 && std::frexp(d, &e) == 0.5;
+}
+
+template <typename R>
+inline PatchView<R> view_of(const hx_patch* in)
+{
+    PatchView<R> v;
+    v.h = (const R*)in->h;
+    v.hu = (const R*)in->hu;
+    v.hv = (const R*)in->hv;
+    v.eta = (const R*)in->eta;
+    v.z = (const R*)in->z;
+    v.n = (const R*)in->n;
+    v.pitch = in->pitch;
+    v.nx = in->nx;
+    v.ny = in->ny;
+    v.bl0 = in->bc_lo[0];
+    v.bl1 = in->bc_lo[1];
+    v.bh0 = in->bc_hi[0];
+    v.bh1 = in->bc_hi[1];
+    return v;
+}
+
+template <typename R>
+inline void fill_geometry(StepArgs<R>& a, int solver, int rows, double dx, double dy)
+{
+    a.dx = (R)dx;
+    a.dy = (R)dy;
+    a.rdx = (R)(1.0 / dx);
+    a.rdy = (R)(1.0 / dy);
+    a.dx_pow2 = is_pow2(dx) && is_pow2(dy) ? 1 : 0;
+    const int cols = (solver == HX_SOLVER_ROE) ? kRoeCols : kHydroCols;
+    a.rows = rows;
+    a.nsx = (a.p.nx + cols - 1) / cols;
+    const int nsy = (a.p.ny + rows - 1) / rows;
+    a.nstrips = a.nsx * nsy;
+}
+
+// Defined in hx_kernels.hip.
+int check_patch(const hx_patch* in, int need_ghost);
+int launch_step_f64(const hx_ctx* c, int solver, const StepArgs<double>& a, hipStream_t st);
+int reduce_cfl_f64(const hx_patch* in, CflSlot* slot, hipStream_t st);
+int clear_slots(CflSlot* s, int n, int* err, hipStream_t st);
+int ensure_hist(hx_ctx* c, int nsteps);
+
+}  // namespace host
+}  // namespace hx

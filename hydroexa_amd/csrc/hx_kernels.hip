@@ -14,6 +14,7 @@
 
 #include "../../include/hydroexa.h"
 #include "hx_types.hpp"
+#include "hx_host.hpp"
 
 namespace hx {
 #define HX_NS exact
@@ -237,31 +238,7 @@ __global__ __launch_bounds__(kBlock) void ens_cfl_kernel(const ScenDesc<R>* desc
 // C-ABI
 // ===========================================================================
 using namespace hx;
-
-struct hx_ctx {
-    int device;
-    CflSlot* slots;      // 4 slots: 3 for the fused loop, 1 for hx_level_dt
-    int* err;
-    double* hist;        // 2 * hist_cap doubles (dt, vel)
-    int hist_cap;
-    uint32_t* pflags;    // prune flags
-    int32_t* pboxes;
-    int pcap;
-    int rows[2][2];      // [solver][dtype]
-    int occ[2][2];       // [solver][dtype] minimum waves per SIMD requested
-    int arith;           // HX_ARITH_EXACT or HX_ARITH_FAST
-};
-
-static int hip_status(hipError_t e)
-{
-    return (e == hipSuccess) ? HX_OK : HX_E_HIP;
-}
-
-#define HX_TRY(x)                                  \
-    do {                                           \
-        hipError_t _e = (x);                       \
-        if (_e != hipSuccess) return HX_E_HIP;     \
-    } while (0)
+using namespace hx::host;
 
 // Launch-shape defaults from the strip-height / occupancy sweep on MI355X
 // (4096^2 plain Roe and 2048^2 hydrostatic workloads): short strips keep
@@ -277,13 +254,6 @@ static int default_occ(int solver, int dtype)
 {
     (void)solver;
     return (dtype == HX_F64) ? 3 : 1;
-}
-
-// 1/d is exact when d is a power of two (mantissa bits all zero).
-static bool is_pow2(double d)
-{
-    int e;
-    return d > 0.0 && std::frexp(d, &e) == 0.5;
 }
 
 extern "C" int hx_abi_version(void) { return HX_ABI_VERSION; }
@@ -362,7 +332,7 @@ extern "C" int hx_set_arith(hx_ctx* c, int mode)
     return HX_OK;
 }
 
-static int check_patch(const hx_patch* in, int need_ghost)
+int hx::host::check_patch(const hx_patch* in, int need_ghost)
 {
     if (!in || in->nx < 2 || in->ny < 2 || in->ng < 0 || in->ng > 8) return HX_E_BADARG;
     if (in->dtype != HX_F64 && in->dtype != HX_F32) return HX_E_BADARG;
@@ -376,41 +346,6 @@ static int check_patch(const hx_patch* in, int need_ghost)
     const int64_t gx = (in->bc_lo[0] == HX_BC_GHOST ? in->ng : 0) + (in->bc_hi[0] == HX_BC_GHOST ? in->ng : 0);
     if (in->pitch < (int64_t)in->nx + gx) return HX_E_BADARG;
     return HX_OK;
-}
-
-template <typename R>
-static PatchView<R> view_of(const hx_patch* in)
-{
-    PatchView<R> v;
-    v.h = (const R*)in->h;
-    v.hu = (const R*)in->hu;
-    v.hv = (const R*)in->hv;
-    v.eta = (const R*)in->eta;
-    v.z = (const R*)in->z;
-    v.n = (const R*)in->n;
-    v.pitch = in->pitch;
-    v.nx = in->nx;
-    v.ny = in->ny;
-    v.bl0 = in->bc_lo[0];
-    v.bl1 = in->bc_lo[1];
-    v.bh0 = in->bc_hi[0];
-    v.bh1 = in->bc_hi[1];
-    return v;
-}
-
-template <typename R>
-static void fill_geometry(StepArgs<R>& a, int solver, int rows, double dx, double dy)
-{
-    a.dx = (R)dx;
-    a.dy = (R)dy;
-    a.rdx = (R)(1.0 / dx);
-    a.rdy = (R)(1.0 / dy);
-    a.dx_pow2 = is_pow2(dx) && is_pow2(dy) ? 1 : 0;
-    const int cols = (solver == HX_SOLVER_ROE) ? kRoeCols : kHydroCols;
-    a.rows = rows;
-    a.nsx = (a.p.nx + cols - 1) / cols;
-    const int nsy = (a.p.ny + rows - 1) / rows;
-    a.nstrips = a.nsx * nsy;
 }
 
 template <typename R>
@@ -461,6 +396,38 @@ static int launch_cfl(const hx_patch* in, CflSlot* slot, hipStream_t st)
     hipLaunchKernelGGL(cfl_kernel<R>, dim3((unsigned)nblk), dim3(kBlock), 0, st, view_of<R>(in),
                        (R)in->dx, (R)in->dy, (R)(1.0 / in->dx), (R)(1.0 / in->dy), mul, slot);
     return hip_status(hipGetLastError());
+}
+
+// Entry points for the other translation units (hx_host.hpp).
+int hx::host::launch_step_f64(const hx_ctx* c, int solver, const StepArgs<double>& a,
+                              hipStream_t st)
+{
+    return launch<double>(c, solver, HX_F64, a, st);
+}
+
+int hx::host::reduce_cfl_f64(const hx_patch* in, CflSlot* slot, hipStream_t st)
+{
+    return launch_cfl<double>(in, slot, st);
+}
+
+int hx::host::clear_slots(CflSlot* s, int n, int* err, hipStream_t st)
+{
+    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, s, n, err);
+    return hip_status(hipGetLastError());
+}
+
+// Grow the context's dt / max|vel| history to hold nsteps entries.
+int hx::host::ensure_hist(hx_ctx* c, int nsteps)
+{
+    if (nsteps <= c->hist_cap) return HX_OK;
+    if (c->hist) hipFree(c->hist);
+    c->hist = nullptr;
+    c->hist_cap = 0;
+    int cap = 1024;
+    while (cap < nsteps) cap *= 2;
+    if (hipMalloc(&c->hist, sizeof(double) * 2 * cap) != hipSuccess) return HX_E_NOMEM;
+    c->hist_cap = cap;
+    return HX_OK;
 }
 
 extern "C" int hx_level_dt(hx_ctx* c, int solver, const hx_patch* in, double cfl, double* dt_out,
@@ -642,15 +609,8 @@ extern "C" int hx_run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah,
     if (nsteps == 0) return HX_OK;
     HX_TRY(hipSetDevice(c->device));
     hipStream_t st = (hipStream_t)stream;
-    if (nsteps > c->hist_cap) {
-        if (c->hist) hipFree(c->hist);
-        c->hist = nullptr;
-        c->hist_cap = 0;
-        int cap = 1024;
-        while (cap < nsteps) cap *= 2;
-        if (hipMalloc(&c->hist, sizeof(double) * 2 * cap) != hipSuccess) return HX_E_NOMEM;
-        c->hist_cap = cap;
-    }
+    rc = ensure_hist(c, nsteps);
+    if (rc) return rc;
     rc = (in->dtype == HX_F64) ? run_t<double>(c, solver, in, ah, ahu, ahv, cfl, nsteps, st)
                                : run_t<float>(c, solver, in, ah, ahu, ahv, cfl, nsteps, st);
     if (rc) return rc;

@@ -215,6 +215,50 @@ int hx_ensemble_set_tuning(hx_ensemble *ens, int rows, int occupancy, int persis
 int hx_ensemble_run(hx_ensemble *ens, double cfl, int nsteps, int use_graph, double *dt_hist,
                     int32_t *steps_done, void *stream);
 
+/*
+ * Adjoint of a single-level rollout (FP64).  No reference counterpart:
+ * HydroEXA computes no derivatives.  The rollout is nsteps of the step
+ * behind hx_advance_level (plain Roe or hydrostatic Roe with Manning
+ * friction, src/solvers/Roe.cpp:250-434, src/solvers/HydrostaticRoe.cpp:400-606)
+ * with physical sides; its outputs are the depth at ngauge cells after
+ * every step (the gauge series) and the final state.  hx_adjoint_forward
+ * runs the rollout and keeps the state every ckpt_every steps;
+ * hx_adjoint_backward recomputes each segment from its checkpoint and runs
+ * the hand-written adjoint step backwards through it, with the dt schedule
+ * of the forward held fixed.
+ *
+ * geom: nx, ny, dx, dy and bc_lo/bc_hi (HX_BC_REFLECT / HX_BC_TRANSMIT),
+ * dtype HX_F64; its pointers are ignored.  gauges (host): flat cell
+ * indices j*nx + i.  Device memory: (ncp + max(ckpt_every-1, 2) + 3) states
+ * of 3*nx*ny doubles plus 10 (Roe) or 16 (hydrostatic) doubles per face,
+ * ncp = ceil(nsteps / ckpt_every).
+ */
+typedef struct hx_adjoint hx_adjoint;
+int hx_adjoint_create(hx_ctx *ctx, int solver, const hx_patch *geom, int nsteps, int ckpt_every,
+                      int ngauge, const int64_t *gauges, hx_adjoint **out);
+int hx_adjoint_destroy(hx_adjoint *adj);
+
+/*
+ * Forward rollout.  state: t^0 (dense, pitch nx, F64, no eta) with z and n,
+ * which must stay alive until the backward pass is done.  cfl > 0: each
+ * step's dt is cfl times the CFL estimate of its input (as hx_run_level),
+ * and dt_sched (host, nsteps) receives the schedule; cfl <= 0: dt_sched
+ * gives it.  series (device, nsteps x ngauge): depth at the gauges after
+ * each step.  h_out/hu_out/hv_out (device, dense): the final state.
+ */
+int hx_adjoint_forward(hx_adjoint *adj, const hx_patch *state, double cfl, double *dt_sched,
+                       double *series, void *h_out, void *hu_out, void *hv_out,
+                       int32_t *steps_done, void *stream);
+
+/*
+ * Backward pass of the last forward.  g_series (device, nsteps x ngauge,
+ * optional) and g_final (device, 3 planes h, hu, hv, optional): adjoints of
+ * the outputs.  g_state (device, 3 planes): gradient with respect to t^0;
+ * g_z, g_n (device, nx*ny): gradients with respect to bed and roughness.
+ */
+int hx_adjoint_backward(hx_adjoint *adj, const double *g_series, const double *g_final,
+                        double *g_state, double *g_z, double *g_n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

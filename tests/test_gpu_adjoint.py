@@ -1,0 +1,239 @@
+"""Hand-written adjoint of the rollout against the complex-step tangent-linear
+reference (oracle/tlm.py, independent numpy code), central finite
+differences, checkpoint-interval invariance, and the autograd Function.
+
+For a scalar J = <w, series> + <W, final state>, the adjoint gives dJ/dx for
+every input at once; the tangent gives dJ/dx . d along one direction d.
+Both linearise the same step with the same frozen dt schedule and the same
+branch conventions, so they agree to rounding (1e-9 relative here)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from oracle import tlm as T
+from hydroexa_amd.pyhydroexa import Rollout, calibrate, synthetic_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def field(nx, ny, seed, model, bc, friction, nodata=False, dry=False):
+    r = np.random.default_rng(seed)
+    X, Y = np.meshgrid((np.arange(nx) + 0.5) / nx, (np.arange(ny) + 0.5) / ny)
+    z = 0.15 * np.exp(-((X - 0.45) ** 2 + (Y - 0.55) ** 2) / 0.03) + 0.01 * r.random((ny, nx))
+    eta = np.where(X < 0.5, 1.0, 0.6) + 0.005 * r.random((ny, nx))
+    if dry and model == "HydrostaticRoe":
+        z = z + np.where(X > 0.75, 0.8, 0.0)          # a dry shelf on the right
+    h = np.maximum(eta - z, 0.0)
+    if dry and model == "Roe":
+        h = np.where(X > 0.8, 0.0, h)                 # a dry bed (the plain Roe thrust is
+                                                      # not balanced on emerged steps)
+    hu = 0.05 * r.standard_normal((ny, nx)) * h
+    hv = 0.05 * r.standard_normal((ny, nx)) * h
+    n = (friction + 0.01 * r.random((ny, nx))) if friction else np.zeros((ny, nx))
+    if nodata:
+        z[:5, :6] = -9999.0
+        n[:5, :6] = -9999.0
+        h[:5, :6] = 0.0
+        hu[:5, :6] = 0.0
+        hv[:5, :6] = 0.0
+    return h, hu, hv, z, n
+
+
+def dev(a):
+    return torch.as_tensor(np.asarray(a), dtype=torch.float64, device="cuda")
+
+
+CASES = [
+    # model, bc_lo, bc_hi, friction, nodata, dry
+    ("Roe", (1, 1), (1, 1), 0.0, False, False),
+    ("Roe", (0, 0), (0, 0), 0.03, False, False),
+    ("Roe", (0, 1), (1, 0), 0.0, False, True),    # the plain Roe solver's explicit friction
+                                                  # source collapses dt at a dry front (the
+                                                  # CPU restatement does the same)
+    ("HydrostaticRoe", (1, 1), (1, 1), 0.0, False, False),
+    ("HydrostaticRoe", (0, 0), (0, 0), 0.03, False, False),
+    ("HydrostaticRoe", (0, 1), (1, 0), 0.03, True, False),
+    ("HydrostaticRoe", (1, 0), (0, 1), 0.02, True, True),
+]
+
+
+def _run_case(model, bc_lo, bc_hi, friction, nodata, dry, every, nx=40, ny=28, steps=24, seed=3):
+    solver = 0 if model == "Roe" else 1
+    h, hu, hv, z, n = field(nx, ny, seed, model, None, friction, nodata, dry)
+    dx, dy = 1.0 / nx, 1.0 / ny
+    r = np.random.default_rng(seed + 100)
+    gauges = r.choice(nx * ny, size=7, replace=False)
+    ro = Rollout(model, nx, ny, dx, dy, bc_lo, bc_hi, steps, gauges, ckpt_every=every, cfl=0.45)
+    U0 = torch.stack([dev(h), dev(hu), dev(hv)]).contiguous()
+    ter = torch.stack([dev(z), dev(n)]).contiguous()
+    series, fin = ro.forward_raw(U0, ter)
+    dts = ro.last_dt.copy()
+    w = r.standard_normal((steps, gauges.size))
+    W = r.standard_normal((3, ny, nx))
+    g_state, g_ter = ro.backward_raw(dev(w), dev(W))
+    torch.cuda.synchronize()
+    out = dict(series=series.cpu().numpy(), fin=fin.cpu().numpy(), g_state=g_state.cpu().numpy(),
+               g_ter=g_ter.cpu().numpy(), dts=dts, w=w, W=W, gauges=gauges,
+               inputs=(h, hu, hv, z, n), dx=dx, dy=dy, solver=solver)
+    ro.close()
+    return out
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}{c[2]}-n{c[3]}-nd{int(c[4])}-dry{int(c[5])}"
+                                             for c in CASES])
+def test_adjoint_matches_complex_step_tangent(gpu, case):
+    model, bc_lo, bc_hi, friction, nodata, dry = case
+    o = _run_case(model, bc_lo, bc_hi, friction, nodata, dry, every=5)
+    h, hu, hv, z, n = o["inputs"]
+    r = np.random.default_rng(7)
+    valid = z > -9000.0
+    dirs = {
+        "h": (r.standard_normal(h.shape) * (h > 0), None, None, None, None),
+        "hu": (None, r.standard_normal(h.shape) * (h > 0), None, None, None),
+        "hv": (None, None, r.standard_normal(h.shape) * (h > 0), None, None),
+        "z": (None, None, None, r.standard_normal(h.shape) * valid, None),
+    }
+    if friction:
+        dirs["n"] = (None, None, None, None, r.standard_normal(h.shape) * valid)
+    grads = {"h": o["g_state"][0], "hu": o["g_state"][1], "hv": o["g_state"][2],
+             "z": o["g_ter"][0], "n": o["g_ter"][1]}
+    for name, d in dirs.items():
+        ds, df, (s_re, f_re) = T.directional_derivative(
+            o["solver"], (h, hu, hv), z, n, o["dts"], o["dx"], o["dy"], case[1], case[2],
+            o["gauges"], d_state=d[:3], d_z=d[3], d_n=d[4])
+        tlm = np.sum(o["w"] * ds) + sum(np.sum(o["W"][k] * df[k]) for k in range(3))
+        scale = np.sum(np.abs(o["w"] * ds)) + sum(np.sum(np.abs(o["W"][k] * df[k])) for k in range(3))
+        dvec = d[["h", "hu", "hv", "z", "n"].index(name)]
+        adj = np.sum(grads[name] * dvec)
+        rel = abs(adj - tlm) / max(scale, 1e-300)
+        print(f"{model} {name}: adjoint {adj:.12e} tangent {tlm:.12e} rel {rel:.2e}")
+        assert rel < 1e-9, (name, adj, tlm, rel)
+    # the forward the adjoint linearised is the oracle's trajectory
+    ref_s, ref_f = T.rollout(o["solver"], h, hu, hv, z, n, o["dts"], o["dx"], o["dy"], case[1],
+                             case[2], o["gauges"])
+    np.testing.assert_allclose(o["series"], ref_s, rtol=1e-12, atol=1e-14)
+
+
+def test_adjoint_checkpoint_interval_does_not_change_gradients(gpu):
+    case = ("HydrostaticRoe", (0, 1), (1, 0), 0.03, True, True)
+    a = _run_case(*case, every=1)
+    b = _run_case(*case, every=7)
+    c = _run_case(*case, every=24)
+    for k in ("g_state", "g_ter"):
+        np.testing.assert_array_equal(a[k], b[k])
+        np.testing.assert_array_equal(a[k], c[k])
+
+
+def _fd_case(model):
+    """Inputs on which central differences measure the derivative.
+
+    Plain Roe: the reference's bed thrust switches between smax(thrust_a,
+    thrust_b) and thrust_b when the sign of dz * d(eta) or u * dz changes
+    (include/solvers/kernels/Roe.H:161-176), a jump; a monotone bed and free
+    surface with flow in one direction keep every sign fixed.
+    Hydrostatic: the van Leer slope is not differentiable where a field is
+    exactly flat (any row with constant velocity), so every field carries
+    seeded noise."""
+    nx, ny = 36, 30
+    X, Y = np.meshgrid((np.arange(nx) + 0.5) / nx, (np.arange(ny) + 0.5) / ny)
+    r = np.random.default_rng(11)
+    if model == "Roe":
+        z = 0.1 * X + 0.05 * Y
+        eta = 1.3 - 0.2 * X - 0.1 * Y + 0.02 * np.exp(-((X - 0.4) ** 2 + (Y - 0.5) ** 2) / 0.05)
+        h = eta - z
+        hu, hv = 0.3 * h, 0.2 * h
+        n = 0.02 + 0.01 * X * Y
+        bc = ((1, 1), (1, 1))
+        eps = {"h": 1e-5, "hu": 1e-5, "z": 1e-5, "n": 1e-5}
+    else:
+        z = 0.1 * np.sin(2 * np.pi * X) * np.cos(np.pi * Y) + 0.01 * r.random((ny, nx))
+        eta = 1.0 + 0.2 * np.exp(-((X - 0.4) ** 2 + (Y - 0.5) ** 2) / 0.02) + 0.01 * r.random((ny, nx))
+        h = eta - z
+        hu = h * (0.1 + 0.05 * r.standard_normal((ny, nx)))
+        hv = h * (0.05 + 0.05 * r.standard_normal((ny, nx)))
+        n = 0.02 + 0.01 * r.random((ny, nx))
+        bc = ((0, 0), (0, 0))
+        # state and bed: small steps keep clear of limiter kinks (r = 0);
+        # roughness enters smoothly and its small gradient wants a larger step
+        eps = {"h": 1e-7, "hu": 1e-7, "z": 1e-7, "n": 1e-5}
+    return nx, ny, h, hu, hv, z, n, bc, eps
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_adjoint_against_central_differences(gpu, model):
+    """Central differences of the product forward (dt frozen) against the
+    adjoint, away from the schemes' kinks and jumps."""
+    nx, ny, h, hu, hv, z, n, bc, eps = _fd_case(model)
+    steps = 16
+    gauges = np.array([5 * nx + 7, 12 * nx + 20, 20 * nx + 30, 25 * nx + 3])
+    ro = Rollout(model, nx, ny, 1.0 / nx, 1.0 / ny, bc[0], bc[1], steps, gauges, ckpt_every=4)
+    U0 = torch.stack([dev(h), dev(hu), dev(hv)]).contiguous()
+    ter = torch.stack([dev(z), dev(n)]).contiguous()
+    ro.forward_raw(U0, ter)
+    ro.dt_schedule = ro.last_dt.copy()             # freeze dt for the differences
+    r = np.random.default_rng(2)
+    w = r.standard_normal((steps, gauges.size))
+    g_state, g_ter = ro.backward_raw(dev(w), None)
+    J = lambda U, Tt: float((ro.forward_raw(U, Tt)[0].cpu().numpy() * w).sum())
+    for name, k, base, grad in (("h", 0, U0, g_state), ("hu", 1, U0, g_state),
+                                ("z", 0, ter, g_ter), ("n", 1, ter, g_ter)):
+        d = dev(r.standard_normal((ny, nx)))
+        e = eps[name]
+        P, M = base.clone(), base.clone()
+        P[k] += e * d
+        M[k] -= e * d
+        if base is U0:
+            fd = (J(P, ter) - J(M, ter)) / (2 * e)
+        else:
+            fd = (J(U0, P) - J(U0, M)) / (2 * e)
+        adj = float((grad[k] * d).sum())
+        rel = abs(fd - adj) / max(abs(adj), 1e-30)
+        print(f"{model} {name}: adjoint {adj:.10e} fd {fd:.10e} rel {rel:.2e}")
+        assert rel < 1e-5, (name, adj, fd)
+    ro.close()
+
+
+def test_autograd_function_and_generation_guard(gpu):
+    nx, ny = 24, 20
+    h, hu, hv, z, n = field(nx, ny, 5, "HydrostaticRoe", None, 0.03)
+    ro = Rollout("HydrostaticRoe", nx, ny, 1.0 / nx, 1.0 / ny, (0, 0), (0, 0), 10, [45, 200, 301],
+                 ckpt_every=3)
+    nn = dev(n).requires_grad_(True)
+    zz = dev(z).requires_grad_(True)
+    hh = dev(h).requires_grad_(True)
+    series, hf, huf, hvf = ro(hh, dev(hu), dev(hv), zz, nn)
+    loss = (series ** 2).sum() + (hf * 0.5).sum() + huf.pow(2).sum()
+    loss.backward()
+    assert nn.grad is not None and zz.grad is not None and hh.grad is not None
+    assert torch.isfinite(nn.grad).all() and torch.isfinite(zz.grad).all()
+    # same gradient through the raw interface
+    U0 = torch.stack([dev(h), dev(hu), dev(hv)]).contiguous()
+    ter = torch.stack([dev(z), dev(n)]).contiguous()
+    s2, f2 = ro.forward_raw(U0, ter)
+    gf = torch.stack([torch.full_like(f2[0], 0.5), 2.0 * f2[1], torch.zeros_like(f2[2])])
+    g_state, g_ter = ro.backward_raw(2.0 * s2, gf)
+    torch.testing.assert_close(nn.grad, g_ter[1], rtol=0, atol=0)
+    torch.testing.assert_close(zz.grad, g_ter[0], rtol=0, atol=0)
+    torch.testing.assert_close(hh.grad, g_state[0], rtol=0, atol=0)
+    # a forward after the first invalidates the first one's backward
+    s_a, *_ = ro(hh, dev(hu), dev(hv), zz, nn)
+    s_b, *_ = ro(hh, dev(hu), dev(hv), zz, nn)
+    with pytest.raises(RuntimeError):
+        s_a.sum().backward()
+    s_b.sum().backward()
+    ro.close()
+
+
+def test_calibration_reduces_misfit(gpu):
+    """Adam through the rollout on a 2 x 2 roughness control grid (the wave
+    crosses one column of it; cells it never reaches get a zero gradient
+    and stay put): the misfit drops and the roughness moves towards the
+    hidden field."""
+    prob = synthetic_problem(n=128, dx=10.0, ctrl=2, hills=12, margin=4, ngauge=12, nsteps=60,
+                             ckpt_every=10)
+    hist = calibrate(prob, iters=10, lr=2e-3)
+    print([(round(h["loss"], 12), round(h["n_mae"], 6)) for h in hist])
+    assert min(h["loss"] for h in hist) < 0.1 * hist[0]["loss"]
+    assert min(h["n_mae"] for h in hist[1:]) < hist[0]["n_mae"]
+    prob.rollout.close()
