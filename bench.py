@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time the other workloads")
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--only", choices=["ensemble", "adjoint"], default=None,
+                    help="run one secondary leg alone and print its JSON (for profiling)")
+    ap.add_argument("--adjoint-steps", type=int, default=400,
+                    help="steps of the 2048^2 calibration rollout timed forward + adjoint (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=120,
                     help="timed steps of the batched ensemble and of the per-scenario loop (0: skip)")
     return ap.parse_args()
@@ -169,6 +173,65 @@ def time_ensemble(steps, arith="fast"):
             "batched_ms_per_step": batched_ms / steps, "plain_ms_per_step": plain_ms / steps}
 
 
+def time_adjoint(steps, every=20, arith="fast"):
+    """Forward rollout and hand-written adjoint on the calibration workload:
+    2048^2 FP64 hydrostatic Roe with Manning friction, 64 hills and a NODATA
+    margin, 32 gauges, checkpoint every `every` steps, dt frozen from the
+    first forward.  Times one forward (gauge series + checkpoints) and one
+    backward (segment recomputation + adjoint steps) with HIP events, and
+    one full calibration iteration through the autograd Function."""
+    import torch
+    from hydroexa_amd import set_arith
+    from hydroexa_amd.pyhydroexa import synthetic_problem
+
+    set_arith(arith)
+    prob = synthetic_problem(n=2048, nsteps=steps, ckpt_every=every)
+    ro = prob.rollout
+    cells = prob.n * prob.n
+    U0 = torch.stack([prob.h0, prob.hu0, prob.hv0]).contiguous()
+    n_ctrl = torch.full((prob.ctrl, prob.ctrl), 0.04, dtype=torch.float64, device=U0.device)
+    ter = torch.stack([prob.bed(), prob.roughness(n_ctrl)]).contiguous()
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    series, _ = ro.forward_raw(U0, ter)                        # warm-up pass
+    gs = 2.0 * (series - prob.observed) / series.numel()
+    ro.backward_raw(gs, None)
+    torch.cuda.synchronize()
+    ev[0].record(st)
+    series, _ = ro.forward_raw(U0, ter)
+    ev[1].record(st)
+    gs = 2.0 * (series - prob.observed) / series.numel()
+    g_state, g_ter = ro.backward_raw(gs, None)
+    ev[2].record(st)
+    torch.cuda.synchronize()
+    fwd_ms, bwd_ms = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    # one calibration iteration through the autograd Function
+    nc = n_ctrl.clone().requires_grad_(True)
+    t0 = time.perf_counter()
+    loss = prob.loss(nc)
+    loss.backward()
+    torch.cuda.synchronize()
+    it_s = time.perf_counter() - t0
+    ncp = (steps + every - 1) // every
+    out = {"workload": "calibration rollout 2048x2048 FP64 hydrostatic Roe + Manning, 64 hills, "
+                       "NODATA margin 32, 32 gauges, 64x64 roughness control grid",
+           "cells": cells, "steps": steps, "ckpt_every": every, "arith": arith,
+           "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+           "forward_Mcups": cells * steps / (fwd_ms / 1e3) / 1e6,
+           "adjoint_Mcups": cells * steps / (bwd_ms / 1e3) / 1e6,
+           "forward_plus_adjoint_Mcups": cells * steps / ((fwd_ms + bwd_ms) / 1e3) / 1e6,
+           "backward_over_forward": bwd_ms / fwd_ms,
+           "recomputed_steps": steps - ncp,
+           "checkpoint_bytes": ncp * 3 * cells * 8,
+           "calibration_iteration_s": it_s, "loss": float(loss.item()),
+           "grad_n_ctrl_norm": float(nc.grad.norm().item())}
+    ro.close()
+    set_arith("exact")
+    del prob, U0, ter, g_state, g_ter
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -179,6 +242,12 @@ def main():
     import torch
     from hydroexa_amd import workloads as W
 
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": time_ensemble(args.ensemble_steps, args.arith)}), flush=True)
+        return
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint": time_adjoint(args.adjoint_steps, 20, args.arith)}), flush=True)
+        return
     if args.solver == "Roe":
         w = W.dam_break(args.n, args.steps)
     else:
@@ -247,6 +316,8 @@ def main():
         out["extra"] = extra
     if args.ensemble_steps > 0:
         out["ensemble"] = time_ensemble(args.ensemble_steps, args.arith)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = time_adjoint(args.adjoint_steps, 20, args.arith)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W.dam_break(args.n, 1) if args.solver == "Roe" else w,
                                            args.cpu_seconds)
