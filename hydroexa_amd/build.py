@@ -20,7 +20,7 @@ LIB = os.path.join(LIBDIR, "libhydroexa.so")
 # (source, FP contraction policy): the exact build must not contract.
 SOURCES = [(os.path.join(CSRC, "hx_kernels.hip"), "off"),
            (os.path.join(CSRC, "hx_fast.hip"), "fast"),
-           (os.path.join(CSRC, "hx_adjoint.hip"), "off")]
+           (os.path.join(CSRC, "hx_adjoint.hip"), "fast")]
 ARCH = os.environ.get("HYDROEXA_ARCH", "gfx950")
 
 

@@ -5,18 +5,18 @@
 // with the dt schedule of that forward held fixed.
 //
 // One adjoint step, lambda^{n+1} -> lambda^n (plus the roughness and bed
-// gradients accumulated), is three kernels, all deterministic (no atomics):
+// gradients accumulated), is deterministic (no atomics):
 //   post    (hydrostatic only) per cell: re-evaluate the cell's four faces,
 //           the flux update, and the reverse of Manning friction and the
 //           dry clamp -> lambda_u (adjoint of the pre-friction state);
-//   face    per face: the face's vector-Jacobian product with respect to
-//           its stencil cells (2 for plain Roe, 4 for the hydrostatic
-//           face) -> one record per face;
-//   gather  per cell: sums the records of every face whose stencil holds
-//           the cell or a ghost filled from it, then applies the ghost-fill
-//           and rebuild chain rules (eta = h + z, h_rebuilt =
+//   tile    per tile of cells: the vector-Jacobian product of every face
+//           whose stencil touches the tile (2 cells for plain Roe, 4 for the
+//           hydrostatic face), summed in LDS, then per cell the gather over
+//           its own position and the ghosts filled from it, with the
+//           ghost-fill and rebuild chain rules (eta = h + z, h_rebuilt =
 //           max(0, eta - z), mirror / edge sources, odd momenta).
-// Compiled with -ffp-contract=off like the exact advance.
+// Compiled with -ffp-contract=fast: derivatives need relative accuracy,
+// not the forward's bit-exact rounding.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -129,288 +129,324 @@ __global__ __launch_bounds__(kThreads) void hpost_kernel(State s, Geo g, double 
     gn[c] += dn;
 }
 
-// Face records: 16 planes (stencil slot q = 0..3 for Lm, L, R, Rp; field
-// m = h, hu, hv, z), plane q*4+m, one value per face.
-template <int DIR>
-__global__ __launch_bounds__(kThreads) void hface_kernel(State s, Geo g, double dt, double ds,
+// ---------------------------------------------------------------------------
+// Fused face VJPs + gather, one workgroup per tile of kTX x kTY cells.
+//
+// The workgroup evaluates the VJP of every face whose stencil touches one of
+// its cells (x faces of its rows, y faces of its columns) and accumulates
+// the results in an LDS image of the tile grown by the stencil reach.  Each
+// face adds its stencil slots in separate passes (slot s of every face in
+// pass s), so no two threads update one LDS cell in a pass and every cell
+// receives its contributions in a fixed order: the sums are deterministic.
+// The gather then reads, for each tile cell, its own position and the ghost
+// positions filled from it, and applies the ghost-fill and rebuild chain
+// rules.  Faces on tile edges are evaluated by both neighbouring tiles
+// ((kTX + 3) / kTX and (kTY + 3) / kTY redundancy for the hydrostatic
+// stencil).
+// ---------------------------------------------------------------------------
+constexpr int kTX = 64, kTY = 16;
+
+// Hydrostatic: LDS image covers x in [i0 - 2, i0 + kTX + 3) and likewise in
+// y: a reflective high side folds ghost n+1 into cell n-2, which may be the
+// last cell of the tile before the one holding n-1.
+constexpr int kHPX = kTX + 5, kHPY = kTY + 5;
+
+__global__ __launch_bounds__(kThreads) void htile_kernel(State s, Geo g, double dt, double dx,
+                                                         double dy,
                                                          const double* __restrict__ lun,
-                                                         double* __restrict__ fb)
+                                                         double* __restrict__ lam_new,
+                                                         double* __restrict__ gz)
 {
+    __shared__ double acc[4][kHPY][kHPX];
     const int64_t N = (int64_t)g.nx * g.ny;
-    const int64_t nf = DIR == 0 ? (int64_t)(g.nx + 1) * g.ny : (int64_t)g.nx * (g.ny + 1);
-    const int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (idx >= nf) return;
-    int f, i, j, n;
-    if (DIR == 0) {
-        j = (int)(idx / (g.nx + 1));
-        f = (int)(idx - (int64_t)j * (g.nx + 1));
-        i = f;
-        n = g.nx;
-    } else {
-        f = (int)(idx / g.nx);
-        i = (int)(idx - (int64_t)f * g.nx);
-        j = f;
-        n = g.ny;
-    }
-    double c[16], cb[16];
-    for (int q = 0; q < 4; ++q)
-        put4(c + 4 * q, DIR == 0 ? pcell(s, g, i - 2 + q, j) : pcell(s, g, i, j - 2 + q));
-    const double r = dt / ds;
-    double mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0};
-    const int64_t cl = DIR == 0 ? (int64_t)j * g.nx + (f - 1) : (int64_t)(f - 1) * g.nx + i;
-    const int64_t cr = DIR == 0 ? (int64_t)j * g.nx + f : (int64_t)f * g.nx + i;
-    if (f >= 1)
-        for (int k = 0; k < 3; ++k) mL[k] = -r * lun[k * N + cl];
-    if (f < n)
-        for (int k = 0; k < 3; ++k) mR[k] = r * lun[k * N + cr];
-    hydro_face_vjp(c, DIR == 0 ? 1.0 : 0.0, DIR == 0 ? 0.0 : 1.0, mL, mR, cb);
-    for (int m = 0; m < 16; ++m) fb[m * nf + idx] = cb[m];
-}
-
-// Sum of the records that land on padded position (px, j) from x faces.
-__device__ __forceinline__ double hsum_x(const double* fb, const Geo& g, int px, int j, int m)
-{
-    const int64_t nf = (int64_t)(g.nx + 1) * g.ny;
-    double acc = 0.0;
-    for (int f = px + 2; f >= px - 1; --f) {
-        if (f < 0 || f > g.nx) continue;
-        const int q = px - f + 2;
-        acc += fb[(q * 4 + m) * nf + (int64_t)j * (g.nx + 1) + f];
-    }
-    return acc;
-}
-
-__device__ __forceinline__ double hsum_y(const double* fb, const Geo& g, int i, int py, int m)
-{
-    const int64_t nf = (int64_t)g.nx * (g.ny + 1);
-    double acc = 0.0;
-    for (int f = py + 2; f >= py - 1; --f) {
-        if (f < 0 || f > g.ny) continue;
-        const int q = py - f + 2;
-        acc += fb[(q * 4 + m) * nf + (int64_t)f * g.nx + i];
-    }
-    return acc;
-}
-
-__global__ __launch_bounds__(kThreads) void hgather_kernel(State s, Geo g,
-                                                           const double* __restrict__ lun,
-                                                           const double* __restrict__ fbx,
-                                                           const double* __restrict__ fby,
-                                                           double* __restrict__ lam_new,
-                                                           double* __restrict__ gz)
-{
-    const int64_t N = (int64_t)g.nx * g.ny;
-    const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (c >= N) return;
-    const int j = (int)(c / g.nx), i = (int)(c - (int64_t)j * g.nx);
-    // the update starts from the stored state
-    double l0 = lun[c], l1 = lun[N + c], l2 = lun[2 * N + c];
-    double eb = 0.0, zb = 0.0;
+    const int i0 = blockIdx.x * kTX, j0 = blockIdx.y * kTY;
+    const int t = threadIdx.x;
+    for (int k = t; k < 4 * kHPY * kHPX; k += kThreads) (&acc[0][0][0])[k] = 0.0;
+    __syncthreads();
+    // x faces f in [i0 - 1, i0 + kTX + 2) of the tile rows
     {
-        const PC p = pcell(s, g, i, j);
-        const double he = hsum_x(fbx, g, i, j, 0) + hsum_y(fby, g, i, j, 0);
-        l1 += hsum_x(fbx, g, i, j, 1) + hsum_y(fby, g, i, j, 1);
-        l2 += hsum_x(fbx, g, i, j, 2) + hsum_y(fby, g, i, j, 2);
-        zb += hsum_x(fbx, g, i, j, 3) + hsum_y(fby, g, i, j, 3);
-        if (0.0 < p.x) { eb += he; zb -= he; }
-    }
-    if (i <= 1 || i >= g.nx - 2) {
-        const int gx[4] = {-2, -1, g.nx, g.nx + 1};
-        for (int t = 0; t < 4; ++t) {
-            int src, edge;
-            bool flip;
-            map1(gx[t], g.nx, g.bl0, g.bh0, src, edge, flip);
-            if (src != i && edge != i) continue;
-            const PC p = pcell(s, g, gx[t], j);
-            const double he = hsum_x(fbx, g, gx[t], j, 0);
-            if (src == i) {
-                const double hu = hsum_x(fbx, g, gx[t], j, 1);
-                l1 += flip ? -hu : hu;
-                l2 += hsum_x(fbx, g, gx[t], j, 2);
-                if (0.0 < p.x) eb += he;
+        const double r = dt / dx;
+        constexpr int nf = (kTX + 3) * kTY;
+        for (int base = 0; base < nf; base += kThreads) {
+            const int q = base + t;
+            double cb[16];
+            bool have = false;
+            int lx0 = 0, ly = 0;
+            if (q < nf) {
+                const int row = q / (kTX + 3), fp = q - row * (kTX + 3);
+                const int j = j0 + row, f = i0 - 1 + fp;
+                if (j < g.ny && f >= 0 && f <= g.nx) {
+                    double c[16];
+                    for (int k = 0; k < 4; ++k) put4(c + 4 * k, pcell(s, g, f - 2 + k, j));
+                    double mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0};
+                    const int64_t cl = (int64_t)j * g.nx + f - 1;
+                    if (f >= 1)
+                        for (int k = 0; k < 3; ++k) mL[k] = -r * lun[k * N + cl];
+                    if (f < g.nx)
+                        for (int k = 0; k < 3; ++k) mR[k] = r * lun[k * N + cl + 1];
+                    hydro_face_vjp(c, 1.0, 0.0, mL, mR, cb);
+                    have = true;
+                    lx0 = fp - 1;        // slot k lands on x index f - 2 + k
+                    ly = row + 2;
+                }
             }
-            if (edge == i) {
-                zb += hsum_x(fbx, g, gx[t], j, 3);
-                if (0.0 < p.x) zb -= he;
-            }
-        }
-    }
-    if (j <= 1 || j >= g.ny - 2) {
-        const int gy[4] = {-2, -1, g.ny, g.ny + 1};
-        for (int t = 0; t < 4; ++t) {
-            int src, edge;
-            bool flip;
-            map1(gy[t], g.ny, g.bl1, g.bh1, src, edge, flip);
-            if (src != j && edge != j) continue;
-            const PC p = pcell(s, g, i, gy[t]);
-            const double he = hsum_y(fby, g, i, gy[t], 0);
-            if (src == j) {
-                l1 += hsum_y(fby, g, i, gy[t], 1);
-                const double hv = hsum_y(fby, g, i, gy[t], 2);
-                l2 += flip ? -hv : hv;
-                if (0.0 < p.x) eb += he;
-            }
-            if (edge == j) {
-                zb += hsum_y(fby, g, i, gy[t], 3);
-                if (0.0 < p.x) zb -= he;
+            for (int k = 0; k < 4; ++k) {
+                const int lx = lx0 + k;
+                if (have && lx >= 0 && lx < kHPX)
+                    for (int m = 0; m < 4; ++m) acc[m][ly][lx] += cb[4 * k + m];
+                __syncthreads();
             }
         }
     }
-    lam_new[c] = l0 + eb;         // eta = h + z
-    lam_new[N + c] = l1;
-    lam_new[2 * N + c] = l2;
-    gz[c] += eb + zb;
+    // y faces f in [j0 - 1, j0 + kTY + 2) of the tile columns
+    {
+        const double r = dt / dy;
+        constexpr int nf = (kTY + 3) * kTX;
+        for (int base = 0; base < nf; base += kThreads) {
+            const int q = base + t;
+            double cb[16];
+            bool have = false;
+            int ly0 = 0, lx = 0;
+            if (q < nf) {
+                const int gp = q / kTX, cp = q - gp * kTX;
+                const int i = i0 + cp, f = j0 - 1 + gp;
+                if (i < g.nx && f >= 0 && f <= g.ny) {
+                    double c[16];
+                    for (int k = 0; k < 4; ++k) put4(c + 4 * k, pcell(s, g, i, f - 2 + k));
+                    double mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0};
+                    if (f >= 1)
+                        for (int k = 0; k < 3; ++k) mL[k] = -r * lun[k * N + (int64_t)(f - 1) * g.nx + i];
+                    if (f < g.ny)
+                        for (int k = 0; k < 3; ++k) mR[k] = r * lun[k * N + (int64_t)f * g.nx + i];
+                    hydro_face_vjp(c, 0.0, 1.0, mL, mR, cb);
+                    have = true;
+                    ly0 = gp - 1;
+                    lx = cp + 2;
+                }
+            }
+            for (int k = 0; k < 4; ++k) {
+                const int ly = ly0 + k;
+                if (have && ly >= 0 && ly < kHPY)
+                    for (int m = 0; m < 4; ++m) acc[m][ly][lx] += cb[4 * k + m];
+                __syncthreads();
+            }
+        }
+    }
+    // gather
+    for (int k = t; k < kTX * kTY; k += kThreads) {
+        const int ci = k % kTX, cj = k / kTX;
+        const int i = i0 + ci, j = j0 + cj;
+        if (i >= g.nx || j >= g.ny) continue;
+        const int64_t c = (int64_t)j * g.nx + i;
+        double l0 = lun[c], l1 = lun[N + c], l2 = lun[2 * N + c];   // stored-state path
+        double eb = 0.0, zb = 0.0;
+        {
+            const PC p = pcell(s, g, i, j);
+            const double he = acc[0][cj + 2][ci + 2];
+            l1 += acc[1][cj + 2][ci + 2];
+            l2 += acc[2][cj + 2][ci + 2];
+            zb += acc[3][cj + 2][ci + 2];
+            if (0.0 < p.x) { eb += he; zb -= he; }
+        }
+        if (i <= 1 || i >= g.nx - 2) {
+            const int gx[4] = {-2, -1, g.nx, g.nx + 1};
+            for (int u = 0; u < 4; ++u) {
+                int src, edge;
+                bool flip;
+                map1(gx[u], g.nx, g.bl0, g.bh0, src, edge, flip);
+                if (src != i && edge != i) continue;
+                const int lx = gx[u] - (i0 - 2);
+                const PC p = pcell(s, g, gx[u], j);
+                const double he = acc[0][cj + 2][lx];
+                if (src == i) {
+                    const double hu = acc[1][cj + 2][lx];
+                    l1 += flip ? -hu : hu;
+                    l2 += acc[2][cj + 2][lx];
+                    if (0.0 < p.x) eb += he;
+                }
+                if (edge == i) {
+                    zb += acc[3][cj + 2][lx];
+                    if (0.0 < p.x) zb -= he;
+                }
+            }
+        }
+        if (j <= 1 || j >= g.ny - 2) {
+            const int gy[4] = {-2, -1, g.ny, g.ny + 1};
+            for (int u = 0; u < 4; ++u) {
+                int src, edge;
+                bool flip;
+                map1(gy[u], g.ny, g.bl1, g.bh1, src, edge, flip);
+                if (src != j && edge != j) continue;
+                const int ly = gy[u] - (j0 - 2);
+                const PC p = pcell(s, g, i, gy[u]);
+                const double he = acc[0][ly][ci + 2];
+                if (src == j) {
+                    l1 += acc[1][ly][ci + 2];
+                    const double hv = acc[2][ly][ci + 2];
+                    l2 += flip ? -hv : hv;
+                    if (0.0 < p.x) eb += he;
+                }
+                if (edge == j) {
+                    zb += acc[3][ly][ci + 2];
+                    if (0.0 < p.x) zb -= he;
+                }
+            }
+        }
+        lam_new[c] = l0 + eb;         // eta = h + z
+        lam_new[N + c] = l1;
+        lam_new[2 * N + c] = l2;
+        gz[c] += eb + zb;
+    }
 }
 
-// ---------------------------------------------------------------------------
-// Plain Roe step (src/solvers/Roe.cpp:250-434)
-// Face records: 10 planes, slot q = 0 (left cell), 1 (right cell), field
-// m = h, hu, hv, z, n: plane q*5+m.
-// ---------------------------------------------------------------------------
-template <int DIR>
-__global__ __launch_bounds__(kThreads) void rface_kernel(State s, Geo g, double dt, double ds,
+// Plain Roe: two-cell faces; LDS image covers [i0 - 1, i0 + kTX + 1) (the
+// high ghost n folds into n - 1, which is always in the tile holding it).
+constexpr int kRPX = kTX + 2, kRPY = kTY + 2;
+
+__global__ __launch_bounds__(kThreads) void rtile_kernel(State s, Geo g, double dt, double dx,
+                                                         double dy,
                                                          const double* __restrict__ lam,
-                                                         double* __restrict__ fb)
+                                                         double* __restrict__ lam_new,
+                                                         double* __restrict__ gz,
+                                                         double* __restrict__ gn)
 {
+    __shared__ double acc[5][kRPY][kRPX];
     const int64_t N = (int64_t)g.nx * g.ny;
-    const int64_t nf = DIR == 0 ? (int64_t)(g.nx + 1) * g.ny : (int64_t)g.nx * (g.ny + 1);
-    const int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (idx >= nf) return;
-    int f, i, j, n;
-    if (DIR == 0) {
-        j = (int)(idx / (g.nx + 1));
-        f = (int)(idx - (int64_t)j * (g.nx + 1));
-        i = f;
-        n = g.nx;
-    } else {
-        f = (int)(idx / g.nx);
-        i = (int)(idx - (int64_t)f * g.nx);
-        j = f;
-        n = g.ny;
-    }
-    const PC a = DIR == 0 ? pcell(s, g, f - 1, j) : pcell(s, g, i, f - 1);
-    const PC b = DIR == 0 ? pcell(s, g, f, j) : pcell(s, g, i, f);
-    const double L[5] = {a.he, a.hu, a.hv, a.z, a.n};
-    const double R[5] = {b.he, b.hu, b.hv, b.z, b.n};
-    const double r = dt / ds;
-    const int64_t cl = DIR == 0 ? (int64_t)j * g.nx + (f - 1) : (int64_t)(f - 1) * g.nx + i;
-    const int64_t cr = DIR == 0 ? (int64_t)j * g.nx + f : (int64_t)f * g.nx + i;
-    double lL[3] = {0.0, 0.0, 0.0}, lR[3] = {0.0, 0.0, 0.0};
-    if (f >= 1)
-        for (int k = 0; k < 3; ++k) lL[k] = lam[k * N + cl];
-    if (f < n)
-        for (int k = 0; k < 3; ++k) lR[k] = lam[k * N + cr];
+    const int i0 = blockIdx.x * kTX, j0 = blockIdx.y * kTY;
+    const int t = threadIdx.x;
+    for (int k = t; k < 5 * kRPY * kRPX; k += kThreads) (&acc[0][0][0])[k] = 0.0;
+    __syncthreads();
     // un(c) = uo(c) - r (F[xr] - F[xl]) - r (Sp[xr] + Sm[xl])
-    double Fb[3], Smb[3], Spb[3], Lb[5], Rb[5];
-    for (int k = 0; k < 3; ++k) {
-        Fb[k] = r * lR[k] - r * lL[k];
-        Spb[k] = -r * lL[k];
-        Smb[k] = -r * lR[k];
-    }
-    roe_face_vjp(L, R, ds, DIR == 0 ? 1.0 : 0.0, DIR == 0 ? 0.0 : 1.0, Fb, Smb, Spb, Lb, Rb);
-    for (int m = 0; m < 5; ++m) {
-        fb[m * nf + idx] = Lb[m];
-        fb[(5 + m) * nf + idx] = Rb[m];
-    }
-}
-
-__device__ __forceinline__ double rsum_x(const double* fb, const Geo& g, int px, int j, int m)
-{
-    const int64_t nf = (int64_t)(g.nx + 1) * g.ny;
-    double acc = 0.0;
-    for (int f = px + 1; f >= px; --f) {
-        if (f < 0 || f > g.nx) continue;
-        const int q = px - f + 1;
-        acc += fb[(q * 5 + m) * nf + (int64_t)j * (g.nx + 1) + f];
-    }
-    return acc;
-}
-
-__device__ __forceinline__ double rsum_y(const double* fb, const Geo& g, int i, int py, int m)
-{
-    const int64_t nf = (int64_t)g.nx * (g.ny + 1);
-    double acc = 0.0;
-    for (int f = py + 1; f >= py; --f) {
-        if (f < 0 || f > g.ny) continue;
-        const int q = py - f + 1;
-        acc += fb[(q * 5 + m) * nf + (int64_t)f * g.nx + i];
-    }
-    return acc;
-}
-
-__global__ __launch_bounds__(kThreads) void rgather_kernel(State s, Geo g,
-                                                           const double* __restrict__ lam,
-                                                           const double* __restrict__ fbx,
-                                                           const double* __restrict__ fby,
-                                                           double* __restrict__ lam_new,
-                                                           double* __restrict__ gz,
-                                                           double* __restrict__ gn)
-{
-    const int64_t N = (int64_t)g.nx * g.ny;
-    const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (c >= N) return;
-    const int j = (int)(c / g.nx), i = (int)(c - (int64_t)j * g.nx);
-    // the update starts from the rebuilt depth and the stored momenta
-    double l1 = lam[N + c], l2 = lam[2 * N + c];
-    double eb = 0.0, zb = 0.0, nb = 0.0;
-    {
-        const PC p = pcell(s, g, i, j);
-        const double he = lam[c] + rsum_x(fbx, g, i, j, 0) + rsum_y(fby, g, i, j, 0);
-        l1 += rsum_x(fbx, g, i, j, 1) + rsum_y(fby, g, i, j, 1);
-        l2 += rsum_x(fbx, g, i, j, 2) + rsum_y(fby, g, i, j, 2);
-        zb += rsum_x(fbx, g, i, j, 3) + rsum_y(fby, g, i, j, 3);
-        nb += rsum_x(fbx, g, i, j, 4) + rsum_y(fby, g, i, j, 4);
-        if (0.0 < p.x) { eb += he; zb -= he; }
-    }
-    if (i == 0 || i == g.nx - 1) {
-        const int gx[2] = {-1, g.nx};
-        for (int t = 0; t < 2; ++t) {
-            int src, edge;
-            bool flip;
-            map1(gx[t], g.nx, g.bl0, g.bh0, src, edge, flip);
-            if (src != i && edge != i) continue;
-            const PC p = pcell(s, g, gx[t], j);
-            const double he = rsum_x(fbx, g, gx[t], j, 0);
-            if (src == i) {
-                const double hu = rsum_x(fbx, g, gx[t], j, 1);
-                l1 += flip ? -hu : hu;
-                l2 += rsum_x(fbx, g, gx[t], j, 2);
-                if (0.0 < p.x) eb += he;
+    for (int dir = 0; dir < 2; ++dir) {
+        const double ds = dir == 0 ? dx : dy;
+        const double r = dt / ds;
+        const int nf = dir == 0 ? (kTX + 1) * kTY : (kTY + 1) * kTX;
+        for (int base = 0; base < nf; base += kThreads) {
+            const int q = base + t;
+            double Lb[5], Rb[5];
+            bool have = false;
+            int lxL = 0, lyL = 0, lxR = 0, lyR = 0;
+            if (q < nf) {
+                int i, j, f, n;
+                int64_t cl, cr;
+                if (dir == 0) {
+                    const int row = q / (kTX + 1), fp = q - row * (kTX + 1);
+                    j = j0 + row;
+                    f = i0 + fp;
+                    i = f;
+                    n = g.nx;
+                    cl = (int64_t)j * g.nx + f - 1;
+                    cr = cl + 1;
+                    lxL = fp; lyL = row + 1; lxR = fp + 1; lyR = row + 1;
+                } else {
+                    const int gp = q / kTX, cp = q - gp * kTX;
+                    i = i0 + cp;
+                    f = j0 + gp;
+                    j = f;
+                    n = g.ny;
+                    cl = (int64_t)(f - 1) * g.nx + i;
+                    cr = (int64_t)f * g.nx + i;
+                    lxL = cp + 1; lyL = gp; lxR = cp + 1; lyR = gp + 1;
+                }
+                const bool inside = dir == 0 ? (j < g.ny && f <= g.nx) : (i < g.nx && f <= g.ny);
+                if (inside) {
+                    const PC a = dir == 0 ? pcell(s, g, f - 1, j) : pcell(s, g, i, f - 1);
+                    const PC b = dir == 0 ? pcell(s, g, f, j) : pcell(s, g, i, f);
+                    const double L[5] = {a.he, a.hu, a.hv, a.z, a.n};
+                    const double R[5] = {b.he, b.hu, b.hv, b.z, b.n};
+                    double lL[3] = {0.0, 0.0, 0.0}, lR[3] = {0.0, 0.0, 0.0};
+                    if (f >= 1)
+                        for (int k = 0; k < 3; ++k) lL[k] = lam[k * N + cl];
+                    if (f < n)
+                        for (int k = 0; k < 3; ++k) lR[k] = lam[k * N + cr];
+                    double Fb[3], Smb[3], Spb[3];
+                    for (int k = 0; k < 3; ++k) {
+                        Fb[k] = r * lR[k] - r * lL[k];
+                        Spb[k] = -r * lL[k];
+                        Smb[k] = -r * lR[k];
+                    }
+                    roe_face_vjp(L, R, ds, dir == 0 ? 1.0 : 0.0, dir == 0 ? 0.0 : 1.0, Fb, Smb,
+                                 Spb, Lb, Rb);
+                    have = true;
+                }
             }
-            if (edge == i) {
-                zb += rsum_x(fbx, g, gx[t], j, 3);
-                nb += rsum_x(fbx, g, gx[t], j, 4);
-                if (0.0 < p.x) zb -= he;
-            }
+            if (have)
+                for (int m = 0; m < 5; ++m) acc[m][lyL][lxL] += Lb[m];
+            __syncthreads();
+            if (have)
+                for (int m = 0; m < 5; ++m) acc[m][lyR][lxR] += Rb[m];
+            __syncthreads();
         }
     }
-    if (j == 0 || j == g.ny - 1) {
-        const int gy[2] = {-1, g.ny};
-        for (int t = 0; t < 2; ++t) {
-            int src, edge;
-            bool flip;
-            map1(gy[t], g.ny, g.bl1, g.bh1, src, edge, flip);
-            if (src != j && edge != j) continue;
-            const PC p = pcell(s, g, i, gy[t]);
-            const double he = rsum_y(fby, g, i, gy[t], 0);
-            if (src == j) {
-                l1 += rsum_y(fby, g, i, gy[t], 1);
-                const double hv = rsum_y(fby, g, i, gy[t], 2);
-                l2 += flip ? -hv : hv;
-                if (0.0 < p.x) eb += he;
-            }
-            if (edge == j) {
-                zb += rsum_y(fby, g, i, gy[t], 3);
-                nb += rsum_y(fby, g, i, gy[t], 4);
-                if (0.0 < p.x) zb -= he;
+    for (int k = t; k < kTX * kTY; k += kThreads) {
+        const int ci = k % kTX, cj = k / kTX;
+        const int i = i0 + ci, j = j0 + cj;
+        if (i >= g.nx || j >= g.ny) continue;
+        const int64_t c = (int64_t)j * g.nx + i;
+        // the update starts from the rebuilt depth and the stored momenta
+        double l1 = lam[N + c], l2 = lam[2 * N + c];
+        double eb = 0.0, zb = 0.0, nb = 0.0;
+        {
+            const PC p = pcell(s, g, i, j);
+            const double he = lam[c] + acc[0][cj + 1][ci + 1];
+            l1 += acc[1][cj + 1][ci + 1];
+            l2 += acc[2][cj + 1][ci + 1];
+            zb += acc[3][cj + 1][ci + 1];
+            nb += acc[4][cj + 1][ci + 1];
+            if (0.0 < p.x) { eb += he; zb -= he; }
+        }
+        if (i == 0 || i == g.nx - 1) {
+            const int gx[2] = {-1, g.nx};
+            for (int u = 0; u < 2; ++u) {
+                int src, edge;
+                bool flip;
+                map1(gx[u], g.nx, g.bl0, g.bh0, src, edge, flip);
+                if (src != i && edge != i) continue;
+                const int lx = gx[u] - (i0 - 1);
+                const PC p = pcell(s, g, gx[u], j);
+                const double he = acc[0][cj + 1][lx];
+                if (src == i) {
+                    const double hu = acc[1][cj + 1][lx];
+                    l1 += flip ? -hu : hu;
+                    l2 += acc[2][cj + 1][lx];
+                    if (0.0 < p.x) eb += he;
+                }
+                if (edge == i) {
+                    zb += acc[3][cj + 1][lx];
+                    nb += acc[4][cj + 1][lx];
+                    if (0.0 < p.x) zb -= he;
+                }
             }
         }
+        if (j == 0 || j == g.ny - 1) {
+            const int gy[2] = {-1, g.ny};
+            for (int u = 0; u < 2; ++u) {
+                int src, edge;
+                bool flip;
+                map1(gy[u], g.ny, g.bl1, g.bh1, src, edge, flip);
+                if (src != j && edge != j) continue;
+                const int ly = gy[u] - (j0 - 1);
+                const PC p = pcell(s, g, i, gy[u]);
+                const double he = acc[0][ly][ci + 1];
+                if (src == j) {
+                    l1 += acc[1][ly][ci + 1];
+                    const double hv = acc[2][ly][ci + 1];
+                    l2 += flip ? -hv : hv;
+                    if (0.0 < p.x) eb += he;
+                }
+                if (edge == j) {
+                    zb += acc[3][ly][ci + 1];
+                    nb += acc[4][ly][ci + 1];
+                    if (0.0 < p.x) zb -= he;
+                }
+            }
+        }
+        lam_new[c] = eb;
+        lam_new[N + c] = l1;
+        lam_new[2 * N + c] = l2;
+        gz[c] += eb + zb;
+        gn[c] += nb;
     }
-    lam_new[c] = eb;
-    lam_new[N + c] = l1;
-    lam_new[2 * N + c] = l2;
-    gz[c] += eb + zb;
-    gn[c] += nb;
 }
 
 // ---------------------------------------------------------------------------
@@ -459,8 +495,6 @@ struct hx_adjoint {
     double* seg;       // nseg states: recomputed states inside a segment
     double* lam;       // 2 states: adjoint ping-pong
     double* lun;       // 1 state: adjoint before friction (hydrostatic)
-    double* fbx;       // x-face records
-    double* fby;       // y-face records
     std::vector<double> dts;
     const double* z;
     const double* n;
@@ -469,10 +503,10 @@ struct hx_adjoint {
 
 static void adj_free(hx_adjoint* a)
 {
-    double* bufs[] = {a->ckpt, a->seg, a->lam, a->lun, a->fbx, a->fby};
+    double* bufs[] = {a->ckpt, a->seg, a->lam, a->lun};
     for (double* b : bufs)
-        if (b) hipFree(b);
-    if (a->gauges) hipFree(a->gauges);
+        if (b) (void)hipFree(b);
+    if (a->gauges) (void)hipFree(a->gauges);
 }
 
 extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, int nsteps,
@@ -513,13 +547,9 @@ extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, in
     a->ngauge = ngauge;
     a->N = N;
     const size_t S = sizeof(double) * 3 * (size_t)N;
-    const int nrec = (solver == HX_SOLVER_ROE) ? 10 : 16;
-    const size_t nfx = (size_t)(a->nx + 1) * a->ny, nfy = (size_t)a->nx * (a->ny + 1);
     bool ok = hipMalloc(&a->ckpt, S * a->ncp) == hipSuccess &&
               hipMalloc(&a->seg, S * a->nseg) == hipSuccess &&
-              hipMalloc(&a->lam, 2 * S) == hipSuccess &&
-              hipMalloc(&a->fbx, sizeof(double) * nrec * nfx) == hipSuccess &&
-              hipMalloc(&a->fby, sizeof(double) * nrec * nfy) == hipSuccess;
+              hipMalloc(&a->lam, 2 * S) == hipSuccess;
     if (ok && solver == HX_SOLVER_HYDROSTATIC_ROE) ok = hipMalloc(&a->lun, S) == hipSuccess;
     if (ok && ngauge > 0) {
         ok = hipMalloc(&a->gauges, sizeof(int64_t) * ngauge) == hipSuccess &&
@@ -539,7 +569,7 @@ extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, in
 extern "C" int hx_adjoint_destroy(hx_adjoint* a)
 {
     if (!a) return HX_OK;
-    hipSetDevice(a->ctx->device);
+    (void)hipSetDevice(a->ctx->device);
     adj_free(a);
     delete a;
     return HX_OK;
@@ -667,24 +697,15 @@ static int adj_step(hx_adjoint* a, const double* U, double dt, const double* lam
     const int64_t N = a->N;
     State s{U, U + N, U + 2 * N, a->z, a->n};
     Geo g{a->nx, a->ny, a->bc_lo[0], a->bc_lo[1], a->bc_hi[0], a->bc_hi[1]};
-    const int64_t nfx = (int64_t)(a->nx + 1) * a->ny, nfy = (int64_t)a->nx * (a->ny + 1);
+    const dim3 tiles((unsigned)((a->nx + kTX - 1) / kTX), (unsigned)((a->ny + kTY - 1) / kTY));
     if (a->solver == HX_SOLVER_HYDROSTATIC_ROE) {
         hipLaunchKernelGGL(hpost_kernel, dim3(nblocks(N)), dim3(kThreads), 0, hs, s, g, dt, a->dx,
                            a->dy, lam, a->lun, gn);
-        hipLaunchKernelGGL(hface_kernel<0>, dim3(nblocks(nfx)), dim3(kThreads), 0, hs, s, g, dt,
-                           a->dx, (const double*)a->lun, a->fbx);
-        hipLaunchKernelGGL(hface_kernel<1>, dim3(nblocks(nfy)), dim3(kThreads), 0, hs, s, g, dt,
-                           a->dy, (const double*)a->lun, a->fby);
-        hipLaunchKernelGGL(hgather_kernel, dim3(nblocks(N)), dim3(kThreads), 0, hs, s, g,
-                           (const double*)a->lun, (const double*)a->fbx, (const double*)a->fby,
-                           lam_new, gz);
+        hipLaunchKernelGGL(htile_kernel, tiles, dim3(kThreads), 0, hs, s, g, dt, a->dx, a->dy,
+                           (const double*)a->lun, lam_new, gz);
     } else {
-        hipLaunchKernelGGL(rface_kernel<0>, dim3(nblocks(nfx)), dim3(kThreads), 0, hs, s, g, dt,
-                           a->dx, lam, a->fbx);
-        hipLaunchKernelGGL(rface_kernel<1>, dim3(nblocks(nfy)), dim3(kThreads), 0, hs, s, g, dt,
-                           a->dy, lam, a->fby);
-        hipLaunchKernelGGL(rgather_kernel, dim3(nblocks(N)), dim3(kThreads), 0, hs, s, g, lam,
-                           (const double*)a->fbx, (const double*)a->fby, lam_new, gz, gn);
+        hipLaunchKernelGGL(rtile_kernel, tiles, dim3(kThreads), 0, hs, s, g, dt, a->dx, a->dy,
+                           lam, lam_new, gz, gn);
     }
     return host::hip_status(hipGetLastError());
 }

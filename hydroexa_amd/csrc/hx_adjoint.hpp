@@ -12,6 +12,14 @@
 // with a wet test (h > 0) contribute nothing when the test fails, so no
 // 0 * inf appears for dry states.
 //
+// Arithmetic: a derivative needs relative accuracy, not correct rounding,
+// so every denominator is inverted once (hardware reciprocal + two Newton
+// steps) and shared by the forward re-evaluation and the reverse sweep;
+// square roots and their reciprocals come from one hardware rsq refined by
+// two coupled Goldschmidt steps.  The branch tests of the positivity guard
+// keep correctly rounded divisions so that the adjoint takes the branches
+// the forward took.
+//
 // Forward expressions restated (reference file:line):
 //   Roe face            include/solvers/kernels/Roe.H:50-309
 //   homogeneous Roe     include/solvers/kernels/HydrostaticRoe.H:43-159
@@ -26,28 +34,55 @@ namespace hx {
 namespace adj {
 
 constexpr double kG = 9.81;        // include/utils/Constants.H:25
+constexpr double kSqrtG = 3.132091952673165;
 constexpr double kDry = 1e-6;      // include/utils/Constants.H:63
 constexpr double kT12 = 1e-12;     // include/utils/Constants.H:70
 constexpr double kT10 = 1e-10;     // include/utils/Constants.H:78
 
-__host__ __device__ inline double sgn0(double x) { return x >= 0.0 ? 1.0 : -1.0; }
-__host__ __device__ inline double fabs0(double x) { return x >= 0.0 ? x : -x; }
+__device__ __forceinline__ double sgn0(double x) { return x >= 0.0 ? 1.0 : -1.0; }
+__device__ __forceinline__ double fabs0(double x) { return x >= 0.0 ? x : -x; }
+
+// 1/b for finite non-zero b: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    return fma(r, e, r);
+}
+
+// sqrt(x) and 1/sqrt(x) for x > 0: hardware rsq and two coupled Goldschmidt
+// steps (g -> sqrt(x), h -> 1/(2 sqrt(x))).
+__device__ __forceinline__ void sqrt_rsq(double x, double& sq, double& rs)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    sq = g;
+    rs = 2.0 * h;
+}
 
 // ---------------------------------------------------------------------------
 // Shared Roe core: dry cut, Roe averages, entropy-fixed eigenvalues, wave
 // strengths and the unrotated flux after the wave sum.
 // ---------------------------------------------------------------------------
 struct RoeTape {
-    double hl, hr, sql, sqr, ul, vl, ur, vr, unl, utl, unr, utr;
-    double ht, ct, dsq, ut, vt, cgl, cgr;
-    double el0, er0, el2, er2, lam0_0, lam2_0;
-    double lam0, lam1, lam2, den, dh, a0, a1, a2;
+    double hl, hr, sql, sqr, rsl, rsr, ul, vl, ur, vr, unl, utl, unr, utr;
+    double ht, ct, rct, dsq, idsq, ut, vt, cgl, cgr;
+    double el0, er0, el2, er2, lam0_0, lam2_0, iD0, iD2;
+    double lam0, lam1, lam2, den, iden, dh, a0, a1, a2;
     double w0, w1, w2;
     double F0, F1, F2;        // unrotated, after the wave sum
     bool cutl, cutr, htact, fix0, fix2;
 };
 
-__host__ __device__ inline void roe_core_fwd(double hl, double hul, double hvl, double hr,
+__device__ __forceinline__ void roe_core_fwd(double hl, double hul, double hvl, double hr,
                                              double hur, double hvr, double nx, double ny,
                                              RoeTape& t)
 {
@@ -55,12 +90,18 @@ __host__ __device__ inline void roe_core_fwd(double hl, double hul, double hvl, 
     t.cutr = hr >= kDry;
     t.hl = hl = t.cutl ? hl : 0.0;
     t.hr = hr = t.cutr ? hr : 0.0;
-    t.sql = (hl > 0.0) ? sqrt(hl) : 0.0;
-    t.sqr = (hr > 0.0) ? sqrt(hr) : 0.0;
-    t.ul = (hl > 0.0) ? hul / hl : 0.0;
-    t.vl = (hl > 0.0) ? hvl / hl : 0.0;
-    t.ur = (hr > 0.0) ? hur / hr : 0.0;
-    t.vr = (hr > 0.0) ? hvr / hr : 0.0;
+    double s, r;
+    sqrt_rsq(hl > 0.0 ? hl : 1.0, s, r);
+    t.sql = hl > 0.0 ? s : 0.0;
+    t.rsl = hl > 0.0 ? r : 0.0;
+    sqrt_rsq(hr > 0.0 ? hr : 1.0, s, r);
+    t.sqr = hr > 0.0 ? s : 0.0;
+    t.rsr = hr > 0.0 ? r : 0.0;
+    const double ihl = t.rsl * t.rsl, ihr = t.rsr * t.rsr;
+    t.ul = hul * ihl;
+    t.vl = hvl * ihl;
+    t.ur = hur * ihr;
+    t.vr = hvr * ihr;
     t.unl = t.ul * nx + t.vl * ny;
     t.utl = -t.ul * ny + t.vl * nx;
     t.unr = t.ur * nx + t.vr * ny;
@@ -68,30 +109,34 @@ __host__ __device__ inline void roe_core_fwd(double hl, double hul, double hvl, 
     const double hmean = 0.5 * (hl + hr);
     t.htact = hmean > kDry;
     t.ht = t.htact ? hmean : kDry;
-    t.ct = sqrt(kG * t.ht);
+    sqrt_rsq(kG * t.ht, t.ct, t.rct);
     t.dsq = t.sql + t.sqr + kT12;
-    t.ut = (t.unl * t.sql + t.unr * t.sqr) / t.dsq;
-    t.vt = (t.utl * t.sql + t.utr * t.sqr) / t.dsq;
+    t.idsq = rcp(t.dsq);
+    t.ut = (t.unl * t.sql + t.unr * t.sqr) * t.idsq;
+    t.vt = (t.utl * t.sql + t.utr * t.sqr) * t.idsq;
     t.lam0_0 = t.ut - t.ct;
     t.lam1 = t.ut;
     t.lam2_0 = t.ut + t.ct;
-    t.cgl = sqrt(kG * hl);
-    t.cgr = sqrt(kG * hr);
+    t.cgl = kSqrtG * t.sql;
+    t.cgr = kSqrtG * t.sqr;
     t.el0 = t.unl - t.cgl;
     t.er0 = t.unr - t.cgr;
     t.fix0 = (t.el0 < 0.0 && t.er0 > 0.0);
-    t.lam0 = t.fix0 ? t.el0 * (t.er0 - t.lam0_0) / (t.er0 - t.el0) : t.lam0_0;
+    t.iD0 = t.fix0 ? rcp(t.er0 - t.el0) : 0.0;
+    t.lam0 = t.fix0 ? t.el0 * (t.er0 - t.lam0_0) * t.iD0 : t.lam0_0;
     t.el2 = t.unl + t.cgl;
     t.er2 = t.unr + t.cgr;
     t.fix2 = (t.el2 < 0.0 && t.er2 > 0.0);
-    t.lam2 = t.fix2 ? t.er2 * (t.lam2_0 - t.el2) / (t.er2 - t.el2) : t.lam2_0;
+    t.iD2 = t.fix2 ? rcp(t.er2 - t.el2) : 0.0;
+    t.lam2 = t.fix2 ? t.er2 * (t.lam2_0 - t.el2) * t.iD2 : t.lam2_0;
     t.den = 2.0 * t.ct + kT12;
+    t.iden = rcp(t.den);
     t.dh = hr - hl;
     const double dhu = hr * t.unr - hl * t.unl;
     const double dhv = hr * t.utr - hl * t.utl;
-    t.a0 = (t.lam2 * t.dh - dhu) / t.den;
-    t.a1 = (-t.vt * t.dh + dhv) / t.ct;
-    t.a2 = (-t.lam0 * t.dh + dhu) / t.den;
+    t.a0 = (t.lam2 * t.dh - dhu) * t.iden;
+    t.a1 = (-t.vt * t.dh + dhv) * t.rct;
+    t.a2 = (-t.lam0 * t.dh + dhu) * t.iden;
     double F0 = 0.5 * (hl * t.unl + hr * t.unr);
     double F1 = 0.5 * (hl * t.unl * t.unl + 0.5 * kG * hl * hl + hr * t.unr * t.unr +
                        0.5 * kG * hr * hr);
@@ -119,7 +164,7 @@ struct RoeBar {
 
 // Reverse sweep of roe_core_fwd.  Fb: adjoint of the unrotated flux; A:
 // extra adjoints of intermediates.  Out: adjoints of the six inputs.
-__host__ __device__ inline void roe_core_rev(const RoeTape& t, double nx, double ny, double Fb0,
+__device__ __forceinline__ void roe_core_rev(const RoeTape& t, double nx, double ny, double Fb0,
                                              double Fb1, double Fb2, RoeBar A, double* hlb_o,
                                              double* hulb, double* hvlb, double* hrb_o,
                                              double* hurb, double* hvrb)
@@ -157,18 +202,18 @@ __host__ __device__ inline void roe_core_rev(const RoeTape& t, double nx, double
     // wave strengths
     double dhb = 0.0, dhub = 0.0, dhvb = 0.0;
     {
-        double nb = a0b / t.den;
-        denb += -a0b * t.a0 / t.den;
+        double nb = a0b * t.iden;
+        denb += -nb * t.a0;
         lam2b += nb * t.dh;
         dhb += nb * t.lam2;
         dhub -= nb;
-        nb = a1b / t.ct;
-        ctb += -a1b * t.a1 / t.ct;
+        nb = a1b * t.rct;
+        ctb += -nb * t.a1;
         vtb += -nb * t.dh;
         dhb += -nb * t.vt;
         dhvb += nb;
-        nb = a2b / t.den;
-        denb += -a2b * t.a2 / t.den;
+        nb = a2b * t.iden;
+        denb += -nb * t.a2;
         lam0b += -nb * t.dh;
         dhb += -nb * t.lam0;
         dhub += nb;
@@ -187,20 +232,22 @@ __host__ __device__ inline void roe_core_rev(const RoeTape& t, double nx, double
     // entropy fix (lam2, then lam0)
     double cglb = 0.0, cgrb = 0.0, lam2_0b = lam2b, lam0_0b = lam0b;
     if (t.fix2) {
-        const double D = t.er2 - t.el2, M = t.lam2_0 - t.el2, L = lam2b;
-        const double er2b = L * M / D - L * t.er2 * M / (D * D);
-        const double el2b = -L * t.er2 / D + L * t.er2 * M / (D * D);
-        lam2_0b = L * t.er2 / D;
+        const double M = t.lam2_0 - t.el2, L = lam2b * t.iD2;      // lam2 = er2 M / D
+        const double q = L * t.er2 * M * t.iD2;
+        const double er2b = L * M - q;
+        const double el2b = -L * t.er2 + q;
+        lam2_0b = L * t.er2;
         unlb += el2b;
         cglb += el2b;
         unrb += er2b;
         cgrb += er2b;
     }
     if (t.fix0) {
-        const double D = t.er0 - t.el0, M = t.er0 - t.lam0_0, L = lam0b;
-        const double el0b = L * M / D + L * t.el0 * M / (D * D);
-        const double er0b = L * t.el0 / D - L * t.el0 * M / (D * D);
-        lam0_0b = -L * t.el0 / D;
+        const double M = t.er0 - t.lam0_0, L = lam0b * t.iD0;      // lam0 = el0 M / D
+        const double q = L * t.el0 * M * t.iD0;
+        const double el0b = L * M + q;
+        const double er0b = L * t.el0 - q;
+        lam0_0b = -L * t.el0;
         unlb += el0b;
         cglb -= el0b;
         unrb += er0b;
@@ -211,14 +258,14 @@ __host__ __device__ inline void roe_core_rev(const RoeTape& t, double nx, double
     // Roe averages
     double sqlb = 0.0, sqrb = 0.0;
     {
-        double nb = utb / t.dsq;
-        double dsqb = -utb * t.ut / t.dsq;
+        double nb = utb * t.idsq;
+        double dsqb = -nb * t.ut;
         unlb += nb * t.sql;
         sqlb += nb * t.unl;
         unrb += nb * t.sqr;
         sqrb += nb * t.unr;
-        nb = vtb / t.dsq;
-        dsqb += -vtb * t.vt / t.dsq;
+        nb = vtb * t.idsq;
+        dsqb += -nb * t.vt;
         utlb += nb * t.sql;
         sqlb += nb * t.utl;
         utrb += nb * t.sqr;
@@ -226,28 +273,31 @@ __host__ __device__ inline void roe_core_rev(const RoeTape& t, double nx, double
         sqlb += dsqb;
         sqrb += dsqb;
     }
-    htb += ctb * kG / (2.0 * t.ct);
+    htb += ctb * kG * 0.5 * t.rct;
     if (t.htact) {
         hlb += 0.5 * htb;
         hrb += 0.5 * htb;
     }
-    if (t.hl > 0.0) hlb += cglb * kG / (2.0 * t.cgl);
-    if (t.hr > 0.0) hrb += cgrb * kG / (2.0 * t.cgr);
+    // cg = sqrt(g) sqrt(h): folded into the sqrt(h) adjoint
+    sqlb += kSqrtG * cglb;
+    sqrb += kSqrtG * cgrb;
     // rotation to the face frame
     const double ulb = unlb * nx - utlb * ny, vlb = unlb * ny + utlb * nx;
     const double urb = unrb * nx - utrb * ny, vrb = unrb * ny + utrb * nx;
     double hulb_ = 0.0, hvlb_ = 0.0, hurb_ = 0.0, hvrb_ = 0.0;
     if (t.hl > 0.0) {
-        hulb_ = ulb / t.hl;
-        hvlb_ = vlb / t.hl;
-        hlb -= (ulb * t.ul + vlb * t.vl) / t.hl;
-        hlb += sqlb / (2.0 * t.sql);
+        const double ih = t.rsl * t.rsl;
+        hulb_ = ulb * ih;
+        hvlb_ = vlb * ih;
+        hlb -= (ulb * t.ul + vlb * t.vl) * ih;
+        hlb += sqlb * 0.5 * t.rsl;
     }
     if (t.hr > 0.0) {
-        hurb_ = urb / t.hr;
-        hvrb_ = vrb / t.hr;
-        hrb -= (urb * t.ur + vrb * t.vr) / t.hr;
-        hrb += sqrb / (2.0 * t.sqr);
+        const double ih = t.rsr * t.rsr;
+        hurb_ = urb * ih;
+        hvrb_ = vrb * ih;
+        hrb -= (urb * t.ur + vrb * t.vr) * ih;
+        hrb += sqrb * 0.5 * t.rsr;
     }
     *hlb_o = t.cutl ? hlb : 0.0;
     *hrb_o = t.cutr ? hrb : 0.0;
@@ -262,7 +312,7 @@ __host__ __device__ inline void roe_core_rev(const RoeTape& t, double nx, double
 // In: L, R = {h, hu, hv, z, n} (h rebuilt); adjoints of F, Sm, Sp (rotated,
 // as the update uses them).  Out: Lb, Rb (5 each).
 // ---------------------------------------------------------------------------
-__host__ __device__ inline void roe_face_vjp(const double* L, const double* R, double ds,
+__device__ __forceinline__ void roe_face_vjp(const double* L, const double* R, double ds,
                                              double nx, double ny, const double* Fb,
                                              const double* Smb, const double* Spb, double* Lb,
                                              double* Rb)
@@ -288,15 +338,20 @@ __host__ __device__ inline void roe_face_vjp(const double* L, const double* R, d
     const bool pick_b = !sel || (ta < tb);
     const double thrust = pick_b ? tb : ta;
     const double nt = 0.5 * (L[4] + R[4]);
-    const double mag = sqrt(t.ut * t.ut + t.vt * t.vt);
+    double mag, rmag;
+    const double m2 = t.ut * t.ut + t.vt * t.vt;
+    sqrt_rsq(m2 > 0.0 ? m2 : 1.0, mag, rmag);
+    if (!(m2 > 0.0)) { mag = 0.0; rmag = 0.0; }
     const double cb = cbrt(t.ht);
     const double h43 = t.ht * cb;
     const bool dnact = kDry < h43;
     const double Dn = dnact ? h43 : kDry;
-    const double sf = (nt * nt * mag * t.ut) / Dn;
+    const double iDn = rcp(Dn);
+    const double sf = (nt * nt * mag * t.ut) * iDn;
     const double tau = kG * t.ht * sf * ds;
-    const double B = (thrust - tau) / t.den;
-    // positivity guard (Roe.H:200-247): b0, b2 as signed picks of B, m1, m3
+    const double B = (thrust - tau) * t.iden;
+    // positivity guard (Roe.H:200-247): b0, b2 as signed picks of B, m1, m3;
+    // the branch tests use correctly rounded divisions, as the forward does
     double c0[3] = {1.0, 0.0, 0.0}, c2[3] = {-1.0, 0.0, 0.0};
     double b0 = B, b2 = -B;
     {
@@ -304,24 +359,25 @@ __host__ __device__ inline void roe_face_vjp(const double* L, const double* R, d
         const double l0s = tol(t.lam0), l2s = tol(t.lam2);
         const double h1s = hl + t.a0 - (b0 / l0s);
         const double h3s = hr - t.a2 + (b2 / l2s);
-        double mlam = fabs0(t.lam0);
-        mlam = (mlam < fabs0(t.lam1)) ? fabs0(t.lam1) : mlam;
-        mlam = (mlam < fabs0(t.lam2)) ? fabs0(t.lam2) : mlam;
-        const double loc_dt = ds / (mlam + kT12);
-        const double d1s = tol(hl - h1s), d3s = tol(hr - h3s);
-        const double dt1 = (ds * 0.5 / l0s) * (hl / d1s);
-        const double dt3 = (ds * 0.5 / l2s) * (hr / d3s);
         const double m1 = -(hl + t.a0) * fabs0(t.lam0);
         const double m3 = -(hr - t.a2) * t.lam2;
-        if ((h1s < -kDry) && (h3s > kDry) && (dt1 < loc_dt)) {
-            if (-m1 >= m3) { b0 = m1; c0[0] = 0.0; c0[1] = 1.0; c0[2] = 0.0; }
-            b2 = -b0;
-            for (int q = 0; q < 3; ++q) c2[q] = -c0[q];
-        }
-        if ((h1s > kDry) && (h3s < -kDry) && (dt3 < loc_dt)) {
-            if (-m3 >= m1) { b2 = m3; c2[0] = 0.0; c2[1] = 0.0; c2[2] = 1.0; }
-            b0 = -b2;
-            for (int q = 0; q < 3; ++q) c0[q] = -c2[q];
+        const bool d1 = (h1s < -kDry) && (h3s > kDry);
+        const bool d3 = (h1s > kDry) && (h3s < -kDry);
+        if (d1 || d3) {
+            double mlam = fabs0(t.lam0);
+            mlam = (mlam < fabs0(t.lam1)) ? fabs0(t.lam1) : mlam;
+            mlam = (mlam < fabs0(t.lam2)) ? fabs0(t.lam2) : mlam;
+            const double loc_dt = ds / (mlam + kT12);
+            if (d1 && ((ds * 0.5 / l0s) * (hl / tol(hl - h1s)) < loc_dt)) {
+                if (-m1 >= m3) { b0 = m1; c0[0] = 0.0; c0[1] = 1.0; c0[2] = 0.0; }
+                b2 = -b0;
+                for (int q = 0; q < 3; ++q) c2[q] = -c0[q];
+            }
+            if (d3 && ((ds * 0.5 / l2s) * (hr / tol(hr - h3s)) < loc_dt)) {
+                if (-m3 >= m1) { b2 = m3; c2[0] = 0.0; c2[1] = 0.0; c2[2] = 1.0; }
+                b0 = -b2;
+                for (int q = 0; q < 3; ++q) c0[q] = -c2[q];
+            }
         }
     }
     // ---- reverse ----
@@ -355,20 +411,20 @@ __host__ __device__ inline void roe_face_vjp(const double* L, const double* R, d
     A.hr -= m3b * t.lam2;
     A.a2 += m3b * t.lam2;
     A.lam2 -= m3b * (hr - t.a2);
-    const double thb = Bb / t.den;
-    const double taub = -Bb / t.den;
-    A.den -= Bb * B / t.den;
+    const double thb = Bb * t.iden;
+    const double taub = -thb;
+    A.den -= thb * B;
     A.ht += taub * kG * sf * ds;
     const double sfb = taub * kG * t.ht * ds;
-    const double numb = sfb / Dn;
-    const double Dnb = -sfb * sf / Dn;
+    const double numb = sfb * iDn;
+    const double Dnb = -numb * sf;
     const double ntb = numb * 2.0 * nt * mag * t.ut;
     const double magb = numb * nt * nt * t.ut;
     A.ut += numb * nt * nt * mag;
-    if (dnact) A.ht += Dnb * (cb + t.ht / (3.0 * cb * cb));
+    if (dnact) A.ht += Dnb * (cb + t.ht * rcp(3.0 * cb * cb));
     if (mag > 0.0) {
-        A.ut += magb * t.ut / mag;
-        A.vt += magb * t.vt / mag;
+        A.ut += magb * t.ut * rmag;
+        A.vt += magb * t.vt * rmag;
     }
     const double tab = pick_b ? 0.0 : thb;
     const double tbb = pick_b ? thb : 0.0;
@@ -390,74 +446,111 @@ __host__ __device__ inline void roe_face_vjp(const double* L, const double* R, d
 }
 
 // ---------------------------------------------------------------------------
-// van Leer slope and its reverse (HydrostaticRoe.cpp:173-195).
+// van Leer slope (HydrostaticRoe.cpp:173-195) with the quantities its
+// reverse sweep reuses.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline double vl_fwd(double qm, double q, double qp)
+struct VL {
+    double dp, idp, r, iD, phi, s;
+    bool flat;
+};
+
+__device__ __forceinline__ VL vl_eval(double qm, double q, double qp)
 {
-    const double dm = q - qm, dp = qp - q;
-    if (fabs0(dp) <= kT12) return 0.0;
-    const double r = dm / dp;
-    return ((r + fabs0(r)) / (1.0 + fabs0(r))) * dp;
+    VL v;
+    const double dm = q - qm;
+    v.dp = qp - q;
+    v.flat = fabs0(v.dp) <= kT12;
+    v.idp = rcp(v.flat ? 1.0 : v.dp);
+    v.r = dm * v.idp;
+    const double ar = fabs0(v.r);
+    v.iD = rcp(1.0 + ar);
+    v.phi = (v.r + ar) * v.iD;
+    v.s = v.flat ? 0.0 : v.phi * v.dp;
+    return v;
 }
 
-__host__ __device__ inline void vl_rev(double qm, double q, double qp, double sb, double& qmb,
-                                       double& qb, double& qpb)
+__device__ __forceinline__ void vl_rev(const VL& v, double sb, double& qmb, double& qb,
+                                       double& qpb)
 {
-    const double dm = q - qm, dp = qp - q;
-    if (fabs0(dp) <= kT12 || sb == 0.0) return;
-    const double r = dm / dp;
-    const double ar = fabs0(r), sg = sgn0(r);
-    const double N = r + ar, D = 1.0 + ar;
-    const double phi = N / D;
-    const double phib = sb * dp;
-    double dpb = sb * phi;
-    const double rb = (phib / D) * (1.0 + sg) - (phib * N / (D * D)) * sg;
-    const double dmb = rb / dp;
-    dpb -= rb * r / dp;
+    if (v.flat) return;
+    const double sg = sgn0(v.r);
+    const double phib = sb * v.dp;
+    double dpb = sb * v.phi;
+    const double Nb = phib * v.iD;                      // phi = N / D, N = r + |r|
+    const double rb = Nb * (1.0 + sg) - Nb * v.phi * sg; // D = 1 + |r|
+    const double dmb = rb * v.idp;
+    dpb -= rb * v.r * v.idp;
     qb += dmb - dpb;
     qmb -= dmb;
     qpb += dpb;
+}
+
+// Reconstructed face states of one hydrostatic face.
+struct HFace {
+    VL se_l, se_r, su_l, su_r, sv_l, sv_r;
+    double u[4], v[4], ih[4];
+    bool wv[4];
+    double zf, uLf, uRf, vLf, vRf, xLu, xRu, xLf, xRf, hLu, hRu, hLf, hRf;
+    bool zfR, nodata;
+};
+
+__device__ __forceinline__ void hface_recon(const double* c, HFace& f)
+{
+    const double *Lm = c, *L = c + 4, *R = c + 8, *Rp = c + 12;
+    const double zL = L[3], zR = R[3];
+    f.nodata = fabs0(zL + 9999.0) < kT10 || fabs0(zR + 9999.0) < kT10;
+    f.zfR = zL < zR;
+    f.zf = f.zfR ? zR : zL;
+    const double eLm = Lm[0] + Lm[3], eL = L[0] + zL, eR = R[0] + zR, eRp = Rp[0] + Rp[3];
+    f.se_l = vl_eval(eLm, eL, eR);
+    f.se_r = vl_eval(eL, eR, eRp);
+    const double etaLf = eL + 0.5 * f.se_l.s;
+    const double etaRf = eR - 0.5 * f.se_r.s;
+    for (int q = 0; q < 4; ++q) {
+        const double h = c[4 * q];
+        f.wv[q] = h > kT12;
+        f.ih[q] = f.wv[q] ? rcp(h) : 0.0;
+        f.u[q] = c[4 * q + 1] * f.ih[q];
+        f.v[q] = c[4 * q + 2] * f.ih[q];
+    }
+    f.su_l = vl_eval(f.u[0], f.u[1], f.u[2]);
+    f.su_r = vl_eval(f.u[1], f.u[2], f.u[3]);
+    f.sv_l = vl_eval(f.v[0], f.v[1], f.v[2]);
+    f.sv_r = vl_eval(f.v[1], f.v[2], f.v[3]);
+    f.uLf = f.u[1] + 0.5 * f.su_l.s;
+    f.uRf = f.u[2] - 0.5 * f.su_r.s;
+    f.vLf = f.v[1] + 0.5 * f.sv_l.s;
+    f.vRf = f.v[2] - 0.5 * f.sv_r.s;
+    f.xLu = etaLf - zL;
+    f.xRu = etaRf - zR;
+    f.xLf = etaLf - f.zf;
+    f.xRf = etaRf - f.zf;
+    f.hLu = (0.0 < f.xLu) ? f.xLu : 0.0;
+    f.hRu = (0.0 < f.xRu) ? f.xRu : 0.0;
+    f.hLf = (0.0 < f.xLf) ? f.xLf : 0.0;
+    f.hRf = (0.0 < f.xRf) ? f.xRf : 0.0;
 }
 
 // ---------------------------------------------------------------------------
 // Hydrostatic-reconstruction face, forward only (HydrostaticRoe.cpp:218-396):
 // the adjoint step needs the flux update of each cell before friction.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline void hydro_face_fwd(const double* c, double nx, double ny,
-                                               double* FL, double* FR)
+__device__ __forceinline__ void hydro_face_fwd(const double* c, double nx, double ny, double* FL,
+                                               double* FR)
 {
-    const double *Lm = c, *L = c + 4, *R = c + 8, *Rp = c + 12;
-    const double zL = L[3], zR = R[3];
-    if (fabs0(zL + 9999.0) < kT10 || fabs0(zR + 9999.0) < kT10) {
+    HFace f;
+    hface_recon(c, f);
+    if (f.nodata) {
         for (int k = 0; k < 3; ++k) { FL[k] = 0.0; FR[k] = 0.0; }
         return;
     }
-    const double zf = (zL < zR) ? zR : zL;
-    const double eLm = Lm[0] + Lm[3], eL = L[0] + zL, eR = R[0] + zR, eRp = Rp[0] + Rp[3];
-    const double etaLf = eL + 0.5 * vl_fwd(eLm, eL, eR);
-    const double etaRf = eR - 0.5 * vl_fwd(eL, eR, eRp);
-    const double* X[4] = {Lm, L, R, Rp};
-    double u[4], v[4];
-    for (int q = 0; q < 4; ++q) {
-        const bool w = X[q][0] > kT12;
-        u[q] = w ? X[q][1] / X[q][0] : 0.0;
-        v[q] = w ? X[q][2] / X[q][0] : 0.0;
-    }
-    const double uLf = u[1] + 0.5 * vl_fwd(u[0], u[1], u[2]);
-    const double uRf = u[2] - 0.5 * vl_fwd(u[1], u[2], u[3]);
-    const double vLf = v[1] + 0.5 * vl_fwd(v[0], v[1], v[2]);
-    const double vRf = v[2] - 0.5 * vl_fwd(v[1], v[2], v[3]);
-    const double xLu = etaLf - zL, xRu = etaRf - zR, xLf = etaLf - zf, xRf = etaRf - zf;
-    const double hLu = (0.0 < xLu) ? xLu : 0.0;
-    const double hRu = (0.0 < xRu) ? xRu : 0.0;
-    const double hLf = (0.0 < xLf) ? xLf : 0.0;
-    const double hRf = (0.0 < xRf) ? xRf : 0.0;
     RoeTape t;
-    roe_core_fwd(hLf, (hLf > 0.0) ? hLf * uLf : 0.0, (hLf > 0.0) ? hLf * vLf : 0.0, hRf,
-                 (hRf > 0.0) ? hRf * uRf : 0.0, (hRf > 0.0) ? hRf * vRf : 0.0, nx, ny, t);
+    roe_core_fwd(f.hLf, (f.hLf > 0.0) ? f.hLf * f.uLf : 0.0, (f.hLf > 0.0) ? f.hLf * f.vLf : 0.0,
+                 f.hRf, (f.hRf > 0.0) ? f.hRf * f.uRf : 0.0, (f.hRf > 0.0) ? f.hRf * f.vRf : 0.0,
+                 nx, ny, t);
     const double f1 = t.F1 * nx - t.F2 * ny, f2 = t.F1 * ny + t.F2 * nx;
-    const double pL = 0.5 * kG * (hLu * hLu - hLf * hLf);
-    const double pR = 0.5 * kG * (hRu * hRu - hRf * hRf);
+    const double pL = 0.5 * kG * (f.hLu * f.hLu - f.hLf * f.hLf);
+    const double pR = 0.5 * kG * (f.hRu * f.hRu - f.hRf * f.hRf);
     FL[0] = t.F0; FL[1] = f1 + pL * nx; FL[2] = f2 + pL * ny;
     FR[0] = t.F0; FR[1] = f1 + pR * nx; FR[2] = f2 + pR * ny;
 }
@@ -467,48 +560,24 @@ __host__ __device__ inline void hydro_face_fwd(const double* c, double nx, doubl
 // c: {Lm, L, R, Rp} x {h, hu, hv, z} (h rebuilt).  FLb, FRb: adjoints of
 // the two face fluxes.  cb (16): adjoints of c.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline void hydro_face_vjp(const double* c, double nx, double ny,
+__device__ __forceinline__ void hydro_face_vjp(const double* c, double nx, double ny,
                                                const double* FLb, const double* FRb, double* cb)
 {
     for (int k = 0; k < 16; ++k) cb[k] = 0.0;
-    const double *Lm = c, *L = c + 4, *R = c + 8, *Rp = c + 12;
-    const double zL = L[3], zR = R[3];
-    if (fabs0(zL + 9999.0) < kT10 || fabs0(zR + 9999.0) < kT10) return;
-    const bool zfR = zL < zR;
-    const double zf = zfR ? zR : zL;
-    const double eLm = Lm[0] + Lm[3], eL = L[0] + zL, eR = R[0] + zR, eRp = Rp[0] + Rp[3];
-    const double etaLf = eL + 0.5 * vl_fwd(eLm, eL, eR);
-    const double etaRf = eR - 0.5 * vl_fwd(eL, eR, eRp);
-    const double* X[4] = {Lm, L, R, Rp};
-    double u[4], v[4];
-    bool wv[4];
-    for (int q = 0; q < 4; ++q) {
-        wv[q] = X[q][0] > kT12;
-        u[q] = wv[q] ? X[q][1] / X[q][0] : 0.0;
-        v[q] = wv[q] ? X[q][2] / X[q][0] : 0.0;
-    }
-    const double uLf = u[1] + 0.5 * vl_fwd(u[0], u[1], u[2]);
-    const double uRf = u[2] - 0.5 * vl_fwd(u[1], u[2], u[3]);
-    const double vLf = v[1] + 0.5 * vl_fwd(v[0], v[1], v[2]);
-    const double vRf = v[2] - 0.5 * vl_fwd(v[1], v[2], v[3]);
-    const double xLu = etaLf - zL, xRu = etaRf - zR, xLf = etaLf - zf, xRf = etaRf - zf;
-    const double hLu = (0.0 < xLu) ? xLu : 0.0;
-    const double hRu = (0.0 < xRu) ? xRu : 0.0;
-    const double hLf = (0.0 < xLf) ? xLf : 0.0;
-    const double hRf = (0.0 < xRf) ? xRf : 0.0;
-    const double huLf = (hLf > 0.0) ? hLf * uLf : 0.0;
-    const double hvLf = (hLf > 0.0) ? hLf * vLf : 0.0;
-    const double huRf = (hRf > 0.0) ? hRf * uRf : 0.0;
-    const double hvRf = (hRf > 0.0) ? hRf * vRf : 0.0;
+    HFace f;
+    hface_recon(c, f);
+    if (f.nodata) return;
     RoeTape t;
-    roe_core_fwd(hLf, huLf, hvLf, hRf, huRf, hvRf, nx, ny, t);
+    roe_core_fwd(f.hLf, (f.hLf > 0.0) ? f.hLf * f.uLf : 0.0, (f.hLf > 0.0) ? f.hLf * f.vLf : 0.0,
+                 f.hRf, (f.hRf > 0.0) ? f.hRf * f.uRf : 0.0, (f.hRf > 0.0) ? f.hRf * f.vRf : 0.0,
+                 nx, ny, t);
     // ---- reverse ----
     // FL = rot(f) + (0, pL nx, pL ny), FR likewise
     const double fb[3] = {FLb[0] + FRb[0], FLb[1] + FRb[1], FLb[2] + FRb[2]};
     const double pLb = FLb[1] * nx + FLb[2] * ny;
     const double pRb = FRb[1] * nx + FRb[2] * ny;
-    double hLub = pLb * kG * hLu, hRub = pRb * kG * hRu;
-    double hLfb = -pLb * kG * hLf, hRfb = -pRb * kG * hRf;
+    double hLub = pLb * kG * f.hLu, hRub = pRb * kG * f.hRu;
+    double hLfb = -pLb * kG * f.hLf, hRfb = -pRb * kG * f.hRf;
     double huLfb, hvLfb, huRfb, hvRfb, hLfb2, hRfb2;
     RoeBar A{};
     roe_core_rev(t, nx, ny, fb[0], fb[1] * nx + fb[2] * ny, -fb[1] * ny + fb[2] * nx, A, &hLfb2,
@@ -516,48 +585,47 @@ __host__ __device__ inline void hydro_face_vjp(const double* c, double nx, doubl
     hLfb += hLfb2;
     hRfb += hRfb2;
     double uLfb = 0.0, vLfb = 0.0, uRfb = 0.0, vRfb = 0.0;
-    if (hLf > 0.0) {
-        hLfb += huLfb * uLf + hvLfb * vLf;
-        uLfb = huLfb * hLf;
-        vLfb = hvLfb * hLf;
+    if (f.hLf > 0.0) {
+        hLfb += huLfb * f.uLf + hvLfb * f.vLf;
+        uLfb = huLfb * f.hLf;
+        vLfb = hvLfb * f.hLf;
     }
-    if (hRf > 0.0) {
-        hRfb += huRfb * uRf + hvRfb * vRf;
-        uRfb = huRfb * hRf;
-        vRfb = hvRfb * hRf;
+    if (f.hRf > 0.0) {
+        hRfb += huRfb * f.uRf + hvRfb * f.vRf;
+        uRfb = huRfb * f.hRf;
+        vRfb = hvRfb * f.hRf;
     }
     double etaLfb = 0.0, etaRfb = 0.0, zfb = 0.0, zLb = 0.0, zRb = 0.0;
-    if (0.0 < xLf) { etaLfb += hLfb; zfb -= hLfb; }
-    if (0.0 < xRf) { etaRfb += hRfb; zfb -= hRfb; }
-    if (0.0 < xLu) { etaLfb += hLub; zLb -= hLub; }
-    if (0.0 < xRu) { etaRfb += hRub; zRb -= hRub; }
+    if (0.0 < f.xLf) { etaLfb += hLfb; zfb -= hLfb; }
+    if (0.0 < f.xRf) { etaRfb += hRfb; zfb -= hRfb; }
+    if (0.0 < f.xLu) { etaLfb += hLub; zLb -= hLub; }
+    if (0.0 < f.xRu) { etaRfb += hRub; zRb -= hRub; }
     // velocity reconstruction
     double ub[4] = {0.0, 0.0, 0.0, 0.0}, vb[4] = {0.0, 0.0, 0.0, 0.0};
     ub[1] += uLfb;
-    vl_rev(u[0], u[1], u[2], 0.5 * uLfb, ub[0], ub[1], ub[2]);
+    vl_rev(f.su_l, 0.5 * uLfb, ub[0], ub[1], ub[2]);
     ub[2] += uRfb;
-    vl_rev(u[1], u[2], u[3], -0.5 * uRfb, ub[1], ub[2], ub[3]);
+    vl_rev(f.su_r, -0.5 * uRfb, ub[1], ub[2], ub[3]);
     vb[1] += vLfb;
-    vl_rev(v[0], v[1], v[2], 0.5 * vLfb, vb[0], vb[1], vb[2]);
+    vl_rev(f.sv_l, 0.5 * vLfb, vb[0], vb[1], vb[2]);
     vb[2] += vRfb;
-    vl_rev(v[1], v[2], v[3], -0.5 * vRfb, vb[1], vb[2], vb[3]);
+    vl_rev(f.sv_r, -0.5 * vRfb, vb[1], vb[2], vb[3]);
     for (int q = 0; q < 4; ++q) {
-        if (wv[q]) {
-            const double h = X[q][0];
-            cb[4 * q + 1] += ub[q] / h;
-            cb[4 * q + 2] += vb[q] / h;
-            cb[4 * q + 0] -= (ub[q] * u[q] + vb[q] * v[q]) / h;
+        if (f.wv[q]) {
+            cb[4 * q + 1] += ub[q] * f.ih[q];
+            cb[4 * q + 2] += vb[q] * f.ih[q];
+            cb[4 * q + 0] -= (ub[q] * f.u[q] + vb[q] * f.v[q]) * f.ih[q];
         }
     }
     // free-surface reconstruction
     double eb[4] = {0.0, etaLfb, etaRfb, 0.0};
-    vl_rev(eLm, eL, eR, 0.5 * etaLfb, eb[0], eb[1], eb[2]);
-    vl_rev(eL, eR, eRp, -0.5 * etaRfb, eb[1], eb[2], eb[3]);
+    vl_rev(f.se_l, 0.5 * etaLfb, eb[0], eb[1], eb[2]);
+    vl_rev(f.se_r, -0.5 * etaRfb, eb[1], eb[2], eb[3]);
     for (int q = 0; q < 4; ++q) {
         cb[4 * q + 0] += eb[q];
         cb[4 * q + 3] += eb[q];
     }
-    if (zfR) zRb += zfb; else zLb += zfb;
+    if (f.zfR) zRb += zfb; else zLb += zfb;
     cb[4 + 3] += zLb;
     cb[8 + 3] += zRb;
 }
@@ -567,32 +635,38 @@ __host__ __device__ inline void hydro_face_vjp(const double* c, double nx, doubl
 // (HydrostaticRoe.cpp:566-603).  un: state after the flux update; ob:
 // adjoint of the output.  unb: adjoint of un; returns d/dn.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline double hydro_post_vjp(const double* un, double n, double dt,
+__device__ __forceinline__ double hydro_post_vjp(const double* un, double n, double dt,
                                                  const double* ob, double* unb)
 {
     const double hn = un[0];
     if (hn >= kDry) {
         const double qx = un[1], qy = un[2];
         const double nsq = n * n;
-        const double inv_h = 1.0 / hn;
+        const double inv_h = rcp(hn);
         const double u = qx * inv_h, v = qy * inv_h;
-        const double modM = sqrt(u * u + v * v);
+        const double m2 = u * u + v * v;
+        double modM, rM;
+        sqrt_rsq(m2 > 0.0 ? m2 : 1.0, modM, rM);
+        if (!(m2 > 0.0)) modM = 0.0;
         unb[0] = ob[0];
         if (nsq > kT12 && modM > kT12) {
             const double cb = cbrt(hn);
             const double h43 = hn * cb;
-            const double tt = dt * kG * nsq * modM / h43;
-            const double ff = sqrt(1.0 + 4.0 * tt);
-            const double fac = 2.0 / (1.0 + ff);
+            const double ih43 = rcp(h43);
+            const double tt = dt * kG * nsq * modM * ih43;
+            double ff, rff;
+            sqrt_rsq(1.0 + 4.0 * tt, ff, rff);
+            const double i1f = rcp(1.0 + ff);
+            const double fac = 2.0 * i1f;
             const double facb = ob[1] * qx + ob[2] * qy;
             double qxb = ob[1] * fac, qyb = ob[2] * fac;
-            const double ffb = -facb * 2.0 / ((1.0 + ff) * (1.0 + ff));
-            const double ttb = ffb * 2.0 / ff;
-            const double nsqb = ttb * dt * kG * modM / h43;
-            const double modMb = ttb * dt * kG * nsq / h43;
-            const double h43b = -ttb * tt / h43;
-            double hnb = h43b * (cb + hn / (3.0 * cb * cb));
-            const double ub = modMb * u / modM, vb = modMb * v / modM;
+            const double ffb = -facb * 2.0 * i1f * i1f;
+            const double ttb = ffb * 2.0 * rff;
+            const double nsqb = ttb * dt * kG * modM * ih43;
+            const double modMb = ttb * dt * kG * nsq * ih43;
+            const double h43b = -ttb * tt * ih43;
+            double hnb = h43b * (cb + hn * rcp(3.0 * cb * cb));
+            const double ub = modMb * u * rM, vb = modMb * v * rM;
             qxb += ub * inv_h;
             qyb += vb * inv_h;
             hnb -= (ub * u + vb * v) * inv_h;

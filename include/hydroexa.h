@@ -229,9 +229,9 @@ int hx_ensemble_run(hx_ensemble *ens, double cfl, int nsteps, int use_graph, dou
  *
  * geom: nx, ny, dx, dy and bc_lo/bc_hi (HX_BC_REFLECT / HX_BC_TRANSMIT),
  * dtype HX_F64; its pointers are ignored.  gauges (host): flat cell
- * indices j*nx + i.  Device memory: (ncp + max(ckpt_every-1, 2) + 3) states
- * of 3*nx*ny doubles plus 10 (Roe) or 16 (hydrostatic) doubles per face,
- * ncp = ceil(nsteps / ckpt_every).
+ * indices j*nx + i.  Device memory: ncp + max(ckpt_every-1, 2) + 2 states
+ * of 3*nx*ny doubles (+1 for the hydrostatic solver), ncp =
+ * ceil(nsteps / ckpt_every).
  */
 typedef struct hx_adjoint hx_adjoint;
 int hx_adjoint_create(hx_ctx *ctx, int solver, const hx_patch *geom, int nsteps, int ckpt_every,
