@@ -6,9 +6,10 @@
 //
 // One adjoint step, lambda^{n+1} -> lambda^n (plus the roughness and bed
 // gradients accumulated), is deterministic (no atomics):
-//   post    (hydrostatic only) per cell: re-evaluate the cell's four faces,
-//           the flux update, and the reverse of Manning friction and the
-//           dry clamp -> lambda_u (adjoint of the pre-friction state);
+//   post    (hydrostatic only): the forward's strip march re-evaluates the
+//           flux update of every cell, then applies the reverse of Manning
+//           friction and the dry clamp -> lambda_u (adjoint of the
+//           pre-friction state; hydro_strip<double, true> in hx_advance.inc);
 //   tile    per tile of cells: the vector-Jacobian product of every face
 //           whose stencil touches the tile (2 cells for plain Roe, 4 for the
 //           hydrostatic face), summed in LDS, then per cell the gather over
@@ -96,37 +97,6 @@ __device__ __forceinline__ void put4(double* c, const PC& p)
     c[1] = p.hu;
     c[2] = p.hv;
     c[3] = p.z;
-}
-
-// ---------------------------------------------------------------------------
-// Hydrostatic step (src/solvers/HydrostaticRoe.cpp:400-606)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void hpost_kernel(State s, Geo g, double dt, double dx,
-                                                         double dy, const double* __restrict__ lam,
-                                                         double* __restrict__ lun,
-                                                         double* __restrict__ gn)
-{
-    const int64_t N = (int64_t)g.nx * g.ny;
-    const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (c >= N) return;
-    const int j = (int)(c / g.nx), i = (int)(c - (int64_t)j * g.nx);
-    double st[20];
-    double FRx[3], FLx[3], FRy[3], FLy[3], dummy[3];
-    for (int q = 0; q < 5; ++q) put4(st + 4 * q, pcell(s, g, i - 2 + q, j));
-    hydro_face_fwd(st, 1.0, 0.0, dummy, FRx);        // face i: FR enters cell i
-    hydro_face_fwd(st + 4, 1.0, 0.0, FLx, dummy);    // face i+1: FL enters cell i
-    for (int q = 0; q < 5; ++q) put4(st + 4 * q, pcell(s, g, i, j - 2 + q));
-    hydro_face_fwd(st, 0.0, 1.0, dummy, FRy);
-    hydro_face_fwd(st + 4, 0.0, 1.0, FLy, dummy);
-    const double uo[3] = {s.h[c], s.hu[c], s.hv[c]};
-    double un[3], ob[3], unb[3];
-    for (int k = 0; k < 3; ++k) {
-        un[k] = uo[k] - (dt / dx) * (FLx[k] - FRx[k]) - (dt / dy) * (FLy[k] - FRy[k]);
-        ob[k] = lam[k * N + c];
-    }
-    const double dn = hydro_post_vjp(un, s.n[c], dt, ob, unb);
-    for (int k = 0; k < 3; ++k) lun[k * N + c] = unb[k];
-    gn[c] += dn;
 }
 
 // ---------------------------------------------------------------------------
@@ -465,11 +435,16 @@ __global__ void sample_kernel(const double* h, const int64_t* gauges, double* ou
     if (g < ng) out[g] = h[gauges[g]];
 }
 
-// Gauge adjoints into lambda_h, in gauge order (repeated gauges add up).
-__global__ void seed_kernel(double* lam_h, const int64_t* gauges, const double* gs, int ng)
+// Gauge adjoints into lambda_h: one thread per distinct gauge cell, which
+// sums the series adjoints of its gauges in gauge order (deterministic).
+__global__ void seed_kernel(double* lam_h, const int64_t* ucell, const int* uptr,
+                            const int* uidx, const double* gs, int nu)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        for (int g = 0; g < ng; ++g) lam_h[gauges[g]] += gs[g];
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    double acc = 0.0;
+    for (int k = uptr[u]; k < uptr[u + 1]; ++k) acc += gs[uidx[k]];
+    lam_h[ucell[u]] += acc;
 }
 
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -491,6 +466,10 @@ struct hx_adjoint {
     int nsteps, every, ncp, nseg, ngauge;
     int64_t N;
     int64_t* gauges;   // device, ngauge flat cell indices
+    int64_t* ucell;    // device, distinct gauge cells
+    int* uptr;         // device, nu + 1: gauges of distinct cell u are
+    int* uidx;         //   uidx[uptr[u] .. uptr[u+1]) (device, ngauge)
+    int nu;
     double* ckpt;      // ncp states (3N each): the state at steps 0, every, 2 every, ...
     double* seg;       // nseg states: recomputed states inside a segment
     double* lam;       // 2 states: adjoint ping-pong
@@ -507,6 +486,9 @@ static void adj_free(hx_adjoint* a)
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (a->gauges) (void)hipFree(a->gauges);
+    if (a->ucell) (void)hipFree(a->ucell);
+    if (a->uptr) (void)hipFree(a->uptr);
+    if (a->uidx) (void)hipFree(a->uidx);
 }
 
 extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, int nsteps,
@@ -552,8 +534,35 @@ extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, in
               hipMalloc(&a->lam, 2 * S) == hipSuccess;
     if (ok && solver == HX_SOLVER_HYDROSTATIC_ROE) ok = hipMalloc(&a->lun, S) == hipSuccess;
     if (ok && ngauge > 0) {
+        // group gauges by cell (first appearance order) for the seed kernel
+        std::vector<int64_t> ucell;
+        std::vector<std::vector<int>> members;
+        for (int gi = 0; gi < ngauge; ++gi) {
+            size_t u = 0;
+            while (u < ucell.size() && ucell[u] != gauges[gi]) ++u;
+            if (u == ucell.size()) {
+                ucell.push_back(gauges[gi]);
+                members.emplace_back();
+            }
+            members[u].push_back(gi);
+        }
+        std::vector<int> uptr(1, 0), uidx;
+        for (auto& m : members) {
+            uidx.insert(uidx.end(), m.begin(), m.end());
+            uptr.push_back((int)uidx.size());
+        }
+        a->nu = (int)ucell.size();
         ok = hipMalloc(&a->gauges, sizeof(int64_t) * ngauge) == hipSuccess &&
              hipMemcpy(a->gauges, gauges, sizeof(int64_t) * ngauge, hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMalloc(&a->ucell, sizeof(int64_t) * a->nu) == hipSuccess &&
+             hipMemcpy(a->ucell, ucell.data(), sizeof(int64_t) * a->nu, hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMalloc(&a->uptr, sizeof(int) * (a->nu + 1)) == hipSuccess &&
+             hipMemcpy(a->uptr, uptr.data(), sizeof(int) * (a->nu + 1), hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMalloc(&a->uidx, sizeof(int) * ngauge) == hipSuccess &&
+             hipMemcpy(a->uidx, uidx.data(), sizeof(int) * ngauge, hipMemcpyHostToDevice) ==
                  hipSuccess;
     }
     if (!ok) {
@@ -690,17 +699,27 @@ extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl,
 }
 
 // One adjoint step: lam (adjoint of the step's output) -> lam_new (adjoint
-// of its input U), accumulating the bed and roughness gradients.
-static int adj_step(hx_adjoint* a, const double* U, double dt, const double* lam, double* lam_new,
-                    double* gz, double* gn, hipStream_t hs)
+// of its input U), accumulating the bed and roughness gradients.  sa holds
+// the step geometry (a hydrostatic forward launch shape) for the post pass.
+static int adj_step(hx_adjoint* a, StepArgs<double> sa, const double* U, double dt,
+                    const double* lam, double* lam_new, double* gz, double* gn, hipStream_t hs)
 {
     const int64_t N = a->N;
     State s{U, U + N, U + 2 * N, a->z, a->n};
     Geo g{a->nx, a->ny, a->bc_lo[0], a->bc_lo[1], a->bc_hi[0], a->bc_hi[1]};
     const dim3 tiles((unsigned)((a->nx + kTX - 1) / kTX), (unsigned)((a->ny + kTY - 1) / kTY));
     if (a->solver == HX_SOLVER_HYDROSTATIC_ROE) {
-        hipLaunchKernelGGL(hpost_kernel, dim3(nblocks(N)), dim3(kThreads), 0, hs, s, g, dt, a->dx,
-                           a->dy, lam, a->lun, gn);
+        sa.p.h = U;
+        sa.p.hu = U + N;
+        sa.p.hv = U + 2 * N;
+        sa.rd = nullptr;
+        sa.acc = nullptr;
+        sa.dt_host = dt;
+        sa.lam = lam;
+        sa.lun = a->lun;
+        sa.gn = gn;
+        const int rc = host::launch_adjpost_f64(a->ctx, sa, hs);
+        if (rc) return rc;
         hipLaunchKernelGGL(htile_kernel, tiles, dim3(kThreads), 0, hs, s, g, dt, a->dx, a->dy,
                            (const double*)a->lun, lam_new, gz);
     } else {
@@ -762,10 +781,11 @@ extern "C" int hx_adjoint_backward(hx_adjoint* a, const double* g_series, const 
         }
         for (int t = s1 - 1; t >= s0; --t) {
             if (g_series && a->ngauge > 0) {
-                hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(64), 0, hs, lam, a->gauges,
-                                   g_series + (size_t)t * a->ngauge, a->ngauge);
+                hipLaunchKernelGGL(seed_kernel, dim3((a->nu + 63) / 64), dim3(64), 0, hs, lam,
+                                   a->ucell, a->uptr, a->uidx,
+                                   g_series + (size_t)t * a->ngauge, a->nu);
             }
-            int rc = adj_step(a, state(t), a->dts[t], lam, lam2, g_z, g_n, hs);
+            int rc = adj_step(a, s, state(t), a->dts[t], lam, lam2, g_z, g_n, hs);
             if (rc) return rc;
             std::swap(lam, lam2);
         }

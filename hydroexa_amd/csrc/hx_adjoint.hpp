@@ -532,30 +532,6 @@ __device__ __forceinline__ void hface_recon(const double* c, HFace& f)
 }
 
 // ---------------------------------------------------------------------------
-// Hydrostatic-reconstruction face, forward only (HydrostaticRoe.cpp:218-396):
-// the adjoint step needs the flux update of each cell before friction.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void hydro_face_fwd(const double* c, double nx, double ny, double* FL,
-                                               double* FR)
-{
-    HFace f;
-    hface_recon(c, f);
-    if (f.nodata) {
-        for (int k = 0; k < 3; ++k) { FL[k] = 0.0; FR[k] = 0.0; }
-        return;
-    }
-    RoeTape t;
-    roe_core_fwd(f.hLf, (f.hLf > 0.0) ? f.hLf * f.uLf : 0.0, (f.hLf > 0.0) ? f.hLf * f.vLf : 0.0,
-                 f.hRf, (f.hRf > 0.0) ? f.hRf * f.uRf : 0.0, (f.hRf > 0.0) ? f.hRf * f.vRf : 0.0,
-                 nx, ny, t);
-    const double f1 = t.F1 * nx - t.F2 * ny, f2 = t.F1 * ny + t.F2 * nx;
-    const double pL = 0.5 * kG * (f.hLu * f.hLu - f.hLf * f.hLf);
-    const double pR = 0.5 * kG * (f.hRu * f.hRu - f.hRf * f.hRf);
-    FL[0] = t.F0; FL[1] = f1 + pL * nx; FL[2] = f2 + pL * ny;
-    FR[0] = t.F0; FR[1] = f1 + pR * nx; FR[2] = f2 + pR * ny;
-}
-
-// ---------------------------------------------------------------------------
 // Hydrostatic-reconstruction face (HydrostaticRoe.cpp:218-396).
 // c: {Lm, L, R, Rp} x {h, hu, hv, z} (h rebuilt).  FLb, FRb: adjoints of
 // the two face fluxes.  cb (16): adjoints of c.

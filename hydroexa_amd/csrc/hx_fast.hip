@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "hx_types.hpp"
+#include "hx_adjoint.hpp"
 
 namespace hx {
 #define HX_NS fast

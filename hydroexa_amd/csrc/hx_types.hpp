@@ -76,6 +76,11 @@ struct StepArgs {
     double* vel_hist;
     int* err;
     int step;
+    // adjoint post pass (hydrostatic, FP64): lambda of the step's output in,
+    // lambda before friction out, d/dn accumulated; null in a forward step
+    const R* lam;
+    R* lun;
+    R* gn;
 };
 
 // Occupancy variants compiled for the advance kernel (minimum waves per
@@ -87,10 +92,12 @@ constexpr int kOcc[kOccVariants] = {1, 3, 4};
 namespace exact {
 template <typename R>
 int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
+int launch_adjpost(int occ, const StepArgs<double>& a, hipStream_t st);
 }
 namespace fast {
 template <typename R>
 int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
+int launch_adjpost(int occ, const StepArgs<double>& a, hipStream_t st);
 }
 
 }  // namespace hx
