@@ -114,6 +114,28 @@ def traffic_from_profile(name):
     return e.get("bytes_per_launch") if isinstance(e, dict) else None
 
 
+def time_conservation(w, dtype, reps=20):
+    """Fused conservation reduction (mass, momenta, KE, min h) of the
+    workload's level, wall time per call including the host sync of the
+    5-value result; 4 words read per cell (h, hu, hv, roughness)."""
+    import torch
+    from hydroexa_amd import Level
+    from hydroexa_amd.diagnostics import level_conservation
+
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi, dtype=dt)
+    vals = level_conservation(lv)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        vals = level_conservation(lv)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    es = 8 if dtype == "f64" else 4
+    return {"kernel": "conservation reduction", "cells": w.cells, "ms_per_call": ms,
+            "GBps_incl_sync": 4 * es * w.cells / (ms / 1e3) / 1e9,
+            "mass": float(vals[0]), "min_h": float(vals[4])}
+
+
 def time_ensemble(steps, arith="fast"):
     """256 dam-break scenarios of uneven size (FP32): the batched ensemble
     (one launch per step for all scenarios, 6-step HIP graph replayed,
@@ -294,6 +316,7 @@ def main():
         "wall_s": wall,
         "final_dt": float(dth[-1]) if len(dth) else None,
     }
+    out["diagnostics"] = time_conservation(w, args.dtype)
     if args.extra:
         extra = {}
         bumpy = W.dam_break(4096, 1)

@@ -41,6 +41,7 @@ EXPORTS = (
     "hx_prune_boxes", "hx_run_level", "hx_set_tuning", "hx_set_arith",
     "hx_ensemble_create", "hx_ensemble_destroy", "hx_ensemble_set_tuning", "hx_ensemble_run",
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
+    "hx_conservation",
 )
 
 
@@ -103,6 +104,7 @@ def load() -> C.CDLL:
     L.hx_adjoint_create.argtypes = [vp, C.c_int, pp, C.c_int, C.c_int, C.c_int,
                                     C.POINTER(C.c_int64), C.POINTER(vp)]
     L.hx_adjoint_destroy.argtypes = [vp]
+    L.hx_conservation.argtypes = [vp, pp, dp, vp]
     L.hx_adjoint_forward.argtypes = [vp, pp, C.c_double, dp, vp, vp, vp, vp, i32p, vp]
     L.hx_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     _lib = L

@@ -22,6 +22,7 @@ struct hx_ctx {
     int rows[2][2];      // [solver][dtype]
     int occ[2][2];       // [solver][dtype] minimum waves per SIMD requested
     int arith;           // HX_ARITH_EXACT or HX_ARITH_FAST
+    double* cons;        // conservation partials (kConsBlocks x 9) + result (5)
 };
 
 #define HX_TRY(x)                                  \

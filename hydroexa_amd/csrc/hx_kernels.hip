@@ -233,6 +233,99 @@ __global__ __launch_bounds__(kBlock) void ens_cfl_kernel(const ScenDesc<R>* desc
     cfl_commit(acc, d.slots);
 }
 
+// ---------------------------------------------------------------------------
+// Conservation diagnostics (src/state/AmrMeshState_Utils.cpp:213-343): one
+// fused pass over the valid cells of a level gives sum h dA, sum hu dA,
+// sum hv dA, sum 0.5 (hu^2 + hv^2)/h dA and min h, over cells whose
+// roughness is not NODATA and h > 0.  Sums are compensated (TwoSum) per
+// thread, folded by a fixed tree per block and by one thread over the blocks,
+// so the result depends only on the level size, not on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int kConsBlocks = 1024;
+
+struct KSum {
+    double s, c;
+};
+
+__device__ __forceinline__ void kadd(KSum& a, double x)
+{
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+    else a.c += (x - t) + a.s;
+    a.s = t;
+}
+
+__device__ __forceinline__ void kmerge(KSum& a, const KSum& b)
+{
+    kadd(a, b.s);
+    a.c += b.c;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void cons_kernel(PatchView<R> p, R dA, double* part)
+{
+    __shared__ KSum sh[4][kBlock];
+    __shared__ double smin_[kBlock];
+    KSum acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    double mn = 1.7976931348623157e308;
+    const int64_t ncell = (int64_t)p.nx * p.ny;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < ncell;
+         q += (int64_t)gridDim.x * kBlock) {
+        const int j = (int)(q / p.nx), i = (int)(q - (int64_t)j * p.nx);
+        const int64_t s = (int64_t)j * p.pitch + i;
+        const R h = p.h[s], qx = p.hu[s], qy = p.hv[s];
+        if (!(p.n[s] == R(-9999.0)) && h > R(0)) {
+            kadd(acc[0], (double)(h * dA));
+            kadd(acc[1], (double)(qx * dA));
+            kadd(acc[2], (double)(qy * dA));
+            kadd(acc[3], (double)(R(0.5) * (qx * qx + qy * qy) / h * dA));
+            mn = ((double)h < mn) ? (double)h : mn;
+        }
+    }
+    const int t = threadIdx.x;
+    for (int k = 0; k < 4; ++k) sh[k][t] = acc[k];
+    smin_[t] = mn;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 4; ++k) kmerge(sh[k][t], sh[k][t + w]);
+            smin_[t] = (smin_[t + w] < smin_[t]) ? smin_[t + w] : smin_[t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        double* o = part + (size_t)blockIdx.x * 9;
+        for (int k = 0; k < 4; ++k) { o[2 * k] = sh[k][0].s; o[2 * k + 1] = sh[k][0].c; }
+        o[8] = smin_[0];
+    }
+}
+
+// One workgroup of kConsBlocks threads folds the block partials by a fixed
+// tree (thread t holds partial t), so the result is repeatable.
+__global__ __launch_bounds__(kConsBlocks) void cons_final_kernel(const double* part, int nblk,
+                                                                 double* out)
+{
+    __shared__ KSum sh[4][kConsBlocks];
+    __shared__ double smin_[kConsBlocks];
+    const int t = threadIdx.x;
+    for (int k = 0; k < 4; ++k)
+        sh[k][t] = (t < nblk) ? KSum{part[(size_t)t * 9 + 2 * k], part[(size_t)t * 9 + 2 * k + 1]}
+                              : KSum{0.0, 0.0};
+    smin_[t] = (t < nblk) ? part[(size_t)t * 9 + 8] : 1.7976931348623157e308;
+    __syncthreads();
+    for (int w = kConsBlocks / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 4; ++k) kmerge(sh[k][t], sh[k][t + w]);
+            smin_[t] = (smin_[t + w] < smin_[t]) ? smin_[t + w] : smin_[t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        for (int k = 0; k < 4; ++k) out[k] = sh[k][0].s + sh[k][0].c;
+        out[4] = smin_[0];
+    }
+}
+
 }  // namespace hx
 
 // ===========================================================================
@@ -309,6 +402,7 @@ extern "C" int hx_ctx_destroy(hx_ctx* c)
     if (c->hist) hipFree(c->hist);
     if (c->pflags) hipFree(c->pflags);
     if (c->pboxes) hipFree(c->pboxes);
+    if (c->cons) hipFree(c->cons);
     delete c;
     return HX_OK;
 }
@@ -563,6 +657,39 @@ extern "C" int hx_prune_boxes(hx_ctx* c, int dtype, int nbox, const int32_t* box
         nk = nbox;
     }
     *nkeep = nk;
+    return HX_OK;
+}
+
+template <typename R>
+static int cons_t(hx_ctx* c, const hx_patch* in, hipStream_t st)
+{
+    const int64_t ncell = (int64_t)in->nx * in->ny;
+    const int nblk = (int)std::min<int64_t>(kConsBlocks, (ncell + kBlock - 1) / kBlock);
+    const R dA = (R)in->dx * (R)in->dy;
+    hipLaunchKernelGGL(cons_kernel<R>, dim3(nblk), dim3(kBlock), 0, st, view_of<R>(in), dA,
+                       c->cons);
+    hipLaunchKernelGGL(cons_final_kernel, dim3(1), dim3(kConsBlocks), 0, st, (const double*)c->cons, nblk,
+                       c->cons + (size_t)kConsBlocks * 9);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int hx_conservation(hx_ctx* c, const hx_patch* in, double* out, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    int rc = check_patch(in, 0);
+    if (rc) return rc;
+    if (!out) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    if (!c->cons && hipMalloc(&c->cons, sizeof(double) * ((size_t)kConsBlocks * 9 + 5)) != hipSuccess) {
+        c->cons = nullptr;
+        return HX_E_NOMEM;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    rc = (in->dtype == HX_F64) ? cons_t<double>(c, in, st) : cons_t<float>(c, in, st);
+    if (rc) return rc;
+    HX_TRY(hipMemcpyAsync(out, c->cons + (size_t)kConsBlocks * 9, sizeof(double) * 5,
+                          hipMemcpyDeviceToHost, st));
+    HX_TRY(hipStreamSynchronize(st));
     return HX_OK;
 }
 

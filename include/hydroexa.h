@@ -150,6 +150,18 @@ int hx_prune_boxes(hx_ctx *ctx, int dtype, int nbox, const int32_t *boxes, int n
                    uint8_t *has_nodata, int32_t *nkeep, void *stream);
 
 /*
+ * Conservation diagnostics of one level in one fused reduction.  Replaces
+ * the per-level reduction of AmrMeshState::ComputeMassDiagnostics
+ * (src/state/AmrMeshState_Utils.cpp:213-343): over valid cells whose
+ * roughness is not NODATA (-9999) and h > 0, out (host, 5 doubles) =
+ * {sum h dA, sum hu dA, sum hv dA, sum 0.5 (hu^2 + hv^2)/h dA, min h}
+ * (min h = DBL_MAX when no cell qualifies).  Sums are compensated and
+ * deterministic; totals over levels and ranks are the caller's, as in the
+ * reference.  Synchronises the stream.
+ */
+int hx_conservation(hx_ctx *ctx, const hx_patch *level, double *out, void *stream);
+
+/*
  * Step-driven single-level time loop with the CFL reduction fused into the
  * advance: nsteps x (ComputeDt; AdvanceLevel).  Replaces the loop body of
  * HydroEXA::Compute (src/HydroEXA.cpp:260-271) for one level with a step

@@ -786,3 +786,32 @@ int hxo_run(int solver, int nx, int ny, double *h, double *hu, double *hv,
     free(tmp);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Conservation diagnostics of one level.                                   */
+/* Restates AmrMeshState::ComputeMassDiagnostics,                           */
+/* src/state/AmrMeshState_Utils.cpp:213-343: over cells whose roughness is  */
+/* not NODATA (== -9999) and h > 0, mass = sum h dA, px = sum hu dA,        */
+/* py = sum hv dA, KE = sum 0.5 (hu^2 + hv^2) / h dA, and min h (DBL_MAX    */
+/* when no cell qualifies).  The sums run in cell order (j outer, i inner), */
+/* the order of a serial CPU build.  out = {mass, px, py, KE, min_h}.       */
+/* ------------------------------------------------------------------------ */
+void hxo_conservation(int nx, int ny, const double *h, const double *hu, const double *hv,
+                      const double *n, double dx, double dy, double *out)
+{
+    const double dA = dx * dy;
+    double m = 0.0, px = 0.0, py = 0.0, ke = 0.0, mn = 1.7976931348623157e308;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const size_t s = (size_t)j * nx + i;
+            const double hh = h[s], qx = hu[s], qy = hv[s];
+            if (!(n[s] == NODATA) && hh > 0.0) {
+                m += hh * dA;
+                px += qx * dA;
+                py += qy * dA;
+                ke += 0.5 * (qx * qx + qy * qy) / hh * dA;
+                mn = smin(mn, hh);
+            }
+        }
+    out[0] = m; out[1] = px; out[2] = py; out[3] = ke; out[4] = mn;
+}

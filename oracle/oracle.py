@@ -56,6 +56,8 @@ def lib():
                                               _dp, C.POINTER(C.c_int32)]
         L.hxo_prune_boxes.argtypes = [C.c_int, C.POINTER(C.c_int32), C.c_int, C.c_int, _dp,
                                       C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
+        L.hxo_conservation.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
+                                       C.c_double, _dp]
         L.hxo_run.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double,
                               C.c_double, _ip, _ip, C.c_double, C.c_int, _dp, _dp]
         _lib = L
@@ -180,3 +182,12 @@ def run(solver, h, hu, hv, z, n, dx, dy, cfl, nsteps, bc_lo=(1, 1), bc_hi=(1, 1)
     if rc != 0:
         raise RuntimeError(f"oracle run failed with status {rc}")
     return h, hu, hv, dth, vh
+
+
+def conservation(h, hu, hv, n, dx, dy):
+    """(mass, px, py, KE, min_h) of one level, serial cell-order sums."""
+    h, hu, hv, n = (_f64(a) for a in (h, hu, hv, n))
+    ny, nx = h.shape
+    out = np.zeros(5)
+    lib().hxo_conservation(nx, ny, _p(h), _p(hu), _p(hv), _p(n), dx, dy, _p(out))
+    return out
