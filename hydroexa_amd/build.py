@@ -44,6 +44,36 @@ def flags() -> list:
     ]
 
 
+HDF5_ROOT = os.environ.get("HYDROEXA_HDF5", "/opt/conda")
+IO_LIB = os.path.join(LIBDIR, "libhydroexa_io.so")
+
+
+def build_io(verbose: bool = False, libdir: str = LIBDIR) -> str:
+    """Host-only HDF5 input library (libhydroexa_io.so) with g++ against the
+    serial HDF5 C library.  Returns the path, or "" when HDF5 is absent."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    lib = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        sys.stderr.write(f"HDF5 not found under {HDF5_ROOT}: libhydroexa_io.so not built\n")
+        return ""
+    os.makedirs(libdir, exist_ok=True)
+    out = os.path.join(libdir, "libhydroexa_io.so")
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-static-libstdc++",
+           "-static-libgcc",
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{inc}",
+           os.path.join(CSRC, "hx_io.cpp"), "-o", tmp,
+           f"-L{lib}", f"-Wl,-rpath,{lib}", "-lhdf5"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("building libhydroexa_io.so failed")
+    os.replace(tmp, out)
+    return out
+
+
 def build(verbose: bool = False, libdir: str = LIBDIR, defines=()) -> str:
     """Compile every HIP source and link the shared library (always).  The
     default target is the in-tree build; libdir/defines build variants."""
@@ -68,6 +98,8 @@ def build(verbose: bool = False, libdir: str = LIBDIR, defines=()) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("linking libhydroexa.so failed")
     os.replace(tmp, lib)
+    if libdir == LIBDIR and not defines:
+        build_io(verbose, libdir)
     return lib
 
 

@@ -60,3 +60,19 @@ def test_patch_struct_layout():
     assert L.HxPatch.pitch.offset == 16
     assert L.HxPatch.dx.offset == 72
     assert L.HxPatch.bc_hi.offset == 96
+
+
+def test_io_header_declarations_match_binding():
+    """include/hydroexa_io.h against libhydroexa_io.so and its binding."""
+    from hydroexa_amd import h5input as H
+    if not os.path.exists(H._IO_PATH):
+        pytest.skip("libhydroexa_io.so not built (no HDF5)")
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "hydroexa_io.h")).read(),
+                  flags=re.S)
+    names = sorted(set(re.findall(r"\b(hxio_[a-z_0-9]+)\s*\(", text)))
+    assert names == sorted(H.IO_EXPORTS)
+    lib = H.load_io()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert C.sizeof(H.HxioMeta) == 4 * 4 + 6 * 8
+    assert C.sizeof(H.HxioGeometry) == 4 * 4 + 7 * 8
