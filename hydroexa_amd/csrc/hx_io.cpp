@@ -1,17 +1,21 @@
-// hx_io.cpp -- HDF5 input schema of HydroEXA (reader, writer, geometry
-// contract) behind include/hydroexa_io.h.  Serial HDF5: the reference opens
-// the file with MPI-IO on every rank and reads independent hyperslabs
-// (include/io/HDF5_File.H:57-81); on one GPU one process reads.
+// hx_io.cpp -- HydroEXA's file formats behind include/hydroexa_io.h: the HDF5
+// input schema (reader, writer, geometry contract) and the native AMReX
+// plotfile writer.  Serial HDF5: the reference opens the file with MPI-IO on
+// every rank and reads independent hyperslabs (include/io/HDF5_File.H:57-81);
+// on one GPU one process reads.
 #include "../../include/hydroexa_io.h"
 
 #include <hdf5.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 namespace {
 
@@ -216,5 +220,173 @@ extern "C" int hxio_check_geometry(const hxio_meta* m, int ref, int bf, hxio_geo
     g.prob_hi[0] = m->x_hi;
     g.prob_hi[1] = m->y_hi;
     *out = g;
+    return HXIO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Native AMReX plotfile (the layout amrex::WriteMultiLevelPlotfile writes:
+// plotfile Header, then per level Level_<l>/Cell_H and Level_<l>/Cell_D_00000).
+// ---------------------------------------------------------------------------
+namespace {
+
+std::string box_str(int ilo, int jlo, int ihi, int jhi)
+{
+    return "((" + std::to_string(ilo) + "," + std::to_string(jlo) + ") (" + std::to_string(ihi) +
+           "," + std::to_string(jhi) + ") (0,0))";
+}
+
+// A double as an ostream with the given precision prints it (%g style).
+std::string gfmt(double v, int prec)
+{
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.*g", prec, v);
+    return buf;
+}
+
+// VisMF writes min/max in scientific notation with precision 15.
+std::string efmt(double v)
+{
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.15e", v);
+    return buf;
+}
+
+bool make_dir(const std::string& d)
+{
+    size_t pos = 0;
+    do {
+        pos = d.find('/', pos + 1);
+        const std::string part = d.substr(0, pos);
+        if (!part.empty()) {
+            struct stat st;
+            if (::stat(part.c_str(), &st) != 0) {
+                if (::mkdir(part.c_str(), 0755) != 0 && errno != EEXIST) return false;
+            } else if (!S_ISDIR(st.st_mode)) {
+                return false;
+            }
+        }
+    } while (pos != std::string::npos);
+    return true;
+}
+
+}  // namespace
+
+extern "C" int hxio_write_plotfile(const char* dir, int nlevels, const hxio_plot_level* levels,
+                                   int ncomp, const char* const* varnames, double time,
+                                   const double* prob_lo, const double* prob_hi,
+                                   const int32_t* ref_ratio)
+{
+    if (!dir || nlevels < 1 || !levels || ncomp < 1 || !varnames || !prob_lo || !prob_hi ||
+        (nlevels > 1 && !ref_ratio))
+        return HXIO_E_BADARG;
+    for (int l = 0; l < nlevels; ++l) {
+        if (levels[l].nboxes < 1 || !levels[l].boxes || !levels[l].data) return HXIO_E_BADARG;
+        for (int b = 0; b < levels[l].nboxes; ++b) {
+            const int32_t* bx = levels[l].boxes + 4 * b;
+            if (bx[2] < bx[0] || bx[3] < bx[1] || !levels[l].data[b]) return HXIO_E_BADARG;
+        }
+    }
+    const std::string root(dir);
+    if (!make_dir(root)) return HXIO_E_OPEN;
+    const int finest = nlevels - 1;
+    std::string h;
+    h += "HyperCLaw-V1.1\n";
+    h += std::to_string(ncomp) + "\n";
+    for (int c = 0; c < ncomp; ++c) h += std::string(varnames[c]) + "\n";
+    h += "2\n";
+    h += gfmt(time, 17) + "\n";
+    h += std::to_string(finest) + "\n";
+    h += gfmt(prob_lo[0], 17) + " " + gfmt(prob_lo[1], 17) + " \n";
+    h += gfmt(prob_hi[0], 17) + " " + gfmt(prob_hi[1], 17) + " \n";
+    for (int l = 0; l < finest; ++l) h += std::to_string(ref_ratio[l]) + " ";
+    h += "\n";
+    for (int l = 0; l < nlevels; ++l) {
+        const int32_t* d = levels[l].domain;
+        h += box_str(d[0], d[1], d[2], d[3]) + " ";
+    }
+    h += "\n";
+    for (int l = 0; l < nlevels; ++l) h += std::to_string(levels[l].step) + " ";
+    h += "\n";
+    for (int l = 0; l < nlevels; ++l)
+        h += gfmt(levels[l].dx, 17) + " " + gfmt(levels[l].dy, 17) + " \n";
+    h += "0\n";   // Cartesian coordinates
+    h += "0\n";   // boundary width
+    for (int l = 0; l < nlevels; ++l) {
+        const hxio_plot_level& L = levels[l];
+        h += std::to_string(l) + " " + std::to_string(L.nboxes) + " " + gfmt(time, 17) + "\n";
+        h += std::to_string(L.step) + "\n";
+        for (int b = 0; b < L.nboxes; ++b) {
+            const int32_t* bx = L.boxes + 4 * b;
+            const double xlo = prob_lo[0] + (bx[0] - L.domain[0]) * L.dx;
+            const double xhi = prob_lo[0] + (bx[2] + 1 - L.domain[0]) * L.dx;
+            const double ylo = prob_lo[1] + (bx[1] - L.domain[1]) * L.dy;
+            const double yhi = prob_lo[1] + (bx[3] + 1 - L.domain[1]) * L.dy;
+            h += gfmt(xlo, 17) + " " + gfmt(xhi, 17) + "\n";
+            h += gfmt(ylo, 17) + " " + gfmt(yhi, 17) + "\n";
+        }
+        h += "Level_" + std::to_string(l) + "/Cell\n";
+
+        // data file: one FAB per box, each a text header then the doubles
+        const std::string ldir = root + "/Level_" + std::to_string(l);
+        if (!make_dir(ldir)) return HXIO_E_OPEN;
+        FILE* fd = std::fopen((ldir + "/Cell_D_00000").c_str(), "wb");
+        if (!fd) return HXIO_E_OPEN;
+        std::vector<long> offs(L.nboxes);
+        std::vector<double> mn((size_t)L.nboxes * ncomp), mx((size_t)L.nboxes * ncomp);
+        for (int b = 0; b < L.nboxes; ++b) {
+            const int32_t* bx = L.boxes + 4 * b;
+            const size_t n = (size_t)(bx[2] - bx[0] + 1) * (bx[3] - bx[1] + 1);
+            offs[b] = std::ftell(fd);
+            const std::string fh = "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" +
+                                   box_str(bx[0], bx[1], bx[2], bx[3]) + " " +
+                                   std::to_string(ncomp) + "\n";
+            if (std::fwrite(fh.data(), 1, fh.size(), fd) != fh.size() ||
+                std::fwrite(L.data[b], sizeof(double), n * ncomp, fd) != n * ncomp) {
+                std::fclose(fd);
+                return HXIO_E_WRITE;
+            }
+            for (int c = 0; c < ncomp; ++c) {
+                const double* p = L.data[b] + c * n;
+                double lo = p[0], hi = p[0];
+                for (size_t k = 1; k < n; ++k) {
+                    lo = std::min(lo, p[k]);
+                    hi = std::max(hi, p[k]);
+                }
+                mn[(size_t)b * ncomp + c] = lo;
+                mx[(size_t)b * ncomp + c] = hi;
+            }
+        }
+        if (std::fclose(fd) != 0) return HXIO_E_WRITE;
+        // VisMF header (version 1, NFiles)
+        std::string m;
+        m += "1\n1\n" + std::to_string(ncomp) + "\n0\n";
+        m += "(" + std::to_string(L.nboxes) + " 0\n";
+        for (int b = 0; b < L.nboxes; ++b) {
+            const int32_t* bx = L.boxes + 4 * b;
+            m += box_str(bx[0], bx[1], bx[2], bx[3]) + "\n";
+        }
+        m += ")\n";
+        m += std::to_string(L.nboxes) + "\n";
+        for (int b = 0; b < L.nboxes; ++b)
+            m += "FabOnDisk: Cell_D_00000 " + std::to_string(offs[b]) + "\n";
+        m += "\n";
+        for (int pass = 0; pass < 2; ++pass) {
+            const std::vector<double>& v = pass == 0 ? mn : mx;
+            m += std::to_string(L.nboxes) + "," + std::to_string(ncomp) + "\n";
+            for (int b = 0; b < L.nboxes; ++b) {
+                for (int c = 0; c < ncomp; ++c) m += efmt(v[(size_t)b * ncomp + c]) + ",";
+                m += "\n";
+            }
+            m += "\n";
+        }
+        FILE* fm = std::fopen((ldir + "/Cell_H").c_str(), "w");
+        if (!fm) return HXIO_E_OPEN;
+        const bool okm = std::fwrite(m.data(), 1, m.size(), fm) == m.size();
+        if (std::fclose(fm) != 0 || !okm) return HXIO_E_WRITE;
+    }
+    FILE* fh = std::fopen((root + "/Header").c_str(), "w");
+    if (!fh) return HXIO_E_OPEN;
+    const bool okh = std::fwrite(h.data(), 1, h.size(), fh) == h.size();
+    if (std::fclose(fh) != 0 || !okh) return HXIO_E_WRITE;
     return HXIO_OK;
 }

@@ -64,7 +64,7 @@ class HxioGeometry(C.Structure):
 
 
 IO_EXPORTS = ("hxio_read_metadata", "hxio_read_components", "hxio_write_input",
-              "hxio_check_geometry")
+              "hxio_check_geometry", "hxio_write_plotfile")
 
 
 def load_io():

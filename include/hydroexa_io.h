@@ -84,6 +84,32 @@ typedef struct hxio_geometry {
 int hxio_check_geometry(const hxio_meta *meta, int terrain_ref_lev, int blocking_factor,
                         hxio_geometry *out, char *msg, int msglen);
 
+/*
+ * One level of a plotfile: nboxes boxes {ilo, jlo, ihi, jhi} in the level's
+ * index space, and per box ncomp planes of its cells (component-major, i
+ * fastest), host memory.
+ */
+typedef struct hxio_plot_level {
+    int32_t nboxes;
+    int32_t domain[4];            /* {0, 0, nx-1, ny-1}                     */
+    int32_t step;                 /* level step count                        */
+    const int32_t *boxes;         /* nboxes x 4                              */
+    const double *const *data;    /* nboxes pointers                         */
+    double dx, dy;
+} hxio_plot_level;
+
+/*
+ * Native AMReX plotfile (HyperCLaw-V1.1 Header, VisMF Cell_H version 1,
+ * one Cell_D_00000 data file per level), as amrex::WriteMultiLevelPlotfile
+ * writes it.  Replaces WriteNativePlotfile (src/io/NativePlotFile.cpp:19-97),
+ * which calls it with varnames h_fluid, hu_momentum, hv_momentum,
+ * z_bathymetry, extra_comp_4 and the directory name prefix + the iteration
+ * in 5 digits.  ref_ratio: nlevels-1 ratios.  Creates dir.
+ */
+int hxio_write_plotfile(const char *dir, int nlevels, const hxio_plot_level *levels, int ncomp,
+                        const char *const *varnames, double time, const double *prob_lo,
+                        const double *prob_hi, const int32_t *ref_ratio);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,3 +76,5 @@ def test_io_header_declarations_match_binding():
         assert hasattr(lib, n), n
     assert C.sizeof(H.HxioMeta) == 4 * 4 + 6 * 8
     assert C.sizeof(H.HxioGeometry) == 4 * 4 + 7 * 8
+    from hydroexa_amd.plotfile import HxioPlotLevel
+    assert C.sizeof(HxioPlotLevel) == 6 * 4 + 2 * 8 + 2 * 8
