@@ -45,12 +45,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time the other workloads")
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
-    ap.add_argument("--only", choices=["ensemble", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "adjoint", "two_level"], default=None,
                     help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="steps of the 2048^2 calibration rollout timed forward + adjoint (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=120,
                     help="timed steps of the batched ensemble and of the per-scenario loop (0: skip)")
+    ap.add_argument("--two-level-steps", type=int, default=100,
+                    help="timed coarse steps of the two-level dam break (0: skip)")
     return ap.parse_args()
 
 
@@ -254,6 +256,85 @@ def time_adjoint(steps, every=20, arith="fast"):
     return out
 
 
+def time_two_level(steps, arith="fast"):
+    """Two-level coupled step (ratio 2, subcycled): the 4096^2 FP64 dam break
+    with the 1024^2 coarse cells around the dam refined (2048^2 fine cells),
+    plain Roe.  One step = dt from both levels' CFL, coarse register +
+    advance, two fine substeps (ghost fill, register, advance), reflux,
+    average_down.  Timed as the device-resident run (hx_two_level_run) and
+    as host-driven steps (CFL read back per level each step).  Composite
+    cell updates per step = coarse cells + 2 x fine cells.  Also times the
+    coupling kernels alone (fills, registers, reflux, average_down) with
+    HIP events."""
+    import ctypes as C
+    import torch
+    from hydroexa_amd import Level, TwoLevel, set_arith
+    from hydroexa_amd import _lib as L
+    from hydroexa_amd import workloads as W
+    from hydroexa_amd.solver import context
+
+    set_arith(arith)
+    w = W.dam_break(4096, 1)
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    box = (1536, 1536, 1024, 1024)
+    T = TwoLevel("Roe", lv, box)
+    T.run(3, w.cfl)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # device-resident run (hx_two_level_run)
+    t0 = time.perf_counter()
+    e0.record(st)
+    T.run(steps, w.cfl)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / steps
+    # host-driven steps (per-level CFL read back each step), for comparison
+    hs = max(steps // 4, 1)
+    e0.record(st)
+    for _ in range(hs):
+        T.step(w.cfl)
+    e1.record(st)
+    torch.cuda.synchronize()
+    host_ms = e0.elapsed_time(e1) / hs
+    composite = lv.nx * lv.ny + 2 * T.fine.nx * T.fine.ny
+    # coupling kernels alone, on the final state
+    lib, ctx, sp = L.load(), context(), C.c_void_p(st.cuda_stream)
+    c, f = T.crse, T.fine
+    reps = 20
+    e0.record(st)
+    for _ in range(reps):
+        cp = c.patch(c.U_new)
+        T._flux(cp, T.cfaces, 1, -1e-3 * c.dy, -1e-3 * c.dx, 0)
+        for k in range(2):
+            L.check(lib.hx_fill_fine_ghosts(ctx, C.byref(c.patch(c.U_old)), C.byref(cp), k / 2.0,
+                                            2, box[0], box[1], C.byref(f.patch()), 0,
+                                            f._ptr(f.U_new, 0), f._ptr(f.U_new, 1),
+                                            f._ptr(f.U_new, 2), sp), "fill")
+            fp = f.patch(f.U_new)
+            T._flux(fp, T.ffaces, 2, 5e-4 * f.dy, 5e-4 * f.dx, 1)
+        L.check(lib.hx_average_down_state(ctx, C.byref(f.patch()), 2, box[0], box[1],
+                                          C.byref(cp), c.U_new[0].data_ptr(),
+                                          c.U_new[1].data_ptr(), c.U_new[2].data_ptr(), sp),
+                "average_down")
+    e1.record(st)
+    torch.cuda.synchronize()
+    coupling_us = e0.elapsed_time(e1) / reps * 1e3
+    set_arith("exact")
+    out = {"workload": "two-level dam break: 4096^2 FP64 coarse, 1024^2-cell box at ratio 2 "
+                       "(2048^2 fine), plain Roe, subcycled, CFL per level each step",
+           "arith": arith, "steps": steps, "composite_cells_per_step": composite,
+           "ms_per_step": ms, "wall_ms_per_step": wall / steps * 1e3,
+           "composite_Mcups": composite / (ms / 1e3) / 1e6,
+           "host_stepped_ms_per_step": host_ms,
+           "coupling_kernels_us_per_step": coupling_us,
+           "coupling_share": coupling_us / 1e3 / ms}
+    del T, lv
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -269,6 +350,10 @@ def main():
         return
     if args.only == "adjoint":
         print(json.dumps({"adjoint": time_adjoint(args.adjoint_steps, 20, args.arith)}), flush=True)
+        return
+    if args.only == "two_level":
+        print(json.dumps({"two_level": time_two_level(args.two_level_steps, args.arith)}),
+              flush=True)
         return
     if args.solver == "Roe":
         w = W.dam_break(args.n, args.steps)
@@ -341,6 +426,8 @@ def main():
         out["ensemble"] = time_ensemble(args.ensemble_steps, args.arith)
     if args.adjoint_steps > 0:
         out["adjoint"] = time_adjoint(args.adjoint_steps, 20, args.arith)
+    if args.two_level_steps > 0:
+        out["two_level"] = time_two_level(args.two_level_steps, args.arith)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W.dam_break(args.n, 1) if args.solver == "Roe" else w,
                                            args.cpu_seconds)

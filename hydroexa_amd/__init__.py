@@ -15,8 +15,9 @@ from .solver import (  # noqa: F401
     set_tuning,
 )
 from .ensemble import Ensemble  # noqa: F401,E402
+from .amr import TwoLevel  # noqa: F401,E402
 from . import _lib  # noqa: F401
 from . import workloads  # noqa: F401,E402
 
-__all__ = ["Ensemble", "Level", "SolverVariant", "masked_average_down", "post_process_base_grids",
+__all__ = ["Ensemble", "Level", "TwoLevel", "SolverVariant", "masked_average_down", "post_process_base_grids",
            "set_tuning", "set_arith", "workloads"]

@@ -41,7 +41,8 @@ EXPORTS = (
     "hx_prune_boxes", "hx_run_level", "hx_set_tuning", "hx_set_arith",
     "hx_ensemble_create", "hx_ensemble_destroy", "hx_ensemble_set_tuning", "hx_ensemble_run",
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
-    "hx_conservation",
+    "hx_conservation", "hx_face_fluxes", "hx_fill_fine_ghosts", "hx_reflux",
+    "hx_average_down_state", "hx_two_level_run",
 )
 
 
@@ -105,6 +106,15 @@ def load() -> C.CDLL:
                                     C.POINTER(C.c_int64), C.POINTER(vp)]
     L.hx_adjoint_destroy.argtypes = [vp]
     L.hx_conservation.argtypes = [vp, pp, dp, vp]
+    L.hx_face_fluxes.argtypes = [vp, C.c_int, pp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
+                                 C.c_int, vp, vp]
+    L.hx_two_level_run.argtypes = [vp, C.c_int, pp, vp, vp, vp, pp, vp, vp, vp, C.c_int, C.c_int,
+                                   C.c_int, C.c_double, C.c_int, vp, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int32), vp]
+    L.hx_fill_fine_ghosts.argtypes = [vp, pp, pp, C.c_double, C.c_int, C.c_int, C.c_int, pp,
+                                      C.c_int, vp, vp, vp, vp]
+    L.hx_reflux.argtypes = [vp, pp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
+    L.hx_average_down_state.argtypes = [vp, pp, C.c_int, C.c_int, C.c_int, pp, vp, vp, vp, vp]
     L.hx_adjoint_forward.argtypes = [vp, pp, C.c_double, dp, vp, vp, vp, vp, i32p, vp]
     L.hx_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     _lib = L

@@ -27,19 +27,38 @@ namespace hx {
 
 using namespace exact;
 
-// CFL reduction of a stored state (compute_dt_Impl's ReduceOps pass).
+// CFL reduction of a stored state (compute_dt_Impl's ReduceOps pass).  A
+// thread owns one column of a run of kCflRows rows (blockIdx.y); the loads of
+// kCflUnroll rows are issued before any is used.
+constexpr int kCflRows = 16;
+constexpr int kCflUnroll = 4;
+
 template <typename R>
 __global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy, R rdx, R rdy,
                                                      int mul, CflSlot* slot)
 {
     CflAcc<R> acc;
     acc.init();
-    const int64_t ncell = (int64_t)p.nx * p.ny;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ncell;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int j = (int)(q / p.nx), i = (int)(q - (int64_t)j * p.nx);
-        const int64_t s = (int64_t)j * p.pitch + i;
-        acc.add(p.h[s], p.hu[s], p.hv[s], p.z[s], dx, dy, rdx, rdy, mul != 0);
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int j0 = blockIdx.y * kCflRows;
+    const int j1 = min(j0 + kCflRows, p.ny);
+    if (i < p.nx) {
+        int j = j0;
+        for (; j + kCflUnroll <= j1; j += kCflUnroll) {
+            R h[kCflUnroll], qx[kCflUnroll], qy[kCflUnroll], z[kCflUnroll];
+#pragma unroll
+            for (int u = 0; u < kCflUnroll; ++u) {
+                const int64_t s = (int64_t)(j + u) * p.pitch + i;
+                h[u] = p.h[s]; qx[u] = p.hu[s]; qy[u] = p.hv[s]; z[u] = p.z[s];
+            }
+#pragma unroll
+            for (int u = 0; u < kCflUnroll; ++u)
+                acc.add(h[u], qx[u], qy[u], z[u], dx, dy, rdx, rdy, mul != 0);
+        }
+        for (; j < j1; ++j) {
+            const int64_t s = (int64_t)j * p.pitch + i;
+            acc.add(p.h[s], p.hu[s], p.hv[s], p.z[s], dx, dy, rdx, rdy, mul != 0);
+        }
     }
     cfl_commit(acc, slot);
 }
@@ -326,6 +345,145 @@ __global__ __launch_bounds__(kConsBlocks) void cons_final_kernel(const double* p
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two-level coupling for one fine box at refinement ratio 2 (the reference's
+// AMR level coupling: src/state/AmrMeshState_AMR.cpp:667-844,
+// src/state/AmrMeshState_Physics.cpp:50-147, and AMReX's FillPatchTwoLevels,
+// FluxRegister and average_down that it calls).  The fine box covers coarse
+// cells [ci0, ci0 + ncx) x [cj0, cj0 + ncy); its state and terrain carry
+// 2 stored ghost layers.
+// ---------------------------------------------------------------------------
+
+// Coarse sample for the fine ghost fill: eta = h + z, and the momenta, at a
+// coarse index that may lie outside the level (physical-BC source cell,
+// normal momentum negated across a reflective side).
+template <typename R>
+__device__ __forceinline__ void coarse_q(const PatchView<R>& c, int i, int j, R& eta, R& qx, R& qy)
+{
+    int si, ei, sj, ej;
+    bool fx, fy;
+    map_axis(i, c.nx, c.bl0, c.bh0, si, ei, fx);
+    map_axis(j, c.ny, c.bl1, c.bh1, sj, ej, fy);
+    const int64_t s = (int64_t)sj * c.pitch + si;
+    eta = c.h[s] + c.z[s];
+    const R u = c.hu[s], v = c.hv[s];
+    qx = fx ? -u : u;
+    qy = fy ? -v : v;
+}
+
+template <typename R>
+__device__ __forceinline__ void coarse_q_t(const PatchView<R>& o, const PatchView<R>& n, R alpha,
+                                           int i, int j, R* q)
+{
+    R a[3], b[3];
+    coarse_q(o, i, j, a[0], a[1], a[2]);
+    if (alpha == R(0)) { q[0] = a[0]; q[1] = a[1]; q[2] = a[2]; return; }
+    coarse_q(n, i, j, b[0], b[1], b[2]);
+    if (alpha == R(1)) { q[0] = b[0]; q[1] = b[1]; q[2] = b[2]; return; }
+    for (int k = 0; k < 3; ++k) q[k] = (R(1) - alpha) * a[k] + alpha * b[k];
+}
+
+// Monotonized-central limited slope of a coarse cell.
+template <typename R>
+__device__ __forceinline__ R mc_slope(R m, R c, R p)
+{
+    const R cen = R(0.5) * (p - m), fw = p - c, bw = c - m;
+    const R lim = (fw * bw > R(0)) ? R(2) * fmin(fabs(fw), fabs(bw)) : R(0);
+    return copysign(fmin(fabs(cen), lim), cen);
+}
+
+// One thread per fine ghost cell (2 layers around the box, corners too):
+// conservative linear interpolation of eta, hu, hv from the time-blended
+// coarse parent, then h = max(0, eta - z) on the fine ghost's own bed.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, PatchView<R> cn,
+                                                          R alpha, int ci0, int cj0, int nxf,
+                                                          int nyf, int64_t fpitch, int whole,
+                                                          const R* __restrict__ zf, R* hf, R* huf,
+                                                          R* hvf, const int* err)
+{
+    if (err && *(volatile const int*)err) return;
+    const int W = nxf + 4, H = nyf + 4;
+    const int n = whole ? W * H : W * H - nxf * nyf;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    // whole: every cell of the box and its ghosts.  Otherwise the ghost ring:
+    // rows -2, -1 and ny, ny+1 in full, then the side columns
+    int fi, fj;
+    if (whole) { fj = t / W - 2; fi = t % W - 2; }
+    else if (t < 2 * W) { fj = t / W - 2; fi = t % W - 2; }
+    else if (t < 4 * W) { const int u = t - 2 * W; fj = nyf + u / W; fi = u % W - 2; }
+    else { const int u = t - 4 * W; fj = u / 4; const int k = u % 4; fi = (k < 2) ? k - 2 : nxf + k - 2; }
+    const int pi = ci0 + (fi >= 0 ? fi / 2 : -((1 - fi) / 2));
+    const int pj = cj0 + (fj >= 0 ? fj / 2 : -((1 - fj) / 2));
+    const R ox = ((fi & 1) == 0) ? R(-0.25) : R(0.25);
+    const R oy = ((fj & 1) == 0) ? R(-0.25) : R(0.25);
+    R qc[3], qw[3], qe[3], qs[3], qn[3];
+    coarse_q_t(co, cn, alpha, pi, pj, qc);
+    coarse_q_t(co, cn, alpha, pi - 1, pj, qw);
+    coarse_q_t(co, cn, alpha, pi + 1, pj, qe);
+    coarse_q_t(co, cn, alpha, pi, pj - 1, qs);
+    coarse_q_t(co, cn, alpha, pi, pj + 1, qn);
+    R v[3];
+    for (int k = 0; k < 3; ++k)
+        v[k] = qc[k] + mc_slope(qw[k], qc[k], qe[k]) * ox + mc_slope(qs[k], qc[k], qn[k]) * oy;
+    const int64_t o = (int64_t)fj * fpitch + fi;
+    const R x = v[0] - zf[o];
+    hf[o] = (R(0) < x) ? x : R(0);
+    huf[o] = v[1];
+    hvf[o] = v[2];
+}
+
+// Reflux: register q of coarse face q holds -dt dA F_coarse + sum over the
+// fine substeps and fine faces of dt_f dA_f F_fine.  The coarse cell outside
+// the box beside the face gains (-reg)/vol on the box's low sides and
+// reg/vol on its high sides, vol = dx dy (AMReX FluxRegister::Reflux).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void reflux_kernel(int ci0, int cj0, int ncx, int ncy,
+                                                        int64_t pitch, R vol,
+                                                        const R* __restrict__ reg, R* h, R* hu,
+                                                        R* hv, const int* err)
+{
+    if (err && *(volatile const int*)err) return;
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    const int nq = 2 * ncy + 2 * ncx;
+    if (q >= nq) return;
+    int i, j;
+    R sgn;
+    if (q < ncy) { i = ci0 - 1; j = cj0 + q; sgn = R(-1); }
+    else if (q < 2 * ncy) { i = ci0 + ncx; j = cj0 + q - ncy; sgn = R(1); }
+    else if (q < 2 * ncy + ncx) { i = ci0 + q - 2 * ncy; j = cj0 - 1; sgn = R(-1); }
+    else { i = ci0 + q - 2 * ncy - ncx; j = cj0 + ncy; sgn = R(1); }
+    const int64_t o = (int64_t)j * pitch + i;
+    const R* r = reg + 3 * q;
+    h[o] += (sgn * r[0]) / vol;
+    hu[o] += (sgn * r[1]) / vol;
+    hv[o] += (sgn * r[2]) / vol;
+}
+
+// Covered coarse cells from the fine cells: c = sum of the 2 x 2 children
+// (jref outer, iref inner) times 1/4, AMReX average_down for cell data.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void avgdown_state_kernel(int ci0, int cj0, int ncx, int ncy,
+                                                               int64_t fpitch, const R* fh,
+                                                               const R* fhu, const R* fhv,
+                                                               int64_t cpitch, R* h, R* hu, R* hv,
+                                                               const int* err)
+{
+    if (err && *(volatile const int*)err) return;
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= ncx * ncy) return;
+    const int ic = q % ncx, jc = q / ncx;
+    const R* src[3] = {fh, fhu, fhv};
+    R* dst[3] = {h, hu, hv};
+    for (int k = 0; k < 3; ++k) {
+        R c = R(0);
+        for (int jr = 0; jr < 2; ++jr)
+            for (int ir = 0; ir < 2; ++ir) c += src[k][(int64_t)(2 * jc + jr) * fpitch + 2 * ic + ir];
+        dst[k][(int64_t)(cj0 + jc) * cpitch + ci0 + ic] = R(0.25) * c;
+    }
+}
+
 }  // namespace hx
 
 // ===========================================================================
@@ -484,11 +642,10 @@ extern "C" int hx_advance_level(hx_ctx* c, int solver, const hx_patch* in, doubl
 template <typename R>
 static int launch_cfl(const hx_patch* in, CflSlot* slot, hipStream_t st)
 {
-    const int64_t ncell = (int64_t)in->nx * in->ny;
-    int64_t nblk = (ncell + kBlock - 1) / kBlock;
-    if (nblk > 4096) nblk = 4096;
     const int mul = is_pow2(in->dx) && is_pow2(in->dy) ? 1 : 0;
-    hipLaunchKernelGGL(cfl_kernel<R>, dim3((unsigned)nblk), dim3(kBlock), 0, st, view_of<R>(in),
+    const dim3 grid((unsigned)((in->nx + kBlock - 1) / kBlock),
+                    (unsigned)((in->ny + kCflRows - 1) / kCflRows));
+    hipLaunchKernelGGL(cfl_kernel<R>, grid, dim3(kBlock), 0, st, view_of<R>(in),
                        (R)in->dx, (R)in->dy, (R)(1.0 / in->dx), (R)(1.0 / in->dy), mul, slot);
     return hip_status(hipGetLastError());
 }
@@ -1019,4 +1176,351 @@ extern "C" int hx_ensemble_run(hx_ensemble* e, double cfl, int nsteps, int use_g
         any = any || err[s];
     }
     return any ? HX_E_DT_COLLAPSE : HX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Two-level coupling entry points
+// ---------------------------------------------------------------------------
+static int launch_faces(const hx_ctx* c, int solver, const hx_patch* in, const FaceReq& q,
+                        void* out, hipStream_t st)
+{
+    int rc;
+    if (in->dtype == HX_F64) {
+        const PatchView<double> v = view_of<double>(in);
+        rc = (c->arith == HX_ARITH_FAST)
+                 ? fast::launch_bflux<double>(solver, v, in->dx, in->dy, q, (double*)out, st)
+                 : exact::launch_bflux<double>(solver, v, in->dx, in->dy, q, (double*)out, st);
+    } else {
+        const PatchView<float> v = view_of<float>(in);
+        rc = (c->arith == HX_ARITH_FAST)
+                 ? fast::launch_bflux<float>(solver, v, (float)in->dx, (float)in->dy, q,
+                                            (float*)out, st)
+                 : exact::launch_bflux<float>(solver, v, (float)in->dx, (float)in->dy, q,
+                                             (float*)out, st);
+    }
+    return rc ? HX_E_HIP : HX_OK;
+}
+
+extern "C" int hx_face_fluxes(hx_ctx* c, int solver, const hx_patch* in, int nfaces,
+                              const int32_t* faces, int pair, double scale_x, double scale_y,
+                              int accumulate, void* out, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    int rc = check_patch(in, solver == HX_SOLVER_ROE ? 1 : 2);
+    if (rc) return rc;
+    if (nfaces < 0 || (nfaces > 0 && (!faces || !out)) || (pair != 1 && pair != 2) ||
+        nfaces % pair)
+        return HX_E_BADARG;
+    if (nfaces == 0) return HX_OK;
+    HX_TRY(hipSetDevice(c->device));
+    FaceReq q{};
+    q.faces = faces;
+    q.nfaces = nfaces;
+    q.pair = pair;
+    q.accumulate = accumulate != 0;
+    q.sx = scale_x;
+    q.sy = scale_y;
+    return launch_faces(c, solver, in, q, out, (hipStream_t)stream);
+}
+
+template <typename R>
+static int fillfine_t(const hx_patch* co, const hx_patch* cn, double alpha, int ci0, int cj0,
+                      const hx_patch* f, int whole, void* h, void* hu, void* hv, hipStream_t st,
+                      const int* err = nullptr)
+{
+    const int W = f->nx + 4, H = f->ny + 4;
+    const int n = whole ? W * H : W * H - f->nx * f->ny;
+    hipLaunchKernelGGL(fillfine_kernel<R>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       view_of<R>(co), view_of<R>(cn), (R)alpha, ci0, cj0, f->nx, f->ny, f->pitch,
+                       whole, (const R*)f->z, (R*)h, (R*)hu, (R*)hv, err);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int hx_fill_fine_ghosts(hx_ctx* c, const hx_patch* co, const hx_patch* cn, double alpha,
+                                   int ratio, int ci0, int cj0, const hx_patch* fine, int whole,
+                                   void* h, void* hu, void* hv, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    int rc = check_patch(co, 0);
+    if (!rc) rc = check_patch(cn, 0);
+    if (rc) return rc;
+    if (!fine || ratio != 2 || fine->ng < 2 || fine->nx % 2 || fine->ny % 2 || !h || !hu || !hv ||
+        !fine->z || fine->dtype != co->dtype || cn->dtype != co->dtype ||
+        fine->pitch < (int64_t)fine->nx + 4 || !(alpha >= 0.0 && alpha <= 1.0) ||
+        co->nx != cn->nx || co->ny != cn->ny)
+        return HX_E_BADARG;
+    // the coarse stencil of the ghost ring (parents +- 1) must map into the
+    // coarse level: parents lie within one coarse cell of the box
+    const int ncx = fine->nx / 2, ncy = fine->ny / 2;
+    if (ci0 < 0 || cj0 < 0 || ci0 + ncx > co->nx || cj0 + ncy > co->ny) return HX_E_BADARG;
+    for (int d = 0; d < 2; ++d)
+        if (co->bc_lo[d] == HX_BC_GHOST || co->bc_hi[d] == HX_BC_GHOST) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    return (co->dtype == HX_F64)
+               ? fillfine_t<double>(co, cn, alpha, ci0, cj0, fine, whole != 0, h, hu, hv, st)
+               : fillfine_t<float>(co, cn, alpha, ci0, cj0, fine, whole != 0, h, hu, hv, st);
+}
+
+template <typename R>
+static int reflux_t(const hx_patch* crse, void* h, void* hu, void* hv, int ci0, int cj0, int ncx,
+                    int ncy, const void* reg, hipStream_t st, const int* err = nullptr)
+{
+    const int nq = 2 * (ncx + ncy);
+    hipLaunchKernelGGL(reflux_kernel<R>, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ci0,
+                       cj0, ncx, ncy, crse->pitch, (R)(crse->dx * crse->dy), (const R*)reg, (R*)h,
+                       (R*)hu, (R*)hv, err);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int hx_reflux(hx_ctx* c, const hx_patch* crse, void* h, void* hu, void* hv, int ci0,
+                         int cj0, int ncx, int ncy, const void* reg, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    int rc = check_patch(crse, 0);
+    if (rc) return rc;
+    if (!h || !hu || !hv || !reg || ncx < 1 || ncy < 1 || ci0 < 1 || cj0 < 1 ||
+        ci0 + ncx >= crse->nx || cj0 + ncy >= crse->ny)
+        return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    return (crse->dtype == HX_F64) ? reflux_t<double>(crse, h, hu, hv, ci0, cj0, ncx, ncy, reg, st)
+                                   : reflux_t<float>(crse, h, hu, hv, ci0, cj0, ncx, ncy, reg, st);
+}
+
+template <typename R>
+static int avgdown_t(const hx_patch* fine, int ci0, int cj0, const hx_patch* crse, void* h,
+                     void* hu, void* hv, hipStream_t st, const int* err = nullptr)
+{
+    const int ncx = fine->nx / 2, ncy = fine->ny / 2;
+    hipLaunchKernelGGL(avgdown_state_kernel<R>, dim3((ncx * ncy + kBlock - 1) / kBlock), dim3(kBlock),
+                       0, st, ci0, cj0, ncx, ncy, fine->pitch, (const R*)fine->h,
+                       (const R*)fine->hu, (const R*)fine->hv, crse->pitch, (R*)h, (R*)hu, (R*)hv,
+                       err);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int hx_average_down_state(hx_ctx* c, const hx_patch* fine, int ratio, int ci0, int cj0,
+                                     const hx_patch* crse, void* h, void* hu, void* hv,
+                                     void* stream)
+{
+    if (!c) return HX_E_STATE;
+    int rc = check_patch(fine, 0);
+    if (!rc) rc = check_patch(crse, 0);
+    if (rc) return rc;
+    if (ratio != 2 || fine->nx % 2 || fine->ny % 2 || !h || !hu || !hv ||
+        fine->dtype != crse->dtype)
+        return HX_E_BADARG;
+    const int ncx = fine->nx / 2, ncy = fine->ny / 2;
+    if (ci0 < 0 || cj0 < 0 || ci0 + ncx > crse->nx || cj0 + ncy > crse->ny) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    return (crse->dtype == HX_F64) ? avgdown_t<double>(fine, ci0, cj0, crse, h, hu, hv, st)
+                                   : avgdown_t<float>(fine, ci0, cj0, crse, h, hu, hv, st);
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident two-level run: the coupled step of the entry points above
+// repeated without host synchronisation.  dt comes from the CFL reductions
+// of both levels through the multi-level rule, on the device: the coarse
+// advance reduces its output outside the box grown by one cell, a pass over
+// the grown box reduces the cells reflux and average_down changed, and the
+// last fine substep reduces the fine level.
+// ---------------------------------------------------------------------------
+
+// dt of a coupled step from the two levels' slots (include/solvers/BaseSolver.H:120-165
+// at ratio 2; lockstep shares the finer dt, src/state/AmrMeshState_Physics.cpp:13-45),
+// the same arithmetic as hx_level_dt + that rule on the host.  Clears the
+// slots this step reduces into; on an estimate below 1e-12 on either level
+// records the step in err and leaves dt alone (later kernels skip).
+template <typename R>
+__global__ void tl_dt_kernel(const CflSlot* rc, const CflSlot* rf, CflSlot* wc, CflSlot* wf,
+                             double cfl, int subcycle, int step, double* dt, double* hist, int* err)
+{
+    if (threadIdx.x != 0 || *err) return;
+    const R ec = slot_dt_est<R>(rc->amax, rc->flags);
+    const R ef = slot_dt_est<R>(rf->amax, rf->flags);
+    if ((double)ec < 1.e-12 || (double)ef < 1.e-12) {
+        *err = step + 1;
+        return;
+    }
+    double d0, d1;
+    if (sizeof(R) == 8) {
+        d0 = (double)ec * cfl;
+        d1 = (double)ef * cfl;
+    } else {
+        d0 = (double)(ec * (R)cfl);
+        d1 = (double)(ef * (R)cfl);
+    }
+    const double t0 = (d1 * 2.0 < d0) ? d1 * 2.0 : d0;
+    const double a = subcycle ? t0 : t0 / 2.0, b = t0 / 2.0;
+    dt[0] = a;
+    dt[1] = b;
+    hist[2 * step] = a;
+    hist[2 * step + 1] = b;
+    wc->amax = 0ull; wc->vmax = 0ull; wc->flags = 0u;
+    wf->amax = 0ull; wf->vmax = 0ull; wf->flags = 0u;
+}
+
+struct TwoLevelWork {
+    CflSlot slots[4];   // coarse read/write, fine read/write
+    double dt[2];
+    int err;
+    int pad;
+};
+
+template <typename R>
+static int two_level_t(hx_ctx* c, int solver, const hx_patch* crse, void* const* calt,
+                       const hx_patch* fine, void* const* falt, int ci0, int cj0, int subcycle,
+                       double cfl, int nsteps, void* reg, TwoLevelWork* wk, double* hist,
+                       hipStream_t st)
+{
+    const int ncx = fine->nx / 2, ncy = fine->ny / 2;
+    CflSlot* slots = wk->slots;
+    double* dt = wk->dt;
+    int* err = &wk->err;
+    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, slots, 4, err);
+    int rc = launch_cfl<R>(crse, slots + 0, st);
+    if (!rc) rc = launch_cfl<R>(fine, slots + 2, st);
+    if (rc) return rc;
+    void* cb[2][3] = {{(void*)crse->h, (void*)crse->hu, (void*)crse->hv},
+                      {calt[0], calt[1], calt[2]}};
+    void* fb[2][3] = {{(void*)fine->h, (void*)fine->hu, (void*)fine->hv},
+                      {falt[0], falt[1], falt[2]}};
+    StepArgs<R> ac{};
+    ac.p = view_of<R>(crse);      // fill_geometry sizes the strips from p
+    ac.opitch = crse->pitch;
+    ac.cfl = cfl;
+    ac.err = err;
+    ac.dt_dev = dt;
+    ac.sk_i0 = ci0 - 1; ac.sk_i1 = ci0 + ncx + 1;
+    ac.sk_j0 = cj0 - 1; ac.sk_j1 = cj0 + ncy + 1;
+    fill_geometry(ac, solver, c->rows[solver][crse->dtype], crse->dx, crse->dy);
+    StepArgs<R> af{};
+    af.p = view_of<R>(fine);
+    af.opitch = fine->pitch;
+    af.cfl = cfl;
+    af.err = err;
+    af.dt_dev = dt + 1;
+    fill_geometry(af, solver, c->rows[solver][fine->dtype], fine->dx, fine->dy);
+    FaceReq qc{};
+    qc.nfaces = 2 * (ncx + ncy); qc.pair = 1; qc.accumulate = 0; qc.box_fine = 0;
+    qc.ci0 = ci0; qc.cj0 = cj0; qc.ncx = ncx; qc.ncy = ncy;
+    qc.sx = -crse->dy; qc.sy = -crse->dx; qc.dt = dt; qc.err = err;
+    FaceReq qf = qc;
+    qf.nfaces = 4 * (ncx + ncy); qf.pair = 2; qf.accumulate = 1; qf.box_fine = 1;
+    qf.sx = fine->dy; qf.sy = fine->dx; qf.dt = dt + 1;
+    const int nsub = subcycle ? 2 : 1;
+    int fcur = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        const int r = s & 1, w = r ^ 1;
+        hipLaunchKernelGGL(tl_dt_kernel<R>, dim3(1), dim3(64), 0, st, slots + r, slots + 2 + r,
+                           slots + w, slots + 2 + w, cfl, subcycle, s, dt, hist, err);
+        hx_patch cold = *crse, cnew = *crse;
+        cold.h = cb[r][0]; cold.hu = cb[r][1]; cold.hv = cb[r][2];
+        cnew.h = cb[w][0]; cnew.hu = cb[w][1]; cnew.hv = cb[w][2];
+        // coarse register from the step's input, then the coarse advance
+        rc = launch_faces(c, solver, &cold, qc, reg, st);
+        if (rc) return rc;
+        ac.p = view_of<R>(&cold);
+        ac.ho = (R*)cb[w][0]; ac.huo = (R*)cb[w][1]; ac.hvo = (R*)cb[w][2];
+        ac.acc = slots + w;
+        ac.step = s;
+        rc = launch(c, solver, crse->dtype, ac, st);
+        if (rc) return rc;
+        for (int k = 0; k < nsub; ++k) {
+            hx_patch fp = *fine;
+            fp.h = fb[fcur][0]; fp.hu = fb[fcur][1]; fp.hv = fb[fcur][2];
+            rc = fillfine_t<R>(&cold, &cnew, k / 2.0, ci0, cj0, &fp, 0, fb[fcur][0], fb[fcur][1],
+                               fb[fcur][2], st, err);
+            if (!rc) rc = launch_faces(c, solver, &fp, qf, reg, st);
+            if (rc) return rc;
+            af.p = view_of<R>(&fp);
+            af.ho = (R*)fb[fcur ^ 1][0]; af.huo = (R*)fb[fcur ^ 1][1]; af.hvo = (R*)fb[fcur ^ 1][2];
+            af.acc = (k == nsub - 1) ? slots + 2 + w : nullptr;
+            af.step = s;
+            rc = launch(c, solver, fine->dtype, af, st);
+            if (rc) return rc;
+            fcur ^= 1;
+        }
+        rc = reflux_t<R>(&cnew, cb[w][0], cb[w][1], cb[w][2], ci0, cj0, ncx, ncy, reg, st, err);
+        hx_patch ffin = *fine;
+        ffin.h = fb[fcur][0]; ffin.hu = fb[fcur][1]; ffin.hv = fb[fcur][2];
+        if (!rc) rc = avgdown_t<R>(&ffin, ci0, cj0, &cnew, cb[w][0], cb[w][1], cb[w][2], st, err);
+        if (rc) return rc;
+        // CFL of the grown box (covered cells, the reflux ring and its corners)
+        hx_patch gb = cnew;
+        const int64_t off = (int64_t)(cj0 - 1) * crse->pitch + (ci0 - 1);
+        const size_t es = (crse->dtype == HX_F64) ? 8 : 4;
+        gb.h = (const char*)cnew.h + off * es;
+        gb.hu = (const char*)cnew.hu + off * es;
+        gb.hv = (const char*)cnew.hv + off * es;
+        gb.z = (const char*)crse->z + off * es;
+        gb.n = (const char*)crse->n + off * es;
+        gb.nx = ncx + 2;
+        gb.ny = ncy + 2;
+        rc = launch_cfl<R>(&gb, slots + w, st);
+        if (rc) return rc;
+    }
+    return HX_OK;
+}
+
+extern "C" int hx_two_level_run(hx_ctx* c, int solver, const hx_patch* crse, void* ch, void* chu,
+                                void* chv, const hx_patch* fine, void* fh, void* fhu, void* fhv,
+                                int ci0, int cj0, int subcycle, double cfl, int nsteps, void* reg,
+                                double* dt_hist, int32_t* steps_done, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    int rc = check_patch(crse, 0);
+    if (!rc) rc = check_patch(fine, 2);
+    if (rc) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if (crse->bc_lo[d] == HX_BC_GHOST || crse->bc_hi[d] == HX_BC_GHOST) return HX_E_BADARG;
+        if (fine->bc_lo[d] != HX_BC_GHOST || fine->bc_hi[d] != HX_BC_GHOST) return HX_E_BADARG;
+    }
+    if (crse->eta || fine->eta || !ch || !chu || !chv || !fh || !fhu || !fhv || !reg ||
+        nsteps < 0 || !(cfl > 0.0) || fine->dtype != crse->dtype || fine->ng < 2 ||
+        fine->nx % 2 || fine->ny % 2 || fine->nx < 2 || fine->ny < 2 ||
+        fine->dx != 0.5 * crse->dx || fine->dy != 0.5 * crse->dy)
+        return HX_E_BADARG;
+    const int ncx = fine->nx / 2, ncy = fine->ny / 2;
+    if (ci0 < 1 || cj0 < 1 || ci0 + ncx > crse->nx - 1 || cj0 + ncy > crse->ny - 1)
+        return HX_E_BADARG;
+    if (steps_done) *steps_done = 0;
+    if (nsteps == 0) return HX_OK;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    TwoLevelWork* wk = nullptr;
+    double* hist = nullptr;
+    HX_TRY(hipMalloc((void**)&wk, sizeof(TwoLevelWork)));
+    if (hipMalloc((void**)&hist, sizeof(double) * 2 * (size_t)nsteps) != hipSuccess) {
+        (void)hipFree(wk);
+        return HX_E_HIP;
+    }
+    void* calt[3] = {ch, chu, chv};
+    void* falt[3] = {fh, fhu, fhv};
+    rc = (crse->dtype == HX_F64)
+             ? two_level_t<double>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0, cfl,
+                                   nsteps, reg, wk, hist, st)
+             : two_level_t<float>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0, cfl,
+                                  nsteps, reg, wk, hist, st);
+    int err = 0;
+    if (!rc && hipMemcpyAsync(&err, &wk->err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = HX_E_HIP;
+    if (!rc && dt_hist &&
+        hipMemcpyAsync(dt_hist, hist, sizeof(double) * 2 * (size_t)nsteps, hipMemcpyDeviceToHost,
+                       st) != hipSuccess)
+        rc = HX_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = HX_E_HIP;
+    (void)hipFree(wk);
+    (void)hipFree(hist);
+    if (rc) return rc;
+    if (err) {
+        if (steps_done) *steps_done = err - 1;
+        return HX_E_DT_COLLAPSE;
+    }
+    if (steps_done) *steps_done = nsteps;
+    return HX_OK;
 }

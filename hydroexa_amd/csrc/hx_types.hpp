@@ -81,6 +81,28 @@ struct StepArgs {
     const R* lam;
     R* lun;
     R* gn;
+    // dt of this step in device memory (a multi-level rule computed it);
+    // used when non-null, with err checked first
+    const double* dt_dev = nullptr;
+    // output cells [sk_i0, sk_i1) x [sk_j0, sk_j1) are left out of the CFL
+    // reduction (a later pass reduces them after they change)
+    int sk_i0 = 0, sk_j0 = 0, sk_i1 = 0, sk_j1 = 0;
+};
+
+// Faces whose effective fluxes build a flux register: a device list of
+// {dir, i, j, side}, or (faces == null) the faces around the fine box of
+// coarse cells [ci0, ci0 + ncx) x [cj0, cj0 + ncy) -- box_fine 0: the
+// coarse faces x-low, x-high, y-low, y-high with the coarse side's view;
+// box_fine 1: the two fine faces of each, in the fine box's own indices,
+// with the fine side's view.  Scales per direction, times *dt when dt is
+// non-null.
+struct FaceReq {
+    const int* faces;
+    int nfaces, pair, accumulate;
+    int box_fine, ci0, cj0, ncx, ncy;
+    double sx, sy;
+    const double* dt;
+    const int* err;     // non-null and set: nothing is done (a stopped run)
 };
 
 // Occupancy variants compiled for the advance kernel (minimum waves per
@@ -93,11 +115,17 @@ namespace exact {
 template <typename R>
 int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
 int launch_adjpost(int occ, const StepArgs<double>& a, hipStream_t st);
+template <typename R>
+int launch_bflux(int solver, const PatchView<R>& p, R dx, R dy, const FaceReq& q, R* out,
+                 hipStream_t st);
 }
 namespace fast {
 template <typename R>
 int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
 int launch_adjpost(int occ, const StepArgs<double>& a, hipStream_t st);
+template <typename R>
+int launch_bflux(int solver, const PatchView<R>& p, R dx, R dy, const FaceReq& q, R* out,
+                 hipStream_t st);
 }
 
 }  // namespace hx

@@ -228,6 +228,94 @@ int hx_ensemble_run(hx_ensemble *ens, double cfl, int nsteps, int use_graph, dou
                     int32_t *steps_done, void *stream);
 
 /*
+ * Two-level coupling for one fine box at refinement ratio 2: the pieces of
+ * the reference's AMR level coupling (src/state/AmrMeshState_Physics.cpp:50-147,
+ * src/state/AmrMeshState_AMR.cpp:667-844) and of the AMReX operations it
+ * calls (FillPatchTwoLevels, FluxRegister, average_down).  The fine box
+ * covers coarse cells [ci0, ci0 + nx_f/2) x [cj0, cj0 + ny_f/2); the fine
+ * patch stores 2 ghost layers (HX_BC_GHOST sides) and its terrain ghosts.
+ *
+ * hx_face_fluxes: effective fluxes at listed faces of a level, re-evaluated
+ * with the advance's face functions from the state the advance read --
+ * plain Roe D+ = F + S+ (side 0, the low cell's view) or D- = F - S-
+ * (side 1), the reference's side selection (src/solvers/Roe.cpp:440-575);
+ * hydrostatic 0.5 (FL + FR) (src/solvers/HydrostaticRoe.cpp:612-627).
+ * faces (device): nfaces x {dir, i, j, side}, x face (i, j) between cells
+ * (i-1, j) and (i, j), 0 <= i <= nx, 0 <= j < ny (y faces alike; a face
+ * outside that range yields NaN).  out (device, level dtype): nfaces/pair
+ * rows of 3 values; row q receives scale F of faces q*pair .. q*pair+pair-1
+ * (scale_x on x faces, scale_y on y faces), stored (accumulate 0) or added
+ * one face at a time (accumulate 1).  With scales -dt dy, -dt dx on the
+ * coarse faces (pair 1, FluxRegister::CrseInit COPY) and dt_f dy_f,
+ * dt_f dx_f on the fine faces (pair 2, accumulate, FineAdd) this builds a
+ * flux register.
+ */
+int hx_face_fluxes(hx_ctx *ctx, int solver, const hx_patch *level, int nfaces,
+                   const int32_t *faces, int pair, double scale_x, double scale_y,
+                   int accumulate, void *out, void *stream);
+
+/*
+ * Fine ghost cells (whole = 0) or the whole fine box with its ghosts
+ * (whole = 1, the initial fill) from the coarse level at coarse time blend
+ * alpha in [0, 1] (state (1 - alpha) crse_old + alpha crse_new; exactly the
+ * old or new state at 0 and 1): eta = h + z and the momenta by conservative
+ * linear interpolation with monotonized-central limited slopes (amrex
+ * cell_cons_interp), then h = max(0, eta - z) on the fine cell's own bed
+ * (fine->z, ghosts included).  h, hu, hv: the fine state's valid (0,0)
+ * elements (writable, fine->pitch).
+ */
+int hx_fill_fine_ghosts(hx_ctx *ctx, const hx_patch *crse_old, const hx_patch *crse_new,
+                        double alpha, int ratio, int ci0, int cj0, const hx_patch *fine,
+                        int whole, void *h, void *hu, void *hv, void *stream);
+
+/*
+ * Reflux (AMReX FluxRegister::Reflux): reg (device) holds one register per
+ * coarse face around the box, in the order x-low (ncy faces), x-high (ncy),
+ * y-low (ncx), y-high (ncx); the coarse cell outside each face gains
+ * -reg/(dx dy) on the low sides and +reg/(dx dy) on the high sides.
+ */
+int hx_reflux(hx_ctx *ctx, const hx_patch *crse, void *h, void *hu, void *hv, int ci0, int cj0,
+              int ncx, int ncy, const void *reg, void *stream);
+
+/*
+ * Covered coarse cells from the fine valid cells: the 2 x 2 children summed
+ * j-outer, i-inner, times 1/4 (amrex::average_down of the state; the
+ * reference's AverageDownTo, src/state/AmrMeshState_AMR.cpp:836-844).
+ */
+int hx_average_down_state(hx_ctx *ctx, const hx_patch *fine, int ratio, int ci0, int cj0,
+                          const hx_patch *crse, void *h, void *hu, void *hv, void *stream);
+
+/*
+ * nsteps coupled steps of one coarse level and one fine box at ratio 2,
+ * without host synchronisation: the reference's recursive subcycled step
+ * (AmrMeshState::TimeStepWithSubcycling, src/state/AmrMeshState_Physics.cpp:50-147)
+ * with ComputeDt (:13-45) for two levels, built from the four calls above
+ * and the fused advance.  Per step: dt from both levels' CFL by the
+ * multi-level rule (lockstep: one shared dt), coarse register + advance,
+ * 2 fine substeps (1 in lockstep) of ghost fill + register + advance,
+ * reflux, average_down.  dt is computed on the device; the coarse advance
+ * reduces the CFL of its output outside the box grown by one cell, which a
+ * separate pass reduces after reflux and average_down.
+ *
+ * crse: coarse state (physical sides); ch/chu/chv its alternate buffer
+ * (crse pitch).  fine: fine box patch (ng >= 2, HX_BC_GHOST on every side,
+ * terrain with its ghosts, dx = crse dx / 2), covering coarse cells
+ * [ci0, ci0 + nx/2) x [cj0, cj0 + ny/2), one coarse cell away from every
+ * domain edge; fh/fhu/fhv its alternate buffer (valid (0,0), fine pitch,
+ * ghost storage included).  reg (device): 6 (ncx + ncy) values of the level
+ * dtype.  dt_hist (host, optional): 2 x nsteps (coarse dt, fine dt).
+ * After the call the coarse state is in crse's buffer when *steps_done is
+ * even, else in ch/chu/chv; the fine state is in fine's buffer when the
+ * number of fine substeps done (2 per step subcycled, 1 lockstep) is even.
+ * Returns HX_E_DT_COLLAPSE (with *steps_done) when either level's estimate
+ * falls below 1e-12.  Synchronises the stream at the end.
+ */
+int hx_two_level_run(hx_ctx *ctx, int solver, const hx_patch *crse, void *ch, void *chu,
+                     void *chv, const hx_patch *fine, void *fh, void *fhu, void *fhv, int ci0,
+                     int cj0, int subcycle, double cfl, int nsteps, void *reg, double *dt_hist,
+                     int32_t *steps_done, void *stream);
+
+/*
  * Adjoint of a single-level rollout (FP64).  No reference counterpart:
  * HydroEXA computes no derivatives.  The rollout is nsteps of the step
  * behind hx_advance_level (plain Roe or hydrostatic Roe with Manning

@@ -23,6 +23,20 @@ Parity pinning status per function (the checks live in tests/test_oracle_*.py):
                           brute-force numpy restatement, NODATA skip.
 * ``masked_average_down`` pinned: closed forms (all valid, 50 % rule, NaN).
 * ``prune_boxes``        pinned: brute-force numpy restatement.
+* ``step_padded``        pinned: equals ``step`` when the ghosts hold the
+                          physical halo.
+* ``amr`` (two-level coupling, one fine box at ratio 2):
+  - ``fill_fine``        pinned: children average to the parent, linear data
+                          reproduced, flat at an extremum, time blends.
+  - ``face_flux``        pinned: the side-selected fluxes reproduce the Roe
+                          update; hydrostatic mass flux drives the depth update.
+  - ``crse_init`` / ``fine_add`` / ``reflux`` / ``average_down``: pinned by
+                          composite mass conservation with reflux (and its
+                          failure without), uniform flow, lake at rest, child
+                          means.  The exact time-blend weight and summation
+                          order of AMReX's FillPatchTwoLevels / FluxRegister
+                          are restated from its published algorithm: parity
+                          unpinned at the last bit (AMReX is absent).
 * AMReX ghost-fill internals (``FilccCell``): restated, parity unpinned (no
   reference test or vector covers them).
 """

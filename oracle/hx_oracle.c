@@ -410,14 +410,12 @@ static void build_work(const pad_t *p, const double *h, const double *hu, const 
 /* fluctuation-difference update).  U_o's depth is overwritten by the       */
 /* rebuilt h before the update (:339), so the update starts from it.        */
 /* ------------------------------------------------------------------------ */
-static void roe_step_ws(int nx, int ny, const double *h, const double *hu, const double *hv,
-                        const double *z, const double *n, double dt, double dx, double dy,
-                        const int *bclo, const int *bchi, double *ho, double *huo, double *hvo,
-                        work_t *wp)
+static void roe_core(const pad_t *pp, const work_t *wp, double dt, double dx, double dy,
+                     double *ho, double *huo, double *hvo)
 {
-    pad_t p = {nx, ny, nx + 2 * G};
-    work_t w = *wp;
-    build_work(&p, h, hu, hv, z, n, bclo, bchi, &w);
+    const pad_t p = *pp;
+    const work_t w = *wp;
+    const int nx = p.nx, ny = p.ny;
     const size_t nfx = (size_t)(nx + 1) * ny, nfy = (size_t)nx * (ny + 1);
     double *fb = w.faces;
     double *Fx = fb, *Smx = Fx + 3 * nfx, *Spx = Smx + 3 * nfx;
@@ -460,6 +458,16 @@ static void roe_step_ws(int nx, int ny, const double *h, const double *hu, const
         }
 }
 
+static void roe_step_ws(int nx, int ny, const double *h, const double *hu, const double *hv,
+                        const double *z, const double *n, double dt, double dx, double dy,
+                        const int *bclo, const int *bchi, double *ho, double *huo, double *hvo,
+                        work_t *wp)
+{
+    pad_t p = {nx, ny, nx + 2 * G};
+    build_work(&p, h, hu, hv, z, n, bclo, bchi, wp);
+    roe_core(&p, wp, dt, dx, dy, ho, huo, hvo);
+}
+
 int hxo_roe_step(int nx, int ny, const double *h, const double *hu, const double *hv,
                  const double *z, const double *n, double dt, double dx, double dy,
                  const int *bclo, const int *bchi, double *ho, double *huo, double *hvo)
@@ -478,14 +486,14 @@ int hxo_roe_step(int nx, int ny, const double *h, const double *hu, const double
 /* Restates src/solvers/HydrostaticRoe.cpp:400-606.  The update starts from */
 /* U_o as stored (its depth is not rebuilt; the rebuild goes to U_ext).     */
 /* ------------------------------------------------------------------------ */
-static void hydro_step_ws(int nx, int ny, const double *h, const double *hu, const double *hv,
-                          const double *z, const double *n, double dt, double dx, double dy,
-                          const int *bclo, const int *bchi, double *ho, double *huo, double *hvo,
-                          work_t *wp)
+/* Stored state of the update: element (i, j) at so + j*sp + i. */
+static void hydro_core(const pad_t *pp, const work_t *wp, const double *h, const double *hu,
+                       const double *hv, const double *n, size_t sp, size_t so, double dt,
+                       double dx, double dy, double *ho, double *huo, double *hvo)
 {
-    pad_t p = {nx, ny, nx + 2 * G};
-    work_t w = *wp;
-    build_work(&p, h, hu, hv, z, n, bclo, bchi, &w);
+    const pad_t p = *pp;
+    const work_t w = *wp;
+    const int nx = p.nx, ny = p.ny;
     const size_t nfx = (size_t)(nx + 1) * ny, nfy = (size_t)nx * (ny + 1);
     double *fb = w.faces;
     double *FLx = fb, *FRx = FLx + 3 * nfx, *FLy = FRx + 3 * nfx, *FRy = FLy + 3 * nfy;
@@ -515,10 +523,10 @@ static void hydro_step_ws(int nx, int ny, const double *h, const double *hu, con
     #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
-            const size_t s = (size_t)j * nx + i;
+            const size_t s = (size_t)j * nx + i, si = so + (size_t)j * sp + i;
             const size_t xl = (size_t)j * (nx + 1) + i, xr = xl + 1;
             const size_t yl = (size_t)j * nx + i, yr = yl + nx;
-            const double uo[3] = {h[s], hu[s], hv[s]};
+            const double uo[3] = {h[si], hu[si], hv[si]};
             double un[3];
             for (int k = 0; k < 3; ++k)                          /* :554-564 */
                 un[k] = uo[k]
@@ -527,7 +535,7 @@ static void hydro_step_ws(int nx, int ny, const double *h, const double *hu, con
             const double hn = un[0];
             if (hn >= TOLDRY) {                                  /* :566-595 */
                 const double qx = un[1], qy = un[2];
-                const double nsq = n[s] * n[s];
+                const double nsq = n[si] * n[si];
                 const double inv_h = 1.0 / hn;
                 const double u = qx * inv_h, v = qy * inv_h;
                 const double modM = sqrt(u * u + v * v);
@@ -548,6 +556,16 @@ static void hydro_step_ws(int nx, int ny, const double *h, const double *hu, con
         }
 }
 
+static void hydro_step_ws(int nx, int ny, const double *h, const double *hu, const double *hv,
+                          const double *z, const double *n, double dt, double dx, double dy,
+                          const int *bclo, const int *bchi, double *ho, double *huo, double *hvo,
+                          work_t *wp)
+{
+    pad_t p = {nx, ny, nx + 2 * G};
+    build_work(&p, h, hu, hv, z, n, bclo, bchi, wp);
+    hydro_core(&p, wp, h, hu, hv, n, (size_t)nx, 0, dt, dx, dy, ho, huo, hvo);
+}
+
 int hxo_hydro_step(int nx, int ny, const double *h, const double *hu, const double *hv,
                    const double *z, const double *n, double dt, double dx, double dy,
                    const int *bclo, const int *bchi, double *ho, double *huo, double *hvo)
@@ -557,6 +575,36 @@ int hxo_hydro_step(int nx, int ny, const double *h, const double *hu, const doub
     work_t w;
     if (work_alloc(&p, &w)) return HXO_E_NOMEM;
     hydro_step_ws(nx, ny, h, hu, hv, z, n, dt, dx, dy, bclo, bchi, ho, huo, hvo, &w);
+    work_free(&w);
+    return HXO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Step of a box whose 2 ghost layers are given (a fine level inside a      */
+/* coarse one: FillPatchTwoLevels wrote them).  Inputs are padded           */
+/* (ny+4) x (nx+4), pitch nx+4, valid cell (0,0) at 2*(nx+4)+2; eta = h + z */
+/* and h = max(0, eta - z) over the whole padded region, then the same      */
+/* faces and update as above.  Outputs are the nx x ny valid cells.         */
+/* ------------------------------------------------------------------------ */
+int hxo_step_padded(int solver, int nx, int ny, const double *h, const double *hu,
+                    const double *hv, const double *z, const double *n, double dt, double dx,
+                    double dy, double *ho, double *huo, double *hvo)
+{
+    if (nx < 2 || ny < 2 || (solver != 0 && solver != 1)) return HXO_E_BADARG;
+    pad_t p = {nx, ny, nx + 2 * G};
+    work_t w;
+    if (work_alloc(&p, &w)) return HXO_E_NOMEM;
+    const size_t np = (size_t)p.P * (ny + 2 * G);
+    for (size_t k = 0; k < np; ++k) {
+        w.eta[k] = h[k] + z[k];
+        w.hu[k] = hu[k]; w.hv[k] = hv[k]; w.z[k] = z[k]; w.n[k] = n[k];
+        w.he[k] = smax(0.0, w.eta[k] - w.z[k]);
+    }
+    if (solver == 0)
+        roe_core(&p, &w, dt, dx, dy, ho, huo, hvo);
+    else
+        hydro_core(&p, &w, h, hu, hv, n, (size_t)p.P, (size_t)G * p.P + G, dt, dx, dy, ho, huo,
+                   hvo);
     work_free(&w);
     return HXO_OK;
 }
