@@ -187,14 +187,15 @@ __device__ __forceinline__ RoeCell<R> roe_cell(R hr, R qx, R qy, R z, R n)
     const bool wet = hd > R(0);
     c.hd = hd;
 #if HX_FAST
-    // one reciprocal square root gives sqrt(h), 1/h and sqrt(g h)
+    // one reciprocal square root gives sqrt(h), 1/h and sqrt(g h); a dry
+    // cell takes 1/h = 0 and sqrt(h) = 0 (two selects instead of four)
     R s, r;
     sqrt_rsq(wet ? hd : R(1), s, r);
-    const R ih = r * r;
+    const R ih = wet ? r * r : R(0);
     c.sq = wet ? s : R(0);
-    c.u = wet ? qx * ih : R(0);
-    c.v = wet ? qy * ih : R(0);
-    c.cg = wet ? SqrtG<R>::v * s : R(0);
+    c.u = qx * ih;
+    c.v = qy * ih;
+    c.cg = SqrtG<R>::v * c.sq;
 #else
     c.sq = wet ? sqrt(hd) : R(0);
     const Div<R> ih(wet ? hd : R(1));
@@ -227,7 +228,7 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
     const R utr = (DIR == 0) ? B.v : -B.u;
 
     const R hmean = R(0.5) * (hl + hr);
-    const R ht = (hmean > k::toldry) ? hmean : k::toldry;
+    const R ht = fmax(hmean, k::toldry);   // = (hmean > TOLDRY) ? hmean : TOLDRY
 #if HX_FAST
     // ct and 1/ct from one reciprocal square root; 1/(2 ct + TOL12) from
     // 1/ct to first order in TOL12/(2 ct) (relative error below 1e-19)
@@ -296,11 +297,14 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
     }
     R b2 = -b0;
 
-    {   // positivity guard, Roe.H:200-247 (with b0 = 0, b/lambda is 0)
-        const R l0s = (fabs(l0) > k::tol12) ? l0 : (l0 >= R(0) ? k::tol12 : -k::tol12);
-        const R l2s = (fabs(l2) > k::tol12) ? l2 : (l2 >= R(0) ? k::tol12 : -k::tol12);
+    {   // positivity guard, Roe.H:200-247 (with b0 = 0, b/lambda is 0; the
+        // clamped speeds are formed only where they are used)
+        const auto clamp_speed = [](R l) {
+            return (fabs(l) > k::tol12) ? l : (l >= R(0) ? k::tol12 : -k::tol12);
+        };
         R h1s = hl + a0, h3s = hr - a2;
         if (!flat) {
+            const R l0s = clamp_speed(l0), l2s = clamp_speed(l2);
 #if HX_FAST
             const R il = rcp(l0s * l2s);   // |l0s|, |l2s| >= TOL12: no underflow
             h1s = h1s - b0 * (l2s * il);
@@ -319,6 +323,7 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
             const R loc_dt = Div<R>(mlam + k::tol12)(ds);
             const R m1 = -(hl + a0) * fabs(l0);
             const R m3 = -(hr - a2) * l2;
+            const R l0s = clamp_speed(l0), l2s = clamp_speed(l2);
             if (c1) {
                 const R d1 = hl - h1s;
                 const R d1s = (fabs(d1) > k::tol12) ? d1 : (d1 >= R(0) ? k::tol12 : -k::tol12);
@@ -567,8 +572,10 @@ struct CflAcc {
 #endif
             const R sx = fabs(u) + c, sy = fabs(v) + c;
             const R a = mul ? (sx * rdx + sy * rdy) : (sx / dx + sy / dy);
-            amax = smax(amax, a);
-            vmax = smax(vmax, smax(fabs(u), fabs(v)));
+            // v_max_f64: the same value as the select for these non-negative
+            // operands (a NaN operand leaves the other, as the select does)
+            amax = fmax(amax, a);
+            vmax = fmax(vmax, fmax(fabs(u), fabs(v)));
             flags |= CFL_WET;
         } else {
             flags |= CFL_DRY;
