@@ -42,7 +42,8 @@ EXPORTS = (
     "hx_ensemble_create", "hx_ensemble_destroy", "hx_ensemble_set_tuning", "hx_ensemble_run",
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
     "hx_conservation", "hx_face_fluxes", "hx_fill_fine_ghosts", "hx_reflux",
-    "hx_average_down_state", "hx_two_level_run",
+    "hx_average_down_state", "hx_two_level_run", "hx_fill_fine_region", "hx_reflux_side",
+    "hx_boxes_dt",
 )
 
 
@@ -108,6 +109,11 @@ def load() -> C.CDLL:
     L.hx_conservation.argtypes = [vp, pp, dp, vp]
     L.hx_face_fluxes.argtypes = [vp, C.c_int, pp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                  C.c_int, vp, vp]
+    L.hx_fill_fine_region.argtypes = [vp, pp, pp, C.c_double, C.c_int, C.c_int, C.c_int, pp,
+                                      C.c_int, vp, C.c_int64, vp, vp, vp, vp]
+    L.hx_reflux_side.argtypes = [vp, pp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp,
+                                 C.c_int, vp]
+    L.hx_boxes_dt.argtypes = [vp, C.c_int, C.c_int, pp, C.c_double, dp, dp, vp]
     L.hx_two_level_run.argtypes = [vp, C.c_int, pp, vp, vp, vp, pp, vp, vp, vp, C.c_int, C.c_int,
                                    C.c_int, C.c_double, C.c_int, vp, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int32), vp]

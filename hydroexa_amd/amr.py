@@ -24,7 +24,7 @@ least one coarse cell away from the domain edge, no regridding.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Optional, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -33,6 +33,7 @@ from . import _lib as L
 from .solver import MODELS, _DT, Level, context
 
 NG = 2
+Box = Tuple[int, int, int, int]
 
 
 def _stream():
@@ -231,3 +232,407 @@ class TwoLevel:
         """Sum of h dA over the coarse level (covered cells hold the average
         of their fine children after each step)."""
         return float(self.crse.U_new[0].double().sum().item()) * self.crse.dx * self.crse.dy
+
+
+# ===========================================================================
+# Multi-level, multi-box hierarchy
+# ===========================================================================
+class DenseLevel:
+    """One level held as a full-domain array with 2 cells of padding on each
+    side: U_new / U_old (3, ny + 4, nx + 4), terrain (2, ny + 4, nx + 4), and
+    a byte mask of the cells the level's boxes own.  Boxes read their ghost
+    rings straight from the array: another box's valid cells where the level
+    has them, values interpolated from the coarser level elsewhere, physical
+    boundary rules outside the domain."""
+
+    def __init__(self, nx: int, ny: int, dx: float, dy: float, bc_lo, bc_hi, dtype):
+        self.nx, self.ny, self.dx, self.dy, self.dtype = nx, ny, dx, dy, dtype
+        self.bc_lo, self.bc_hi = tuple(bc_lo), tuple(bc_hi)
+        shape = (ny + 2 * NG, nx + 2 * NG)
+        self.U_new = torch.zeros((3,) + shape, dtype=dtype, device="cuda")
+        self.U_old = torch.zeros((3,) + shape, dtype=dtype, device="cuda")
+        self.terrain = torch.zeros((2,) + shape, dtype=dtype, device="cuda")
+        self.mask = torch.zeros(shape, dtype=torch.uint8, device="cuda")
+        self.boxes: List[Box] = []
+        self.t_new = 0.0
+        self.t_old = -1.0e200
+        self.faces: List[Tuple[torch.Tensor, torch.Tensor]] = []
+
+    @property
+    def pitch(self) -> int:
+        return self.nx + 2 * NG
+
+    def addr(self, t: torch.Tensor, comp: int, i: int, j: int) -> int:
+        """Address of element (comp, j, i), i and j in level index space."""
+        plane = (self.ny + 2 * NG) * self.pitch
+        return t.data_ptr() + (comp * plane + (j + NG) * self.pitch + (i + NG)) * t.element_size()
+
+    def mask_addr(self, m: torch.Tensor, i: int, j: int) -> int:
+        return m.data_ptr() + (j + NG) * self.pitch + (i + NG)
+
+    def patch(self, U: torch.Tensor, box: Optional[Box] = None) -> L.HxPatch:
+        """The whole level (physical sides) or one box: sides on the domain
+        boundary keep the physical rule, the others read stored ghosts."""
+        i0, j0, w, h = box if box is not None else (0, 0, self.nx, self.ny)
+        p = L.HxPatch()
+        p.nx, p.ny, p.ng, p.dtype = w, h, NG, _DT[self.dtype]
+        p.pitch = self.pitch
+        p.h, p.hu, p.hv = (self.addr(U, k, i0, j0) for k in range(3))
+        p.eta = None
+        p.z, p.n = self.addr(self.terrain, 0, i0, j0), self.addr(self.terrain, 1, i0, j0)
+        p.dx, p.dy = self.dx, self.dy
+        p.bc_lo[0] = self.bc_lo[0] if i0 == 0 else L.HX_BC_GHOST
+        p.bc_lo[1] = self.bc_lo[1] if j0 == 0 else L.HX_BC_GHOST
+        p.bc_hi[0] = self.bc_hi[0] if i0 + w == self.nx else L.HX_BC_GHOST
+        p.bc_hi[1] = self.bc_hi[1] if j0 + h == self.ny else L.HX_BC_GHOST
+        return p
+
+    def interior(self, t: torch.Tensor) -> torch.Tensor:
+        return t[..., NG:NG + self.ny, NG:NG + self.nx]
+
+    def set_boxes(self, boxes: List[Box]) -> None:
+        self.boxes = [tuple(int(v) for v in b) for b in boxes]
+        self.mask.zero_()
+        for (i0, j0, w, h) in self.boxes:
+            self.mask[NG + j0:NG + j0 + h, NG + i0:NG + i0 + w] = 1
+        self.faces = []
+        for (i0, j0, w, h) in self.boxes:
+            ci0, cj0, ncx, ncy = i0 // 2, j0 // 2, w // 2, h // 2
+            cx = [(0, ci0, cj0 + m, 0) for m in range(ncy)] + \
+                 [(0, ci0 + ncx, cj0 + m, 1) for m in range(ncy)]
+            cy = [(1, ci0 + m, cj0, 0) for m in range(ncx)] + \
+                 [(1, ci0 + m, cj0 + ncy, 1) for m in range(ncx)]
+            fx = [(0, 0, 2 * m + s, 1) for m in range(ncy) for s in (0, 1)] + \
+                 [(0, w, 2 * m + s, 0) for m in range(ncy) for s in (0, 1)]
+            fy = [(1, 2 * m + s, 0, 1) for m in range(ncx) for s in (0, 1)] + \
+                 [(1, 2 * m + s, h, 0) for m in range(ncx) for s in (0, 1)]
+            T = lambda a: torch.as_tensor(np.asarray(a, dtype=np.int32).reshape(-1, 4), device="cuda")
+            self.faces.append((T(cx + cy), T(fx + fy)))
+
+    def swap(self) -> None:
+        self.U_new, self.U_old = self.U_old, self.U_new
+
+
+def _pad_edges(a: np.ndarray) -> np.ndarray:
+    """(ny, nx) -> (ny + 4, nx + 4), edge values replicated (foextrap)."""
+    return np.pad(a, NG, mode="edge")
+
+
+def _pc_refine(a: np.ndarray) -> np.ndarray:
+    return np.repeat(np.repeat(a, 2, axis=0), 2, axis=1)
+
+
+def _almost_equal(x: float, y: float) -> bool:
+    return abs(x - y) <= 1e-12 * max(abs(x), abs(y), 1e-300)
+
+
+class AmrHierarchy:
+    """Levels 0..max_level at refinement ratio 2, each a list of boxes,
+    advanced by the reference's recursive step (src/state/AmrMeshState_Physics.cpp:50-147):
+    dt from every level's CFL (include/solvers/BaseSolver.H:120-165; lockstep
+    shares the finest dt), regrid every regrid_int steps of a level from its
+    gradient tags (hydroexa_amd.regrid), ghost rings from the coarser level
+    at the substep's time, flux registers, reflux in AMReX's side order, and
+    average_down.
+
+    static(level) may return the level's initial (h, hu, hv, z, n) on the
+    whole domain at that level's resolution (the reference's static input
+    layer); otherwise new levels take their terrain piecewise-constant and
+    their state by interpolation from the coarser level.  boxes maps a level
+    to a fixed box list (the level is then not regridded)."""
+
+    def __init__(self, model: str, h, hu, hv, z, n, dx: float, dy: float,
+                 bc_lo=(0, 0), bc_hi=(0, 0), max_level: int = 1, regrid_int: int = 0,
+                 blocking_factor: int = 8, max_grid_size: int = 128, n_error_buf: int = 1,
+                 grid_eff: float = 0.7, nest: int = 2,
+                 tag_thresh: Sequence[Tuple[float, float, float]] = ((0.0, 0.0, 0.0),),
+                 subcycle: bool = True, do_reflux: bool = True, dtype=torch.float64,
+                 static=None, boxes: Optional[dict] = None):
+        if model not in MODELS:
+            raise ValueError(f"unknown model {model!r}")
+        self.model, self.solver = model, MODELS[model]
+        self.max_level, self.regrid_int = max_level, regrid_int
+        self.bf, self.mgs, self.nbuf = blocking_factor, max_grid_size, n_error_buf
+        self.eff, self.nest = grid_eff, max(nest, 2)
+        self.tag_thresh = list(tag_thresh)
+        self.subcycle, self.do_reflux, self.dtype = subcycle, do_reflux, dtype
+        self.static, self.fixed = static, dict(boxes or {})
+        self.bc_lo, self.bc_hi = tuple(bc_lo), tuple(bc_hi)
+        h = np.asarray(h, np.float64)
+        ny, nx = h.shape
+        lv0 = DenseLevel(nx, ny, dx, dy, bc_lo, bc_hi, dtype)
+        self._load(lv0, h, hu, hv, z, n)
+        lv0.set_boxes([(0, 0, nx, ny)])
+        self.levels: List[DenseLevel] = [lv0]
+        self.dt = [0.0] * (max_level + 1)
+        self.istep = [0] * (max_level + 1)
+        self.last_regrid = [0] * (max_level + 1)
+        self.regs: List[List[torch.Tensor]] = [[] for _ in range(max_level + 1)]
+        self.time = 0.0
+        self.iteration = 0
+        self._make_grids(0, 0.0, from_scratch=True)
+
+    # -- construction ----------------------------------------------------
+    def _load(self, lv: DenseLevel, h, hu, hv, z, n) -> None:
+        T = lambda a: torch.as_tensor(_pad_edges(np.asarray(a, np.float64)), dtype=self.dtype,
+                                      device="cuda")
+        lv.terrain[0].copy_(T(z))
+        lv.terrain[1].copy_(T(n))
+        for k, a in enumerate((h, hu, hv)):
+            lv.U_new[k].copy_(T(a))
+
+    @property
+    def finest(self) -> int:
+        return len(self.levels) - 1
+
+    def _new_level(self, lev: int) -> DenseLevel:
+        c = self.levels[lev - 1]
+        lv = DenseLevel(2 * c.nx, 2 * c.ny, c.dx / 2, c.dy / 2, self.bc_lo, self.bc_hi, self.dtype)
+        st = self.static(lev) if self.static is not None else None
+        if st is not None:
+            self._load(lv, *st)
+            lv.from_static = True
+        else:
+            zc = c.interior(c.terrain).double().cpu().numpy()
+            lv.terrain.copy_(torch.as_tensor(np.stack([_pad_edges(_pc_refine(zc[k])) for k in range(2)]),
+                                             dtype=self.dtype, device="cuda"))
+            lv.from_static = False
+        return lv
+
+    def _make_grids(self, lbase: int, time: float, from_scratch: bool = False) -> None:
+        """New grids for levels lbase + 1 .. max_level (AmrMesh::MakeNewGrids +
+        MakeNewLevelFromScratch / MakeNewLevelFromCoarse / RemakeLevel)."""
+        lib = L.load()
+        for k in range(lbase, self.max_level):
+            if k > self.finest:
+                break
+            lv = self.levels[k]
+            if (k + 1) in self.fixed:
+                new = list(self.fixed[k + 1])
+                if not from_scratch and k + 1 <= self.finest:
+                    continue
+            else:
+                if k > 0:
+                    self._fill_ghosts(k, time)
+                th = self.tag_thresh[min(k, len(self.tag_thresh) - 1)]
+                tags = torch.empty((lv.ny, lv.nx), dtype=torch.uint8, device="cuda")
+                thr = (C.c_double * 3)(*th)
+                full = lv.patch(lv.U_new)
+                full.ng = 0
+                for d in range(2):
+                    full.bc_lo[d], full.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
+                L.check(lib.hx_tag_level(context(), self.solver, C.byref(full), float(time), thr,
+                                         tags.data_ptr(), _stream()), "hx_tag_level")
+                valid = lv.interior(lv.mask).bool().cpu().numpy()
+                from . import regrid as RG
+                new = RG.new_grids(tags.bool().cpu().numpy(), valid, 2, self.bf, self.mgs,
+                                   self.nbuf, self.eff, self.nest)
+            if not new:
+                del self.levels[k + 1:]
+                break
+            if k + 1 > self.finest:
+                fl = self._new_level(k + 1)
+                self.levels.append(fl)
+                old_mask = None
+                fl.t_new, fl.t_old = time, time - 1.0e200
+                self.dt[k + 1] = self.dt[k] / (2.0 if self.subcycle else 1.0)
+            else:
+                fl = self.levels[k + 1]
+                old_mask = fl.mask.clone()
+            fl.set_boxes(new)
+            if from_scratch and getattr(fl, "from_static", False):
+                continue                        # MakeNewLevelFromScratch: the static layer
+            cp = lv.patch(lv.U_new)
+            cp.ng = 0
+            for d in range(2):
+                cp.bc_lo[d], cp.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
+            for b in fl.boxes:
+                bp = fl.patch(fl.U_new, b)
+                skip = fl.mask_addr(old_mask, b[0], b[1]) if old_mask is not None else None
+                L.check(lib.hx_fill_fine_region(
+                    context(), C.byref(cp), C.byref(cp), 0.0, 2, b[0] // 2, b[1] // 2, C.byref(bp),
+                    1, skip, fl.pitch, fl.addr(fl.U_new, 0, b[0], b[1]),
+                    fl.addr(fl.U_new, 1, b[0], b[1]), fl.addr(fl.U_new, 2, b[0], b[1]), _stream()),
+                    "hx_fill_fine_region")
+
+    # -- one level's pieces ------------------------------------------------
+    def _crse_patch(self, lv: DenseLevel, U: torch.Tensor) -> L.HxPatch:
+        p = lv.patch(U)
+        p.ng = 0
+        for d in range(2):
+            p.bc_lo[d], p.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
+        return p
+
+    def _fill_ghosts(self, lev: int, t: float) -> None:
+        """Ring cells of every box the level does not own, from the coarser
+        level at time t (GetData's time selection: the new state, the old
+        state, or their blend)."""
+        c, f = self.levels[lev - 1], self.levels[lev]
+        if _almost_equal(t, c.t_new):
+            co = cn = c.U_new
+            alpha = 0.0
+        elif _almost_equal(t, c.t_old):
+            co = cn = c.U_old
+            alpha = 0.0
+        else:
+            co, cn = c.U_old, c.U_new
+            alpha = (t - c.t_old) / (c.t_new - c.t_old)
+        po, pn = self._crse_patch(c, co), self._crse_patch(c, cn)
+        lib = L.load()
+        for b in f.boxes:
+            bp = f.patch(f.U_new, b)
+            L.check(lib.hx_fill_fine_region(
+                context(), C.byref(po), C.byref(pn), float(alpha), 2, b[0] // 2, b[1] // 2,
+                C.byref(bp), 0, f.mask_addr(f.mask, b[0], b[1]), f.pitch,
+                f.addr(f.U_new, 0, b[0], b[1]), f.addr(f.U_new, 1, b[0], b[1]),
+                f.addr(f.U_new, 2, b[0], b[1]), _stream()), "hx_fill_fine_region")
+
+    def _flux(self, patch, faces, pair, sx, sy, accumulate, reg) -> None:
+        L.check(L.load().hx_face_fluxes(context(), self.solver, C.byref(patch), faces.shape[0],
+                                        C.c_void_p(faces.data_ptr()), pair, float(sx), float(sy),
+                                        int(accumulate), C.c_void_p(reg.data_ptr()), _stream()),
+                "hx_face_fluxes")
+
+    def _advance_level(self, lev: int, t: float, dt: float) -> None:
+        lv = self.levels[lev]
+        lib = L.load()
+        if lev > 0:
+            self._fill_ghosts(lev, t)
+        if self.do_reflux and lev < self.finest:
+            fl = self.levels[lev + 1]
+            cp = self._crse_patch(lv, lv.U_new)
+            self.regs[lev + 1] = []
+            for b, (cf, _) in zip(fl.boxes, fl.faces):
+                reg = torch.zeros((cf.shape[0], 3), dtype=self.dtype, device="cuda")
+                self._flux(cp, cf, 1, -dt * lv.dy, -dt * lv.dx, 0, reg)
+                self.regs[lev + 1].append(reg)
+        if self.do_reflux and lev > 0:
+            for b, (_, ff), reg in zip(lv.boxes, lv.faces, self.regs[lev]):
+                self._flux(lv.patch(lv.U_new, b), ff, 2, dt * lv.dy, dt * lv.dx, 1, reg)
+        lv.swap()
+        for b in lv.boxes:
+            L.check(lib.hx_advance_level(context(), self.solver, C.byref(lv.patch(lv.U_old, b)),
+                                         float(dt), lv.addr(lv.U_new, 0, b[0], b[1]),
+                                         lv.addr(lv.U_new, 1, b[0], b[1]),
+                                         lv.addr(lv.U_new, 2, b[0], b[1]), lv.pitch, _stream()),
+                    "hx_advance_level")
+
+    def _reflux(self, lev: int) -> None:
+        lv, fl = self.levels[lev], self.levels[lev + 1]
+        cp = self._crse_patch(lv, lv.U_new)
+        lib = L.load()
+        for orient in range(4):            # AMReX Orientation order: x-lo, y-lo, x-hi, y-hi
+            for b, reg in zip(fl.boxes, self.regs[lev + 1]):
+                L.check(lib.hx_reflux_side(context(), C.byref(cp), lv.addr(lv.U_new, 0, 0, 0),
+                                           lv.addr(lv.U_new, 1, 0, 0), lv.addr(lv.U_new, 2, 0, 0),
+                                           b[0] // 2, b[1] // 2, b[2] // 2, b[3] // 2,
+                                           C.c_void_p(reg.data_ptr()), orient, _stream()),
+                        "hx_reflux_side")
+
+    def _average_down(self, lev: int) -> None:
+        lv, fl = self.levels[lev], self.levels[lev + 1]
+        cp = self._crse_patch(lv, lv.U_new)
+        lib = L.load()
+        for b in fl.boxes:
+            L.check(lib.hx_average_down_state(context(), C.byref(fl.patch(fl.U_new, b)), 2,
+                                              b[0] // 2, b[1] // 2, C.byref(cp),
+                                              lv.addr(lv.U_new, 0, 0, 0), lv.addr(lv.U_new, 1, 0, 0),
+                                              lv.addr(lv.U_new, 2, 0, 0), _stream()),
+                    "hx_average_down_state")
+
+    # -- dt and the recursive step -----------------------------------------
+    def level_dt(self, lev: int, cfl: float) -> float:
+        lv = self.levels[lev]
+        arr = (L.HxPatch * len(lv.boxes))(*[lv.patch(lv.U_new, b) for b in lv.boxes])
+        dt, vel = C.c_double(), C.c_double()
+        L.check(L.load().hx_boxes_dt(context(), self.solver, len(lv.boxes), arr, float(cfl),
+                                     C.byref(dt), C.byref(vel), _stream()), "hx_boxes_dt")
+        return dt.value
+
+    def compute_dt(self, cfl: float) -> List[float]:
+        """BaseSolver::compute_dt at ratio 2, then the lockstep rule."""
+        tmp = [self.level_dt(l, cfl) for l in range(self.finest + 1)]
+        dt0, factor = tmp[0], 1.0
+        for l in range(1, self.finest + 1):
+            factor *= 2.0
+            dt0 = min(dt0, tmp[l] * factor)
+        self.dt[0] = dt0
+        for l in range(1, self.finest + 1):
+            self.dt[l] = self.dt[l - 1] / 2.0
+        if not self.subcycle:
+            m = min(self.dt[:self.finest + 1])
+            for l in range(self.finest + 1):
+                self.dt[l] = m
+        return self.dt[:self.finest + 1]
+
+    def _time_step(self, lev: int, time: float, iteration: int) -> None:
+        if (self.regrid_int > 0 and lev < self.max_level and self.istep[lev] > self.last_regrid[lev]
+                and self.istep[lev] % self.regrid_int == 0):
+            old_finest = self.finest
+            self._make_grids(lev, time)
+            for k in range(lev, self.finest + 1):
+                self.last_regrid[k] = self.istep[k]
+            for k in range(old_finest + 1, self.finest + 1):
+                self.dt[k] = self.dt[k - 1] / (2.0 if self.subcycle else 1.0)
+        lv = self.levels[lev]
+        lv.t_old = lv.t_new
+        lv.t_new += self.dt[lev]
+        self._advance_level(lev, lv.t_old, self.dt[lev])
+        self.istep[lev] += 1
+        if lev < self.finest:
+            nsub = 2 if self.subcycle else 1
+            for i in range(1, nsub + 1):
+                self._time_step(lev + 1, time + (i - 1) * self.dt[lev + 1], i)
+            if self.do_reflux:
+                self._reflux(lev)
+            self._average_down(lev)
+
+    def step(self, cfl: Optional[float] = None, dts: Optional[Sequence[float]] = None) -> float:
+        """ComputeDt then one coarse step (HydroEXA::Compute's loop body); dts
+        gives every level's dt instead."""
+        if dts is None:
+            self.compute_dt(cfl)
+        else:
+            for l, d in enumerate(dts):
+                self.dt[l] = float(d)
+        self._time_step(0, self.time, self.iteration)
+        self.time = self.levels[0].t_new
+        self.iteration += 1
+        return self.dt[0]
+
+    def run_until(self, t_end: float, cfl: float, max_steps: int = 10 ** 9) -> int:
+        k = 0
+        while self.time < t_end and k < max_steps:
+            self.step(cfl)
+            k += 1
+        return k
+
+    # -- views -------------------------------------------------------------
+    def state(self, lev: int) -> torch.Tensor:
+        lv = self.levels[lev]
+        return lv.interior(lv.U_new)
+
+    def composite_mass(self) -> float:
+        lv = self.levels[0]
+        return float(lv.interior(lv.U_new)[0].double().sum().item()) * lv.dx * lv.dy
+
+    def centerline(self, row_frac: float = 0.5):
+        """x and h along a row of the finest data (each coarse cell replaced by
+        its finer cells where a finer level holds them)."""
+        xs, hs = [], []
+        for l in range(self.finest + 1):
+            lv = self.levels[l]
+            j = min(int(row_frac * lv.ny), lv.ny - 1)
+            m = lv.interior(lv.mask)[j].bool().cpu().numpy()
+            if l < self.finest:
+                fm = self.levels[l + 1]
+                jf = min(int(row_frac * fm.ny), fm.ny - 1)
+                covered = fm.interior(fm.mask)[jf].bool().cpu().numpy().reshape(-1, 2).any(axis=1)
+                m = m & ~covered
+            h = lv.interior(lv.U_new)[0, j].double().cpu().numpy()
+            x = (np.arange(lv.nx) + 0.5) * lv.dx
+            xs.append(x[m])
+            hs.append(h[m])
+        x, h = np.concatenate(xs), np.concatenate(hs)
+        o = np.argsort(x)
+        return x[o], h[o]

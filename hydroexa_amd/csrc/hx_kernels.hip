@@ -400,7 +400,9 @@ __global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, Patch
                                                           R alpha, int ci0, int cj0, int nxf,
                                                           int nyf, int64_t fpitch, int whole,
                                                           const R* __restrict__ zf, R* hf, R* huf,
-                                                          R* hvf, const int* err)
+                                                          R* hvf, const int* err,
+                                                          const uint8_t* __restrict__ skip,
+                                                          int64_t spitch)
 {
     if (err && *(volatile const int*)err) return;
     const int W = nxf + 4, H = nyf + 4;
@@ -414,6 +416,8 @@ __global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, Patch
     else if (t < 2 * W) { fj = t / W - 2; fi = t % W - 2; }
     else if (t < 4 * W) { const int u = t - 2 * W; fj = nyf + u / W; fi = u % W - 2; }
     else { const int u = t - 4 * W; fj = u / 4; const int k = u % 4; fi = (k < 2) ? k - 2 : nxf + k - 2; }
+    // cells the fine level already holds (another box's valid cells) keep them
+    if (skip && skip[(int64_t)fj * spitch + fi]) return;
     const int pi = ci0 + (fi >= 0 ? fi / 2 : -((1 - fi) / 2));
     const int pj = cj0 + (fj >= 0 ? fj / 2 : -((1 - fj) / 2));
     const R ox = ((fi & 1) == 0) ? R(-0.25) : R(0.25);
@@ -438,22 +442,28 @@ __global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, Patch
 // fine substeps and fine faces of dt_f dA_f F_fine.  The coarse cell outside
 // the box beside the face gains (-reg)/vol on the box's low sides and
 // reg/vol on its high sides, vol = dx dy (AMReX FluxRegister::Reflux).
+// orient (-1: all) restricts to one side, numbered as AMReX's Orientation
+// (0 x-low, 1 y-low, 2 x-high, 3 y-high), so several boxes can be refluxed
+// side by side in AMReX's order; cells outside the level (a box side on the
+// domain boundary) are left alone.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void reflux_kernel(int ci0, int cj0, int ncx, int ncy,
                                                         int64_t pitch, R vol,
                                                         const R* __restrict__ reg, R* h, R* hu,
-                                                        R* hv, const int* err)
+                                                        R* hv, const int* err, int orient,
+                                                        int cnx, int cny)
 {
     if (err && *(volatile const int*)err) return;
     const int q = blockIdx.x * kBlock + threadIdx.x;
     const int nq = 2 * ncy + 2 * ncx;
     if (q >= nq) return;
-    int i, j;
+    int i, j, side;
     R sgn;
-    if (q < ncy) { i = ci0 - 1; j = cj0 + q; sgn = R(-1); }
-    else if (q < 2 * ncy) { i = ci0 + ncx; j = cj0 + q - ncy; sgn = R(1); }
-    else if (q < 2 * ncy + ncx) { i = ci0 + q - 2 * ncy; j = cj0 - 1; sgn = R(-1); }
-    else { i = ci0 + q - 2 * ncy - ncx; j = cj0 + ncy; sgn = R(1); }
+    if (q < ncy) { i = ci0 - 1; j = cj0 + q; sgn = R(-1); side = 0; }
+    else if (q < 2 * ncy) { i = ci0 + ncx; j = cj0 + q - ncy; sgn = R(1); side = 2; }
+    else if (q < 2 * ncy + ncx) { i = ci0 + q - 2 * ncy; j = cj0 - 1; sgn = R(-1); side = 1; }
+    else { i = ci0 + q - 2 * ncy - ncx; j = cj0 + ncy; sgn = R(1); side = 3; }
+    if ((orient >= 0 && side != orient) || i < 0 || j < 0 || i >= cnx || j >= cny) return;
     const int64_t o = (int64_t)j * pitch + i;
     const R* r = reg + 3 * q;
     h[o] += (sgn * r[0]) / vol;
@@ -1227,13 +1237,14 @@ extern "C" int hx_face_fluxes(hx_ctx* c, int solver, const hx_patch* in, int nfa
 template <typename R>
 static int fillfine_t(const hx_patch* co, const hx_patch* cn, double alpha, int ci0, int cj0,
                       const hx_patch* f, int whole, void* h, void* hu, void* hv, hipStream_t st,
-                      const int* err = nullptr)
+                      const int* err = nullptr, const uint8_t* skip = nullptr,
+                      int64_t spitch = 0)
 {
     const int W = f->nx + 4, H = f->ny + 4;
     const int n = whole ? W * H : W * H - f->nx * f->ny;
     hipLaunchKernelGGL(fillfine_kernel<R>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
                        view_of<R>(co), view_of<R>(cn), (R)alpha, ci0, cj0, f->nx, f->ny, f->pitch,
-                       whole, (const R*)f->z, (R*)h, (R*)hu, (R*)hv, err);
+                       whole, (const R*)f->z, (R*)h, (R*)hu, (R*)hv, err, skip, spitch);
     return hip_status(hipGetLastError());
 }
 
@@ -1265,12 +1276,13 @@ extern "C" int hx_fill_fine_ghosts(hx_ctx* c, const hx_patch* co, const hx_patch
 
 template <typename R>
 static int reflux_t(const hx_patch* crse, void* h, void* hu, void* hv, int ci0, int cj0, int ncx,
-                    int ncy, const void* reg, hipStream_t st, const int* err = nullptr)
+                    int ncy, const void* reg, hipStream_t st, const int* err = nullptr,
+                    int orient = -1)
 {
     const int nq = 2 * (ncx + ncy);
     hipLaunchKernelGGL(reflux_kernel<R>, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ci0,
                        cj0, ncx, ncy, crse->pitch, (R)(crse->dx * crse->dy), (const R*)reg, (R*)h,
-                       (R*)hu, (R*)hv, err);
+                       (R*)hu, (R*)hv, err, orient, crse->nx, crse->ny);
     return hip_status(hipGetLastError());
 }
 
@@ -1523,4 +1535,88 @@ extern "C" int hx_two_level_run(hx_ctx* c, int solver, const hx_patch* crse, voi
     }
     if (steps_done) *steps_done = nsteps;
     return HX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Entry points of the multi-box hierarchy (dense levels, box lists)
+// ---------------------------------------------------------------------------
+extern "C" int hx_fill_fine_region(hx_ctx* c, const hx_patch* co, const hx_patch* cn, double alpha,
+                                   int ratio, int ci0, int cj0, const hx_patch* fine, int whole,
+                                   const uint8_t* skip, int64_t skip_pitch, void* h, void* hu,
+                                   void* hv, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    int rc = check_patch(co, 0);
+    if (!rc) rc = check_patch(cn, 0);
+    if (rc) return rc;
+    if (!fine || ratio != 2 || fine->ng < 2 || fine->nx % 2 || fine->ny % 2 || !h || !hu || !hv ||
+        !fine->z || fine->dtype != co->dtype || cn->dtype != co->dtype ||
+        fine->pitch < (int64_t)fine->nx + 4 || !(alpha >= 0.0 && alpha <= 1.0) ||
+        co->nx != cn->nx || co->ny != cn->ny || (skip && skip_pitch < (int64_t)fine->nx + 4))
+        return HX_E_BADARG;
+    const int ncx = fine->nx / 2, ncy = fine->ny / 2;
+    if (ci0 < 0 || cj0 < 0 || ci0 + ncx > co->nx || cj0 + ncy > co->ny) return HX_E_BADARG;
+    for (int d = 0; d < 2; ++d)
+        if (co->bc_lo[d] == HX_BC_GHOST || co->bc_hi[d] == HX_BC_GHOST) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    return (co->dtype == HX_F64)
+               ? fillfine_t<double>(co, cn, alpha, ci0, cj0, fine, whole != 0, h, hu, hv, st,
+                                    nullptr, skip, skip_pitch)
+               : fillfine_t<float>(co, cn, alpha, ci0, cj0, fine, whole != 0, h, hu, hv, st,
+                                   nullptr, skip, skip_pitch);
+}
+
+extern "C" int hx_reflux_side(hx_ctx* c, const hx_patch* crse, void* h, void* hu, void* hv,
+                              int ci0, int cj0, int ncx, int ncy, const void* reg, int orient,
+                              void* stream)
+{
+    if (!c) return HX_E_STATE;
+    int rc = check_patch(crse, 0);
+    if (rc) return rc;
+    if (!h || !hu || !hv || !reg || ncx < 1 || ncy < 1 || ci0 < 0 || cj0 < 0 ||
+        ci0 + ncx > crse->nx || cj0 + ncy > crse->ny || orient < -1 || orient > 3)
+        return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    return (crse->dtype == HX_F64)
+               ? reflux_t<double>(crse, h, hu, hv, ci0, cj0, ncx, ncy, reg, st, nullptr, orient)
+               : reflux_t<float>(crse, h, hu, hv, ci0, cj0, ncx, ncy, reg, st, nullptr, orient);
+}
+
+extern "C" int hx_boxes_dt(hx_ctx* c, int solver, int nbox, const hx_patch* boxes, double cfl,
+                           double* dt_out, double* max_vel_out, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    if (nbox < 1 || !boxes || !dt_out || !(cfl > 0.0)) return HX_E_BADARG;
+    for (int b = 0; b < nbox; ++b) {
+        const int rc = check_patch(&boxes[b], 0);
+        if (rc) return rc;
+        if (boxes[b].dtype != boxes[0].dtype) return HX_E_BADARG;
+    }
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots + 3, 1, (int*)nullptr);
+    for (int b = 0; b < nbox; ++b) {
+        const int rc = (boxes[b].dtype == HX_F64) ? launch_cfl<double>(&boxes[b], c->slots + 3, st)
+                                                  : launch_cfl<float>(&boxes[b], c->slots + 3, st);
+        if (rc) return rc;
+    }
+    CflSlot sl;
+    HX_TRY(hipMemcpyAsync(&sl, c->slots + 3, sizeof(sl), hipMemcpyDeviceToHost, st));
+    HX_TRY(hipStreamSynchronize(st));
+    double vmax;
+    std::memcpy(&vmax, &sl.vmax, sizeof(vmax));
+    double est;
+    if (boxes[0].dtype == HX_F64) {
+        est = slot_dt_est<double>(sl.amax, sl.flags);
+        *dt_out = est * cfl;
+    } else {
+        const float e = slot_dt_est<float>(sl.amax, sl.flags);
+        est = e;
+        *dt_out = (double)(e * (float)cfl);
+    }
+    if (max_vel_out) *max_vel_out = (sl.flags ? vmax : -1.7976931348623157e308);
+    return (est < 1.e-12) ? HX_E_DT_COLLAPSE : HX_OK;
 }

@@ -286,6 +286,41 @@ int hx_average_down_state(hx_ctx *ctx, const hx_patch *fine, int ratio, int ci0,
                           const hx_patch *crse, void *h, void *hu, void *hv, void *stream);
 
 /*
+ * Multi-box hierarchy pieces (levels held as full-domain arrays with two
+ * cells of padding, each level a list of boxes; hydroexa_amd/amr.py's
+ * AmrHierarchy drives them).
+ *
+ * hx_fill_fine_region: hx_fill_fine_ghosts for a box of a level that other
+ * boxes share: cells whose skip byte is non-zero (skip: the fine level's
+ * valid-cell mask at the box's (0,0), skip_pitch bytes between rows, its
+ * ghost ring addressable) are left as they are.  whole = 1 fills the box and
+ * its ring (a level remade after regridding: skip = the cells valid before).
+ * The box may touch the domain boundary; ring cells outside the domain are
+ * written into the padding and never read.
+ */
+int hx_fill_fine_region(hx_ctx *ctx, const hx_patch *crse_old, const hx_patch *crse_new,
+                        double alpha, int ratio, int ci0, int cj0, const hx_patch *fine, int whole,
+                        const uint8_t *skip, int64_t skip_pitch, void *h, void *hu, void *hv,
+                        void *stream);
+
+/*
+ * hx_reflux for one side of a box (orient 0 x-low, 1 y-low, 2 x-high,
+ * 3 y-high, AMReX's Orientation order; -1 all sides).  Refluxing every box
+ * side by side in that order reproduces FluxRegister::Reflux's order of
+ * additions at a coarse cell two boxes border.  The box may touch the
+ * domain boundary: faces there have no coarse cell outside and are skipped.
+ */
+int hx_reflux_side(hx_ctx *ctx, const hx_patch *crse, void *h, void *hu, void *hv, int ci0,
+                   int cj0, int ncx, int ncy, const void *reg, int orient, void *stream);
+
+/*
+ * hx_level_dt over the valid cells of nbox boxes of one level (host array
+ * of patches), one reduction and one synchronisation.
+ */
+int hx_boxes_dt(hx_ctx *ctx, int solver, int nbox, const hx_patch *boxes, double cfl,
+                double *dt_out, double *max_vel_out, void *stream);
+
+/*
  * nsteps coupled steps of one coarse level and one fine box at ratio 2,
  * without host synchronisation: the reference's recursive subcycled step
  * (AmrMeshState::TimeStepWithSubcycling, src/state/AmrMeshState_Physics.cpp:50-147)

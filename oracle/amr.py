@@ -314,3 +314,247 @@ class TwoLevelOracle:
         Uc = [C["h"], C["hu"], C["hv"]]
         reflux(Uc, reg, box, dx, dy)
         average_down([F["U"][c][G:-G, G:-G] for c in range(3)], Uc, box)
+
+
+# ---------------------------------------------------------------------------
+# multi-box hierarchy (dense levels with 2 cells of padding, fixed boxes)
+# ---------------------------------------------------------------------------
+def _axis_map(idx: np.ndarray, n: int, phys_lo: bool, phys_hi: bool, blo: int, bhi: int):
+    """Source cell, edge cell and normal flip of level indices idx along one
+    axis: a side on the domain boundary maps with the physical rule, any
+    other index is read as stored."""
+    src, edge, flip = idx.copy(), idx.copy(), np.zeros(idx.shape, bool)
+    if phys_lo:
+        low = idx < 0
+        edge[low] = 0
+        if blo == 0:
+            src[low] = -1 - idx[low]
+            flip[low] = True
+        else:
+            src[low] = 0
+    if phys_hi:
+        high = idx >= n
+        edge[high] = n - 1
+        if bhi == 0:
+            src[high] = 2 * n - 1 - idx[high]
+            flip[high] = True
+        else:
+            src[high] = n - 1
+    return src, edge, flip
+
+
+class DenseLevelOracle:
+    def __init__(self, nx, ny, dx, dy, bc_lo, bc_hi):
+        self.nx, self.ny, self.dx, self.dy = nx, ny, dx, dy
+        self.bc_lo, self.bc_hi = tuple(bc_lo), tuple(bc_hi)
+        shp = (ny + 2 * G, nx + 2 * G)
+        self.U = np.zeros((3,) + shp)
+        self.Uo = np.zeros((3,) + shp)
+        self.T = np.zeros((2,) + shp)
+        self.mask = np.zeros(shp, bool)
+        self.boxes = []
+        self.t_new, self.t_old = 0.0, -1.0e200
+
+    def set_boxes(self, boxes):
+        self.boxes = [tuple(b) for b in boxes]
+        self.mask[:] = False
+        for (i0, j0, w, h) in self.boxes:
+            self.mask[G + j0:G + j0 + h, G + i0:G + i0 + w] = True
+
+    def interior(self, a):
+        return a[..., G:G + self.ny, G:G + self.nx]
+
+    def full_working(self, U):
+        """Working state of the whole level with the physical halo."""
+        I = lambda a: self.interior(a)
+        return working_state(I(U[0]), I(U[1]), I(U[2]), I(self.T[0]), I(self.T[1]),
+                             self.bc_lo, self.bc_hi)
+
+    def gather_box(self, U, box):
+        """Padded (h, hu, hv, z, n, eta) of a box with its 2-cell ring, as the
+        advance's loader sees them."""
+        i0, j0, w, h = box
+        ii = np.arange(i0 - G, i0 + w + G)
+        jj = np.arange(j0 - G, j0 + h + G)
+        si, ei, fi = _axis_map(ii, self.nx, i0 == 0, i0 + w == self.nx, self.bc_lo[0], self.bc_hi[0])
+        sj, ej, fj = _axis_map(jj, self.ny, j0 == 0, j0 + h == self.ny, self.bc_lo[1], self.bc_hi[1])
+        S = np.ix_(sj + G, si + G)
+        E = np.ix_(ej + G, ei + G)
+        hs = U[0][S]
+        eta = hs + self.T[0][S]
+        hu = np.where(fi[None, :], -U[1][S], U[1][S])
+        hv = np.where(fj[:, None], -U[2][S], U[2][S])
+        return hs, hu, hv, self.T[0][E], self.T[1][E], eta
+
+
+class AmrOracle:
+    """Restatement of hydroexa_amd.amr.AmrHierarchy on fixed boxes: the same
+    recursion, ghost fills (mask-skipped), registers, reflux order and
+    average_down, with the C oracle's ghost-fed step per box."""
+
+    def __init__(self, solver, h, hu, hv, z, n, dx, dy, bc_lo, bc_hi, boxes, terrain=None,
+                 subcycle=True, do_reflux=True):
+        self.solver, self.subcycle, self.do_reflux = solver, subcycle, do_reflux
+        self.bc_lo, self.bc_hi = tuple(bc_lo), tuple(bc_hi)
+        ny, nx = np.asarray(h).shape
+        lv = DenseLevelOracle(nx, ny, dx, dy, bc_lo, bc_hi)
+        pad = lambda a: np.pad(np.asarray(a, np.float64), G, mode="edge")
+        lv.U[:] = [pad(h), pad(hu), pad(hv)]
+        lv.T[:] = [pad(z), pad(n)]
+        lv.set_boxes([(0, 0, nx, ny)])
+        self.levels = [lv]
+        for lev in sorted(boxes):
+            c = self.levels[-1]
+            f = DenseLevelOracle(2 * c.nx, 2 * c.ny, c.dx / 2, c.dy / 2, bc_lo, bc_hi)
+            if terrain is not None and lev in terrain:
+                zf, nf = terrain[lev]
+                f.T[:] = [pad(zf), pad(nf)]
+            else:
+                ref = lambda a: np.repeat(np.repeat(a, 2, 0), 2, 1)
+                f.T[:] = [pad(ref(c.interior(c.T[0]))), pad(ref(c.interior(c.T[1])))]
+            f.set_boxes(boxes[lev])
+            self.levels.append(f)
+            self._fill(lev, 0.0, whole=True, skip=None, crse=(c.U, c.U, 0.0))
+        self.dt = [0.0] * len(self.levels)
+        self.time = 0.0
+        self.regs = [[] for _ in self.levels]
+
+    @property
+    def finest(self):
+        return len(self.levels) - 1
+
+    def _crse_dict(self, lv, U):
+        I = lambda a: lv.interior(a)
+        return {"h": I(U[0]), "hu": I(U[1]), "hv": I(U[2]), "z": I(lv.T[0]), "n": I(lv.T[1])}
+
+    def _fill(self, lev, t, whole, skip, crse=None):
+        c, f = self.levels[lev - 1], self.levels[lev]
+        if crse is None:
+            if abs(t - c.t_new) <= 1e-12 * max(abs(t), abs(c.t_new), 1e-300):
+                co, cn, alpha = c.U, c.U, 0.0
+            elif abs(t - c.t_old) <= 1e-12 * max(abs(t), abs(c.t_old), 1e-300):
+                co, cn, alpha = c.Uo, c.Uo, 0.0
+            else:
+                co, cn, alpha = c.Uo, c.U, (t - c.t_old) / (c.t_new - c.t_old)
+        else:
+            co, cn, alpha = crse
+        Co, Cn = self._crse_dict(c, co), self._crse_dict(c, cn)
+        for (i0, j0, w, h) in f.boxes:
+            sub = f.U[:, j0:j0 + h + 2 * G, i0:i0 + w + 2 * G].copy()
+            zf = f.T[0][j0:j0 + h + 2 * G, i0:i0 + w + 2 * G]
+            before = sub.copy()
+            fill_fine(Co, Cn, alpha, (i0 // 2, j0 // 2, w // 2, h // 2), self.bc_lo, self.bc_hi,
+                      zf, sub, whole=whole)
+            keep = (f.mask if skip is None else skip)[j0:j0 + h + 2 * G, i0:i0 + w + 2 * G]
+            if skip is None and whole:
+                keep = np.zeros_like(keep)
+            sub[:, keep] = before[:, keep]
+            f.U[:, j0:j0 + h + 2 * G, i0:i0 + w + 2 * G] = sub
+
+    def _face_list(self, box):
+        i0, j0, w, h = box
+        cx, cy, fx, fy = box_faces((i0 // 2, j0 // 2, w // 2, h // 2))
+        return cx + cy, fx + fy
+
+    def _advance(self, lev, t, dt):
+        lv = self.levels[lev]
+        if lev > 0:
+            self._fill(lev, t, whole=False, skip=None)
+        if self.do_reflux and lev < self.finest:
+            fl = self.levels[lev + 1]
+            W = lv.full_working(lv.U)
+            self.regs[lev + 1] = []
+            for b in fl.boxes:
+                cf, _ = self._face_list(b)
+                reg = np.asarray([(-dt * (lv.dy if f[0] == 0 else lv.dx)) *
+                                  face_flux(self.solver, W, f, lv.dx, lv.dy) for f in cf])
+                self.regs[lev + 1].append(reg)
+        if self.do_reflux and lev > 0:
+            for b, reg in zip(lv.boxes, self.regs[lev]):
+                hs, hu, hv, z, n, eta = lv.gather_box(lv.U, b)
+                he = np.where(eta - z > 0.0, eta - z, 0.0)
+                Wb = {"he": he, "hu": hu, "hv": hv, "z": z, "n": n}
+                _, ff = self._face_list(b)
+                for q in range(reg.shape[0]):
+                    for f in ff[2 * q:2 * q + 2]:
+                        s = dt * (lv.dy if f[0] == 0 else lv.dx)
+                        reg[q] = reg[q] + s * face_flux(self.solver, Wb, f, lv.dx, lv.dy)
+        lv.U, lv.Uo = lv.Uo, lv.U
+        lv.U[:] = lv.Uo      # the invalid cells keep the last values, as on the device
+        for b in lv.boxes:
+            i0, j0, w, h = b
+            hs, hu, hv, z, n, eta = lv.gather_box(lv.Uo, b)
+            out = O.step_padded(self.solver, hs, hu, hv, z, n, dt, lv.dx, lv.dy, eta=eta)
+            for k in range(3):
+                lv.U[k, G + j0:G + j0 + h, G + i0:G + i0 + w] = out[k]
+
+    def _reflux(self, lev):
+        lv, fl = self.levels[lev], self.levels[lev + 1]
+        I = [lv.interior(lv.U[k]) for k in range(3)]
+        for orient in range(4):
+            for b, reg in zip(fl.boxes, self.regs[lev + 1]):
+                ci0, cj0, ncx, ncy = b[0] // 2, b[1] // 2, b[2] // 2, b[3] // 2
+                for q in range(reg.shape[0]):
+                    if q < ncy:
+                        i, j, sg, side = ci0 - 1, cj0 + q, -1.0, 0
+                    elif q < 2 * ncy:
+                        i, j, sg, side = ci0 + ncx, cj0 + q - ncy, 1.0, 2
+                    elif q < 2 * ncy + ncx:
+                        i, j, sg, side = ci0 + q - 2 * ncy, cj0 - 1, -1.0, 1
+                    else:
+                        i, j, sg, side = ci0 + q - 2 * ncy - ncx, cj0 + ncy, 1.0, 3
+                    if side != orient or i < 0 or j < 0 or i >= lv.nx or j >= lv.ny:
+                        continue
+                    for k in range(3):
+                        I[k][j, i] = I[k][j, i] + (sg * reg[q, k]) / (lv.dx * lv.dy)
+
+    def _average_down(self, lev):
+        lv, fl = self.levels[lev], self.levels[lev + 1]
+        for (i0, j0, w, h) in fl.boxes:
+            src = [fl.U[k, G + j0:G + j0 + h, G + i0:G + i0 + w] for k in range(3)]
+            dst = [lv.interior(lv.U[k]) for k in range(3)]
+            average_down(src, dst, (i0 // 2, j0 // 2, w // 2, h // 2))
+
+    def level_dt(self, lev):
+        lv = self.levels[lev]
+        est = np.inf
+        for (i0, j0, w, h) in lv.boxes:
+            S = (slice(G + j0, G + j0 + h), slice(G + i0, G + i0 + w))
+            d, _ = O.level_dt(lv.U[0][S], lv.U[1][S], lv.U[2][S], lv.T[0][S], lv.dx, lv.dy)
+            est = min(est, d)
+        return est
+
+    def compute_dt(self, cfl):
+        tmp = [cfl * self.level_dt(l) for l in range(self.finest + 1)]
+        dt0, factor = tmp[0], 1.0
+        for l in range(1, self.finest + 1):
+            factor *= 2.0
+            dt0 = min(dt0, tmp[l] * factor)
+        self.dt[0] = dt0
+        for l in range(1, self.finest + 1):
+            self.dt[l] = self.dt[l - 1] / 2.0
+        if not self.subcycle:
+            m = min(self.dt)
+            self.dt = [m] * len(self.dt)
+        return list(self.dt)
+
+    def _time_step(self, lev, time):
+        lv = self.levels[lev]
+        lv.t_old = lv.t_new
+        lv.t_new += self.dt[lev]
+        self._advance(lev, lv.t_old, self.dt[lev])
+        if lev < self.finest:
+            nsub = 2 if self.subcycle else 1
+            for i in range(1, nsub + 1):
+                self._time_step(lev + 1, time + (i - 1) * self.dt[lev + 1])
+            if self.do_reflux:
+                self._reflux(lev)
+            self._average_down(lev)
+
+    def step(self, dts=None, cfl=None):
+        if dts is None:
+            self.compute_dt(cfl)
+        else:
+            self.dt = list(dts)
+        self._time_step(0, self.time)
+        self.time = self.levels[0].t_new

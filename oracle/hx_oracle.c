@@ -584,11 +584,14 @@ int hxo_hydro_step(int nx, int ny, const double *h, const double *hu, const doub
 /* coarse one: FillPatchTwoLevels wrote them).  Inputs are padded           */
 /* (ny+4) x (nx+4), pitch nx+4, valid cell (0,0) at 2*(nx+4)+2; eta = h + z */
 /* and h = max(0, eta - z) over the whole padded region, then the same      */
-/* faces and update as above.  Outputs are the nx x ny valid cells.         */
+/* faces and update as above.  Outputs are the nx x ny valid cells.  eta   */
+/* (optional, padded) replaces h + z as the free surface of each cell: a    */
+/* ghost across a physical side carries its source cell's h + z on the     */
+/* edge cell's bed.                                                         */
 /* ------------------------------------------------------------------------ */
 int hxo_step_padded(int solver, int nx, int ny, const double *h, const double *hu,
-                    const double *hv, const double *z, const double *n, double dt, double dx,
-                    double dy, double *ho, double *huo, double *hvo)
+                    const double *hv, const double *z, const double *n, const double *eta,
+                    double dt, double dx, double dy, double *ho, double *huo, double *hvo)
 {
     if (nx < 2 || ny < 2 || (solver != 0 && solver != 1)) return HXO_E_BADARG;
     pad_t p = {nx, ny, nx + 2 * G};
@@ -596,7 +599,7 @@ int hxo_step_padded(int solver, int nx, int ny, const double *h, const double *h
     if (work_alloc(&p, &w)) return HXO_E_NOMEM;
     const size_t np = (size_t)p.P * (ny + 2 * G);
     for (size_t k = 0; k < np; ++k) {
-        w.eta[k] = h[k] + z[k];
+        w.eta[k] = eta ? eta[k] : h[k] + z[k];
         w.hu[k] = hu[k]; w.hv[k] = hv[k]; w.z[k] = z[k]; w.n[k] = n[k];
         w.he[k] = smax(0.0, w.eta[k] - w.z[k]);
     }

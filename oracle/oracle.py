@@ -46,7 +46,7 @@ def lib():
                      C.c_double, _ip, _ip, _dp, _dp, _dp]
         L.hxo_roe_step.argtypes = step_args
         L.hxo_hydro_step.argtypes = step_args
-        L.hxo_step_padded.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
+        L.hxo_step_padded.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
                                       C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         L.hxo_level_dt.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
                                    C.c_double, _dp, _dp]
@@ -120,13 +120,16 @@ def step(solver, h, hu, hv, z, n, dt, dx, dy, bc_lo=(1, 1), bc_hi=(1, 1)):
     return ho, huo, hvo
 
 
-def step_padded(solver, h, hu, hv, z, n, dt, dx, dy):
+def step_padded(solver, h, hu, hv, z, n, dt, dx, dy, eta=None):
     """One step of a box whose 2 ghost layers are given: inputs (ny+4, nx+4),
-    outputs (ny, nx)."""
+    outputs (ny, nx).  eta (optional) is each cell's free surface in place of
+    h + z."""
     a = [_f64(x) for x in (h, hu, hv, z, n)]
+    e = _f64(eta) if eta is not None else None
     ny, nx = a[0].shape[0] - 4, a[0].shape[1] - 4
     out = [np.empty((ny, nx)) for _ in range(3)]
-    rc = lib().hxo_step_padded(solver, nx, ny, *[_p(x) for x in a], dt, dx, dy,
+    rc = lib().hxo_step_padded(solver, nx, ny, *[_p(x) for x in a],
+                               _p(e) if e is not None else None, dt, dx, dy,
                                *[_p(o) for o in out])
     if rc != 0:
         raise ValueError(f"hxo_step_padded failed ({rc})")

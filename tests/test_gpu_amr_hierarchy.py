@@ -1,0 +1,173 @@
+"""GPU tests of the multi-level, multi-box hierarchy (hydroexa_amd.amr.AmrHierarchy).
+
+* Parity against the oracle's restatement (oracle/amr.py AmrOracle) on fixed
+  boxes -- boxes touching the domain boundary, adjacent boxes, three levels,
+  subcycled and lockstep: FP64 within 1e-12 relative L-infinity per level.
+* Composite mass conserved to round-off in a closed box while the grids are
+  regridded from gradient tags every two steps.
+* The reference's AMR dam break (tests/dambreak: 10 x 1.25 domain, 512 x 64
+  base, one level at ratio 2, blocking factor 32, Roe, CFL 0.5, tags at
+  |grad| 0.0075): the L1 depth error against the SWASHES solution at t = 15,
+  next to the reference's published values.
+"""
+import numpy as np
+import pytest
+import torch
+from scipy.optimize import brentq
+
+from oracle import amr as A
+from oracle import oracle as O
+from hydroexa_amd import set_arith
+from hydroexa_amd.amr import AmrHierarchy
+
+pytestmark = pytest.mark.gpu
+
+MODELS = {"Roe": O.ROE, "HydrostaticRoe": O.HYDROSTATIC_ROE}
+G = A.G
+
+
+def rel_linf(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def field(ny, nx, seed=1, flat_ring=True):
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    rng = np.random.default_rng(seed)
+    z = 0.05 * np.sin(0.5 * ii) * np.cos(0.4 * jj)
+    if flat_ring:
+        z[:2, :] = z[-2:, :] = 0.0
+        z[:, :2] = z[:, -2:] = 0.0
+    h = 1.0 + 0.3 * np.exp(-((ii - 0.4 * nx) ** 2 + (jj - 0.5 * ny) ** 2) / (0.05 * nx * ny)) - z
+    hu = 0.1 * h * rng.standard_normal((ny, nx))
+    hv = 0.1 * h * rng.standard_normal((ny, nx))
+    n = rng.uniform(0.0, 0.03, (ny, nx))
+    return h, hu, hv, z, n
+
+
+CASES = {
+    # model, base (ny, nx), bc_lo, bc_hi, fixed boxes per level
+    "roe_edges": ("Roe", (24, 40), (1, 0), (0, 1),
+                  {1: [(0, 8, 16, 24), (16, 8, 24, 16), (48, 0, 32, 16), (56, 32, 24, 16)]}),
+    "hydro_3lev": ("HydrostaticRoe", (20, 24), (0, 0), (0, 0),
+                   {1: [(6, 6, 24, 20), (30, 6, 10, 16)], 2: [(18, 18, 32, 18), (50, 18, 16, 12)]}),
+    "roe_3lev": ("Roe", (20, 24), (1, 1), (1, 1),
+                 {1: [(8, 4, 32, 24)], 2: [(20, 14, 40, 28)]}),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("subcycle", [True, False])
+def test_hierarchy_parity_fixed_boxes(gpu, case, subcycle):
+    model, (ny, nx), lo, hi, boxes = CASES[case]
+    st = field(ny, nx, seed=7)
+    H = AmrHierarchy(model, *st, 1.0, 1.0, lo, hi, max_level=max(boxes), boxes=boxes,
+                     subcycle=subcycle)
+    R = A.AmrOracle(MODELS[model], *st, 1.0, 1.0, lo, hi, boxes, subcycle=subcycle)
+    for it in range(4):
+        dts = H.compute_dt(0.45)
+        want = R.compute_dt(0.45)
+        assert all(abs(a - b) <= 1e-12 * b for a, b in zip(dts, want)), (dts, want)
+        print(f"hierarchy {case} subcycle {subcycle} step {it} dts {dts}", flush=True)
+        H.step(dts=dts)
+        R.step(dts=dts)
+        torch.cuda.synchronize()
+        for l, (lv, ro) in enumerate(zip(H.levels, R.levels)):
+            m = ro.interior(ro.mask)
+            got = lv.interior(lv.U_new).double().cpu().numpy()[:, m]
+            ref = ro.interior(ro.U)[:, m]
+            for k in range(3):
+                err = rel_linf(got[k], ref[k])
+                assert err <= 1e-12, (case, it, l, k, err)
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_hierarchy_regrid_conserves_mass(gpu, model):
+    ny, nx = 64, 64
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    z = np.zeros((ny, nx))
+    h = 1.0 + 0.5 * (np.hypot(ii - 24.0, jj - 30.0) < 8)
+    zero = np.zeros((ny, nx))
+    H = AmrHierarchy(model, h, zero, zero, z, zero, 1.0, 1.0, (0, 0), (0, 0), max_level=2,
+                     regrid_int=2, blocking_factor=8, max_grid_size=32,
+                     tag_thresh=[(0.02, 0.0, 0.0), (0.03, 0.0, 0.0)])
+    for l in range(H.finest - 1, -1, -1):
+        H._average_down(l)
+    m0 = H.composite_mass()
+    seen = set()
+    for _ in range(16):
+        H.step(0.45)
+        seen.add(tuple(tuple(lv.boxes) for lv in H.levels[1:]))
+    m1 = H.composite_mass()
+    print(f"{model}: {len(seen)} distinct grid sets, mass drift {abs(m1 - m0) / m0:.2e}", flush=True)
+    assert H.finest == 2
+    assert len(seen) > 1
+    assert abs(m1 - m0) / m0 < 1e-13
+
+
+GRAV = 9.81
+
+
+def swashes(t, x, hl, hr, x0):
+    """SWASHES dam break on a wet bed (tests/dambreak/analytical.py:25-65)."""
+    cl, cr = np.sqrt(GRAV * hl), np.sqrt(GRAV * hr)
+    f = lambda cm: -8 * cr ** 2 * cm ** 2 * (cl - cm) ** 2 + (cm ** 2 - cr ** 2) ** 2 * (cm ** 2 + cr ** 2)
+    cm = brentq(f, cr * (1 + 1e-9), cl)
+    xa, xb = x0 - t * cl, x0 + t * (2 * cl - 3 * cm)
+    xc = x0 + t * (2 * cm ** 2 * (cl - cm)) / (cm ** 2 - cr ** 2)
+    return np.where(x <= xa, hl, np.where(x <= xb, (4 / (9 * GRAV)) * (cl - (x - x0) / (2 * t)) ** 2,
+                                          np.where(x <= xc, cm ** 2 / GRAV, hr)))
+
+
+def l1_weighted(x, e):
+    """tests/dambreak/analyze.py:160-195: trapezoidal cell widths."""
+    w = np.zeros_like(x)
+    w[1:-1] = 0.5 * (x[2:] - x[:-2])
+    w[0], w[-1] = 0.5 * (x[1] - x[0]), 0.5 * (x[-1] - x[-2])
+    return float(np.sum(w * np.abs(e)) / w.sum())
+
+
+def dambreak_run(max_level, regrid_int, t_end=15.0, nx=512, ny=64):
+    L, hl, hr, x0 = 10.0, 0.005, 0.001, 5.0
+    dx = L / nx
+
+    def state(nxl, nyl):
+        x = (np.arange(nxl) + 0.5) * (L / nxl)
+        h = np.where(x < x0, hl, hr)[None, :] * np.ones((nyl, 1))
+        z = np.zeros_like(h)
+        return h, z.copy(), z.copy(), z.copy(), z.copy()
+
+    H = AmrHierarchy("Roe", *state(nx, ny), dx, dx, (1, 0), (1, 0), max_level=max_level,
+                     regrid_int=regrid_int, blocking_factor=32, max_grid_size=128,
+                     n_error_buf=1, tag_thresh=[(0.0075, 0.0075, 0.0075)],
+                     static=lambda lev: state(nx << lev, ny << lev))
+    H.run_until(t_end, 0.5)
+    x, h = H.centerline(0.5)
+    return H, x, h, l1_weighted(x, h - swashes(H.time, x, hl, hr, x0))
+
+
+def test_reference_amr_dambreak_accuracy(gpu):
+    """The reference publishes L1 (t = 15) = 1.85e-5 on 512 x 64, 1.07e-5 on
+    1024 x 128, 1.30e-5 with one AMR level regridded every 50 steps and
+    1.06e-5 every 5 steps (tests/dambreak/README.md:272-284, 304-315).  Every
+    run must land on the published value to its printed precision (the AMR
+    runs use this library's Berger-Rigoutsos grids, not AMReX's, and still
+    do: measured 1.851e-5, 1.068e-5, 1.295e-5, 1.058e-5), and meet the
+    reference's acceptance rules (tests/dambreak/README.md, metrics section):
+    better than the unrefined run, and with regrid 5 within 10 % of the
+    uniform fine run."""
+    set_arith("exact")
+    _, _, _, base = dambreak_run(0, 0)
+    _, _, _, fine = dambreak_run(0, 0, nx=1024, ny=128)
+    H50, *_, amr50 = dambreak_run(1, 50)
+    H5, *_, amr5 = dambreak_run(1, 5)
+    print(f"L1 t=15: base {base:.3e} fine {fine:.3e} amr(regrid 50) {amr50:.3e} "
+          f"amr(regrid 5) {amr5:.3e}; level-1 boxes {H5.levels[1].boxes if H5.finest else []}",
+          flush=True)
+    assert float(f"{base:.2e}") == pytest.approx(1.85e-5, abs=0.011e-5)
+    assert float(f"{fine:.2e}") == pytest.approx(1.07e-5, abs=0.011e-5)
+    assert float(f"{amr50:.2e}") == pytest.approx(1.30e-5, abs=0.011e-5)
+    assert float(f"{amr5:.2e}") == pytest.approx(1.06e-5, abs=0.011e-5)
+    assert amr50 < base and amr5 < base
+    assert amr5 <= 1.1 * fine
+    assert amr5 < amr50

@@ -233,3 +233,89 @@ def test_lake_at_rest_hydrostatic():
     Uf = T.fine_valid()
     np.testing.assert_allclose(Uf[0] + T.F["z"][G:-G, G:-G], eta, rtol=0, atol=1e-13)
     np.testing.assert_allclose(Uf[1:], 0.0, rtol=0, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# the multi-box hierarchy restatement
+# ---------------------------------------------------------------------------
+def _field(ny, nx, seed=1, flat_ring=True):
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    rng = np.random.default_rng(seed)
+    z = 0.05 * np.sin(0.5 * ii) * np.cos(0.4 * jj)
+    if flat_ring:
+        z[:2, :] = z[-2:, :] = 0.0
+        z[:, :2] = z[:, -2:] = 0.0
+    h = 1.0 + 0.3 * np.exp(-((ii - 0.4 * nx) ** 2 + (jj - 0.5 * ny) ** 2) / (0.05 * nx * ny))
+    hu = 0.05 * rng.standard_normal((ny, nx))
+    hv = 0.05 * rng.standard_normal((ny, nx))
+    return h, hu, hv, z, np.zeros((ny, nx))
+
+
+@pytest.mark.parametrize("solver", [O.ROE, O.HYDROSTATIC_ROE])
+def test_hierarchy_single_box_equals_two_level(solver):
+    ny, nx = 14, 16
+    st = _field(ny, nx)
+    box = (4, 3, 6, 5)
+    T = A.TwoLevelOracle(solver, *st, 1.0, 1.0, (0, 1), (1, 0), box)
+    H = A.AmrOracle(solver, *st, 1.0, 1.0, (0, 1), (1, 0), {1: [(8, 6, 12, 10)]})
+    for _ in range(3):
+        dts = H.compute_dt(0.4)
+        T.step(*dts)
+        H.step(dts=dts)
+    # the hierarchy blends the second substep's ghosts with AMReX's clock
+    # ratio (t - t_old) / (t_new - t_old), which can miss 1/2 by an ulp; the
+    # two-level restatement uses 1/2 exactly
+    tol = dict(rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(H.levels[0].interior(H.levels[0].U[0]), T.C["h"], **tol)
+    np.testing.assert_allclose(H.levels[0].interior(H.levels[0].U[1]), T.C["hu"], **tol)
+    fine = H.levels[1].U[:, G + 6:G + 16, G + 8:G + 20]
+    np.testing.assert_allclose(fine, T.fine_valid(), **tol)
+
+
+@pytest.mark.parametrize("solver", [O.ROE, O.HYDROSTATIC_ROE])
+def test_hierarchy_split_boxes_equal_merged_box(solver):
+    ny, nx = 16, 20
+    st = _field(ny, nx, seed=3)
+    merged = A.AmrOracle(solver, *st, 1.0, 1.0, (0, 0), (0, 0), {1: [(8, 8, 20, 12)]})
+    split = A.AmrOracle(solver, *st, 1.0, 1.0, (0, 0), (0, 0),
+                        {1: [(8, 8, 12, 12), (20, 8, 8, 6), (20, 14, 8, 6)]})
+    for _ in range(3):
+        dts = merged.compute_dt(0.4)
+        merged.step(dts=dts)
+        split.step(dts=dts)
+    for l in range(2):
+        a, b = merged.levels[l], split.levels[l]
+        np.testing.assert_array_equal(a.interior(a.U)[:, a.interior(a.mask)],
+                                      b.interior(b.U)[:, b.interior(b.mask)])
+
+
+@pytest.mark.parametrize("subcycle", [True, False])
+def test_hierarchy_three_levels_conserve_mass(subcycle):
+    ny, nx = 16, 16
+    st = _field(ny, nx, seed=5)
+    boxes = {1: [(6, 6, 14, 12), (20, 6, 6, 10)], 2: [(18, 18, 16, 12)]}
+    H = A.AmrOracle(O.HYDROSTATIC_ROE, *st, 1.0, 1.0, (0, 0), (0, 0), boxes, subcycle=subcycle)
+    for l in (1, 0):                                   # composite start
+        H._average_down(l)
+    m0 = H.levels[0].interior(H.levels[0].U[0]).sum()
+    for _ in range(4):
+        H.step(cfl=0.4)
+    m1 = H.levels[0].interior(H.levels[0].U[0]).sum()
+    assert abs(m1 - m0) / m0 < 1e-14
+
+
+def test_hierarchy_lake_at_rest_three_levels():
+    ny, nx = 16, 16
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    z = 0.3 * np.sin(0.7 * ii) * np.cos(0.5 * jj)
+    zero = np.zeros((ny, nx))
+    boxes = {1: [(4, 4, 16, 14), (20, 8, 8, 12)], 2: [(14, 14, 20, 16)]}
+    H = A.AmrOracle(O.HYDROSTATIC_ROE, 2.0 - z, zero, zero, z, np.full((ny, nx), 0.02),
+                    1.0, 1.0, (0, 0), (0, 0), boxes)
+    for _ in range(3):
+        H.step(cfl=0.4)
+    for lv in H.levels:
+        m = lv.interior(lv.mask)
+        np.testing.assert_allclose((lv.interior(lv.U[0]) + lv.interior(lv.T[0]))[m], 2.0,
+                                   rtol=0, atol=1e-13)
+        np.testing.assert_allclose(lv.interior(lv.U[1:])[:, m], 0.0, rtol=0, atol=1e-13)
