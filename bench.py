@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time the other workloads")
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
-    ap.add_argument("--only", choices=["ensemble", "adjoint", "two_level"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "adjoint", "two_level", "amr"], default=None,
                     help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="steps of the 2048^2 calibration rollout timed forward + adjoint (0: skip)")
@@ -53,6 +53,8 @@ def parse():
                     help="timed steps of the batched ensemble and of the per-scenario loop (0: skip)")
     ap.add_argument("--two-level-steps", type=int, default=100,
                     help="timed coarse steps of the two-level dam break (0: skip)")
+    ap.add_argument("--amr-steps", type=int, default=100,
+                    help="timed coarse steps of the 3-level cylinder dam break hierarchy (0: skip)")
     return ap.parse_args()
 
 
@@ -335,6 +337,57 @@ def time_two_level(steps, arith="fast"):
     return out
 
 
+def time_amr_c3(steps, arith="fast"):
+    """The cylinder dam break on an AMR hierarchy (two refined levels at
+    ratio 2): 512^2 base on [0,1]^2, h = 2 inside r = 0.2 around the centre,
+    0.5 outside, bed 0.05 U[0,1) (seeded), Manning 0.03, hydrostatic Roe,
+    transmissive sides, tags where |grad h| > 0.1, regrid every 4 coarse
+    steps, blocking factor 16, subcycled, FP64.  Times `steps` coarse steps
+    (after 4 warm-up steps) on the wall clock, whose timed region ends with a
+    synchronisation, and counts composite cell updates (every level's cells
+    times its substeps)."""
+    import torch
+    from hydroexa_amd import set_arith
+    from hydroexa_amd.amr import AmrHierarchy
+
+    set_arith(arith)
+    n = 512
+    dx = 1.0 / n
+    xc = (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(xc, xc)
+    rng = np.random.default_rng(3)
+    z = 0.05 * rng.uniform(size=(n, n))
+    eta = np.where(np.hypot(X - 0.5, Y - 0.5) < 0.2, 2.0, 0.5)
+    h = np.maximum(eta - z, 0.0)
+    zero = np.zeros_like(h)
+    H = AmrHierarchy("HydrostaticRoe", h, zero, zero, z, np.full_like(h, 0.03), dx, dx,
+                     (1, 1), (1, 1), max_level=2, regrid_int=4, blocking_factor=16,
+                     max_grid_size=128, tag_thresh=[(0.1, 1e300, 1e300)])
+    for _ in range(4):
+        H.step(0.45)
+    torch.cuda.synchronize()
+    cells = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        H.step(0.45)
+        cells += sum(lv.interior(lv.mask).sum().item() * (2 ** l if H.subcycle else 1)
+                     for l, lv in enumerate(H.levels))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    set_arith("exact")
+    out = {"workload": "cylinder dam break 512^2 base, max_level 2 at ratio 2, hydrostatic Roe + "
+                       "Manning 0.03, random bed, tags |grad h| > 0.1, regrid_int 4, subcycled, FP64",
+           "arith": arith, "coarse_steps": steps, "wall_ms_per_coarse_step": wall / steps * 1e3,
+           "composite_cell_updates_per_step": cells / steps,
+           "composite_Mcups": cells / wall / 1e6,
+           "boxes_per_level": [len(lv.boxes) for lv in H.levels],
+           "cells_per_level": [int(lv.interior(lv.mask).sum().item()) for lv in H.levels],
+           "time": H.time}
+    del H
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -350,6 +403,9 @@ def main():
         return
     if args.only == "adjoint":
         print(json.dumps({"adjoint": time_adjoint(args.adjoint_steps, 20, args.arith)}), flush=True)
+        return
+    if args.only == "amr":
+        print(json.dumps({"amr_c3": time_amr_c3(args.amr_steps, args.arith)}), flush=True)
         return
     if args.only == "two_level":
         print(json.dumps({"two_level": time_two_level(args.two_level_steps, args.arith)}),
@@ -428,6 +484,8 @@ def main():
         out["adjoint"] = time_adjoint(args.adjoint_steps, 20, args.arith)
     if args.two_level_steps > 0:
         out["two_level"] = time_two_level(args.two_level_steps, args.arith)
+    if args.amr_steps > 0:
+        out["amr_c3"] = time_amr_c3(args.amr_steps, args.arith)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W.dam_break(args.n, 1) if args.solver == "Roe" else w,
                                            args.cpu_seconds)

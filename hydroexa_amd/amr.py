@@ -254,6 +254,7 @@ class DenseLevel:
         self.terrain = torch.zeros((2,) + shape, dtype=dtype, device="cuda")
         self.mask = torch.zeros(shape, dtype=torch.uint8, device="cuda")
         self.boxes: List[Box] = []
+        self.patches: List[Box] = []
         self.t_new = 0.0
         self.t_old = -1.0e200
         self.faces: List[Tuple[torch.Tensor, torch.Tensor]] = []
@@ -291,12 +292,36 @@ class DenseLevel:
         return t[..., NG:NG + self.ny, NG:NG + self.nx]
 
     def set_boxes(self, boxes: List[Box]) -> None:
-        self.boxes = [tuple(int(v) for v in b) for b in boxes]
+        """The level's grids (bookkeeping, as AMReX's BoxArray) and the patches
+        it executes on: maximal rectangles of their union, aligned to 2.
+        Splitting a region into boxes changes no result (every box reads its
+        neighbours' cells as ghosts and their shared faces carry the same
+        fluxes), so fewer, larger patches give the same answer with fewer
+        launches."""
+        from math import gcd
+        from .regrid import _rects
+        new = [tuple(int(v) for v in b) for b in boxes]
+        if new == self.boxes:
+            return False
+        self.boxes = new
         self.mask.zero_()
         for (i0, j0, w, h) in self.boxes:
             self.mask[NG + j0:NG + j0 + h, NG + i0:NG + i0 + w] = 1
+        if len(self.boxes) == 1:
+            self.patches = list(self.boxes)
+        else:
+            # rasterise the union at the boxes' common alignment (at least 2)
+            g = 0
+            for b in self.boxes:
+                for v in b:
+                    g = gcd(g, v)
+            g = max(g, 1)
+            m = np.zeros((-(-self.ny // g), -(-self.nx // g)), bool)
+            for (i0, j0, w, h) in self.boxes:
+                m[j0 // g:(j0 + h) // g, i0 // g:(i0 + w) // g] = True
+            self.patches = [(g * a, g * b, g * w, g * h) for (a, b, w, h) in _rects(m)]
         self.faces = []
-        for (i0, j0, w, h) in self.boxes:
+        for (i0, j0, w, h) in self.patches:
             ci0, cj0, ncx, ncy = i0 // 2, j0 // 2, w // 2, h // 2
             cx = [(0, ci0, cj0 + m, 0) for m in range(ncy)] + \
                  [(0, ci0 + ncx, cj0 + m, 1) for m in range(ncy)]
@@ -308,6 +333,7 @@ class DenseLevel:
                  [(1, 2 * m + s, h, 0) for m in range(ncx) for s in (0, 1)]
             T = lambda a: torch.as_tensor(np.asarray(a, dtype=np.int32).reshape(-1, 4), device="cuda")
             self.faces.append((T(cx + cy), T(fx + fy)))
+        return True
 
     def swap(self) -> None:
         self.U_new, self.U_old = self.U_old, self.U_new
@@ -370,6 +396,7 @@ class AmrHierarchy:
         self.regs: List[List[torch.Tensor]] = [[] for _ in range(max_level + 1)]
         self.time = 0.0
         self.iteration = 0
+        self.ctx = context()
         self._make_grids(0, 0.0, from_scratch=True)
 
     # -- construction ----------------------------------------------------
@@ -421,7 +448,7 @@ class AmrHierarchy:
                 full.ng = 0
                 for d in range(2):
                     full.bc_lo[d], full.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
-                L.check(lib.hx_tag_level(context(), self.solver, C.byref(full), float(time), thr,
+                L.check(lib.hx_tag_level(self.ctx, self.solver, C.byref(full), float(time), thr,
                                          tags.data_ptr(), _stream()), "hx_tag_level")
                 valid = lv.interior(lv.mask).bool().cpu().numpy()
                 from . import regrid as RG
@@ -439,18 +466,19 @@ class AmrHierarchy:
             else:
                 fl = self.levels[k + 1]
                 old_mask = fl.mask.clone()
-            fl.set_boxes(new)
+            if not fl.set_boxes(new) and old_mask is not None:
+                continue                        # the same grids: nothing to remake
             if from_scratch and getattr(fl, "from_static", False):
                 continue                        # MakeNewLevelFromScratch: the static layer
             cp = lv.patch(lv.U_new)
             cp.ng = 0
             for d in range(2):
                 cp.bc_lo[d], cp.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
-            for b in fl.boxes:
+            for b in fl.patches:
                 bp = fl.patch(fl.U_new, b)
                 skip = fl.mask_addr(old_mask, b[0], b[1]) if old_mask is not None else None
                 L.check(lib.hx_fill_fine_region(
-                    context(), C.byref(cp), C.byref(cp), 0.0, 2, b[0] // 2, b[1] // 2, C.byref(bp),
+                    self.ctx, C.byref(cp), C.byref(cp), 0.0, 2, b[0] // 2, b[1] // 2, C.byref(bp),
                     1, skip, fl.pitch, fl.addr(fl.U_new, 0, b[0], b[1]),
                     fl.addr(fl.U_new, 1, b[0], b[1]), fl.addr(fl.U_new, 2, b[0], b[1]), _stream()),
                     "hx_fill_fine_region")
@@ -479,16 +507,16 @@ class AmrHierarchy:
             alpha = (t - c.t_old) / (c.t_new - c.t_old)
         po, pn = self._crse_patch(c, co), self._crse_patch(c, cn)
         lib = L.load()
-        for b in f.boxes:
+        for b in f.patches:
             bp = f.patch(f.U_new, b)
             L.check(lib.hx_fill_fine_region(
-                context(), C.byref(po), C.byref(pn), float(alpha), 2, b[0] // 2, b[1] // 2,
+                self.ctx, C.byref(po), C.byref(pn), float(alpha), 2, b[0] // 2, b[1] // 2,
                 C.byref(bp), 0, f.mask_addr(f.mask, b[0], b[1]), f.pitch,
                 f.addr(f.U_new, 0, b[0], b[1]), f.addr(f.U_new, 1, b[0], b[1]),
                 f.addr(f.U_new, 2, b[0], b[1]), _stream()), "hx_fill_fine_region")
 
     def _flux(self, patch, faces, pair, sx, sy, accumulate, reg) -> None:
-        L.check(L.load().hx_face_fluxes(context(), self.solver, C.byref(patch), faces.shape[0],
+        L.check(L.load().hx_face_fluxes(self.ctx, self.solver, C.byref(patch), faces.shape[0],
                                         C.c_void_p(faces.data_ptr()), pair, float(sx), float(sy),
                                         int(accumulate), C.c_void_p(reg.data_ptr()), _stream()),
                 "hx_face_fluxes")
@@ -502,16 +530,16 @@ class AmrHierarchy:
             fl = self.levels[lev + 1]
             cp = self._crse_patch(lv, lv.U_new)
             self.regs[lev + 1] = []
-            for b, (cf, _) in zip(fl.boxes, fl.faces):
+            for b, (cf, _) in zip(fl.patches, fl.faces):
                 reg = torch.zeros((cf.shape[0], 3), dtype=self.dtype, device="cuda")
                 self._flux(cp, cf, 1, -dt * lv.dy, -dt * lv.dx, 0, reg)
                 self.regs[lev + 1].append(reg)
         if self.do_reflux and lev > 0:
-            for b, (_, ff), reg in zip(lv.boxes, lv.faces, self.regs[lev]):
+            for b, (_, ff), reg in zip(lv.patches, lv.faces, self.regs[lev]):
                 self._flux(lv.patch(lv.U_new, b), ff, 2, dt * lv.dy, dt * lv.dx, 1, reg)
         lv.swap()
-        for b in lv.boxes:
-            L.check(lib.hx_advance_level(context(), self.solver, C.byref(lv.patch(lv.U_old, b)),
+        for b in lv.patches:
+            L.check(lib.hx_advance_level(self.ctx, self.solver, C.byref(lv.patch(lv.U_old, b)),
                                          float(dt), lv.addr(lv.U_new, 0, b[0], b[1]),
                                          lv.addr(lv.U_new, 1, b[0], b[1]),
                                          lv.addr(lv.U_new, 2, b[0], b[1]), lv.pitch, _stream()),
@@ -522,8 +550,8 @@ class AmrHierarchy:
         cp = self._crse_patch(lv, lv.U_new)
         lib = L.load()
         for orient in range(4):            # AMReX Orientation order: x-lo, y-lo, x-hi, y-hi
-            for b, reg in zip(fl.boxes, self.regs[lev + 1]):
-                L.check(lib.hx_reflux_side(context(), C.byref(cp), lv.addr(lv.U_new, 0, 0, 0),
+            for b, reg in zip(fl.patches, self.regs[lev + 1]):
+                L.check(lib.hx_reflux_side(self.ctx, C.byref(cp), lv.addr(lv.U_new, 0, 0, 0),
                                            lv.addr(lv.U_new, 1, 0, 0), lv.addr(lv.U_new, 2, 0, 0),
                                            b[0] // 2, b[1] // 2, b[2] // 2, b[3] // 2,
                                            C.c_void_p(reg.data_ptr()), orient, _stream()),
@@ -533,8 +561,8 @@ class AmrHierarchy:
         lv, fl = self.levels[lev], self.levels[lev + 1]
         cp = self._crse_patch(lv, lv.U_new)
         lib = L.load()
-        for b in fl.boxes:
-            L.check(lib.hx_average_down_state(context(), C.byref(fl.patch(fl.U_new, b)), 2,
+        for b in fl.patches:
+            L.check(lib.hx_average_down_state(self.ctx, C.byref(fl.patch(fl.U_new, b)), 2,
                                               b[0] // 2, b[1] // 2, C.byref(cp),
                                               lv.addr(lv.U_new, 0, 0, 0), lv.addr(lv.U_new, 1, 0, 0),
                                               lv.addr(lv.U_new, 2, 0, 0), _stream()),
@@ -543,9 +571,9 @@ class AmrHierarchy:
     # -- dt and the recursive step -----------------------------------------
     def level_dt(self, lev: int, cfl: float) -> float:
         lv = self.levels[lev]
-        arr = (L.HxPatch * len(lv.boxes))(*[lv.patch(lv.U_new, b) for b in lv.boxes])
+        arr = (L.HxPatch * len(lv.patches))(*[lv.patch(lv.U_new, b) for b in lv.patches])
         dt, vel = C.c_double(), C.c_double()
-        L.check(L.load().hx_boxes_dt(context(), self.solver, len(lv.boxes), arr, float(cfl),
+        L.check(L.load().hx_boxes_dt(self.ctx, self.solver, len(lv.patches), arr, float(cfl),
                                      C.byref(dt), C.byref(vel), _stream()), "hx_boxes_dt")
         return dt.value
 

@@ -156,6 +156,12 @@ def cluster(tags: np.ndarray, grid_eff: float) -> List[Box]:
             for c in work]
 
 
+def _row_runs(row: np.ndarray) -> List[Tuple[int, int]]:
+    d = np.diff(np.concatenate(([0], row.astype(np.int8), [0])))
+    starts, ends = np.flatnonzero(d == 1), np.flatnonzero(d == -1)
+    return list(zip(starts.tolist(), ends.tolist()))
+
+
 def _rects(mask: np.ndarray) -> List[Box]:
     """A partition of mask's true cells into rectangles (row runs merged
     down while they repeat)."""
@@ -163,18 +169,7 @@ def _rects(mask: np.ndarray) -> List[Box]:
     out = []
     open_runs = {}
     for j in range(ny + 1):
-        runs = set()
-        if j < ny:
-            row = mask[j]
-            i = 0
-            while i < nx:
-                if row[i]:
-                    i0 = i
-                    while i < nx and row[i]:
-                        i += 1
-                    runs.add((i0, i))
-                else:
-                    i += 1
+        runs = set(_row_runs(mask[j])) if j < ny else set()
         nxt = {}
         for r, j0 in open_runs.items():
             if r in runs:
@@ -244,7 +239,13 @@ def new_grids(tags: np.ndarray, valid: np.ndarray, ratio: int, blocking_factor: 
     if nx % b or ny % b:
         raise ValueError(f"level extents {nx}x{ny} are not multiples of blocking_factor/ratio = {b}")
     tb = coarsen_tags(t, b)
-    allowed = erode(valid, nest).reshape(ny // b, b, nx // b, b).all(axis=(1, 3))
+    vb_all = valid.reshape(ny // b, b, nx // b, b).all(axis=(1, 3))
+    if 1 <= nest <= b and (vb_all == coarsen_tags(valid, b)).all():
+        # valid is a union of whole blocks: a block is nested when it and its
+        # eight neighbours (outside the level counts) are valid
+        allowed = erode(vb_all, 1)
+    else:
+        allowed = erode(valid, nest).reshape(ny // b, b, nx // b, b).all(axis=(1, 3))
     tb &= allowed
     boxes = intersect_nesting(cluster(tb, grid_eff), allowed, tb)
     fine = [((i0 * b) * ratio, (j0 * b) * ratio, (w * b) * ratio, (h * b) * ratio)
