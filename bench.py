@@ -363,17 +363,24 @@ def time_amr_c3(steps, arith="fast"):
     H = AmrHierarchy("HydrostaticRoe", h, zero, zero, z, np.full_like(h, 0.03), dx, dx,
                      (1, 1), (1, 1), max_level=2, regrid_int=4, blocking_factor=16,
                      max_grid_size=128, tag_thresh=[(0.1, 1e300, 1e300)])
-    for _ in range(4):
-        H.step(0.45)
+    H.run_native(1e30, 0.45, max_steps=4)
     torch.cuda.synchronize()
+    # composite cell updates: every level's cells times its substeps per
+    # coarse step, counted on the grids of each step (host side, untimed)
     cells = 0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        H.step(0.45)
-        cells += sum(lv.interior(lv.mask).sum().item() * (2 ** l if H.subcycle else 1)
-                     for l, lv in enumerate(H.levels))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    wall = 0.0
+    done = 0
+    while done < steps:
+        k = min(4, steps - done)            # one regrid interval per call
+        per = sum(lv.interior(lv.mask).sum().item() * (2 ** l if H.subcycle else 1)
+                  for l, lv in enumerate(H.levels))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        H.run_native(1e30, 0.45, max_steps=k)
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+        cells += per * k
+        done += k
     set_arith("exact")
     out = {"workload": "cylinder dam break 512^2 base, max_level 2 at ratio 2, hydrostatic Roe + "
                        "Manning 0.03, random bed, tags |grad h| > 0.1, regrid_int 4, subcycled, FP64",

@@ -43,8 +43,20 @@ EXPORTS = (
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
     "hx_conservation", "hx_face_fluxes", "hx_fill_fine_ghosts", "hx_reflux",
     "hx_average_down_state", "hx_two_level_run", "hx_fill_fine_region", "hx_reflux_side",
-    "hx_boxes_dt",
+    "hx_boxes_dt", "hx_amr_run", "hx_tag_blocks",
 )
+
+
+class HxAmrLevel(C.Structure):
+    """Mirror of ``hx_amr_level``."""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("npatch", C.c_int32), ("cur", C.c_int32),
+                ("patches", C.POINTER(C.c_int32)), ("u", C.c_void_p * 2), ("terrain", C.c_void_p),
+                ("mask", C.c_void_p), ("regs", C.c_void_p), ("dx", C.c_double), ("dy", C.c_double),
+                ("t_new", C.c_double), ("t_old", C.c_double), ("istep", C.c_int32),
+                ("last_regrid", C.c_int32)]
+
+
+HX_AMR_REGRID_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double)
 
 
 class HxPatch(C.Structure):
@@ -114,6 +126,11 @@ def load() -> C.CDLL:
     L.hx_reflux_side.argtypes = [vp, pp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp,
                                  C.c_int, vp]
     L.hx_boxes_dt.argtypes = [vp, C.c_int, C.c_int, pp, C.c_double, dp, dp, vp]
+    L.hx_tag_blocks.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.c_int, C.c_int, C.c_int,
+                                C.c_int, vp, vp]
+    L.hx_amr_run.argtypes = [vp, C.c_int, C.c_int, C.POINTER(HxAmrLevel), C.POINTER(C.c_int),
+                             C.c_int, i32p, i32p, C.c_int, C.c_int, C.c_int, HX_AMR_REGRID_FN, vp,
+                             C.c_double, C.c_double, C.c_int, dp, dp, i32p, vp]
     L.hx_two_level_run.argtypes = [vp, C.c_int, pp, vp, vp, vp, pp, vp, vp, vp, C.c_int, C.c_int,
                                    C.c_int, C.c_double, C.c_int, vp, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int32), vp]

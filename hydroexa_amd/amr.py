@@ -333,6 +333,9 @@ class DenseLevel:
                  [(1, 2 * m + s, h, 0) for m in range(ncx) for s in (0, 1)]
             T = lambda a: torch.as_tensor(np.asarray(a, dtype=np.int32).reshape(-1, 4), device="cuda")
             self.faces.append((T(cx + cy), T(fx + fy)))
+        nreg = sum(6 * (w // 2 + h // 2) for (_, _, w, h) in self.patches)
+        self.regbuf = torch.zeros(max(nreg, 1), dtype=self.dtype, device="cuda")
+        self.patch_arr = np.ascontiguousarray(np.asarray(self.patches, np.int32).reshape(-1, 4))
         return True
 
     def swap(self) -> None:
@@ -450,10 +453,25 @@ class AmrHierarchy:
                     full.bc_lo[d], full.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
                 L.check(lib.hx_tag_level(self.ctx, self.solver, C.byref(full), float(time), thr,
                                          tags.data_ptr(), _stream()), "hx_tag_level")
-                valid = lv.interior(lv.mask).bool().cpu().numpy()
                 from . import regrid as RG
-                new = RG.new_grids(tags.bool().cpu().numpy(), valid, 2, self.bf, self.mgs,
-                                   self.nbuf, self.eff, self.nest)
+                b = max(self.bf // 2, 1)
+                aligned = (lv.nx % b == 0 and lv.ny % b == 0 and self.nest <= b and
+                           all(v % b == 0 for bx in lv.boxes for v in bx))
+                if aligned:
+                    # tags grown, masked and blocked on the device; only the
+                    # block array comes to the host
+                    blk = torch.empty((lv.ny // b, lv.nx // b), dtype=torch.uint8, device="cuda")
+                    L.check(lib.hx_tag_blocks(self.ctx, tags.data_ptr(), lv.nx,
+                                              lv.mask_addr(lv.mask, 0, 0), lv.pitch, lv.nx, lv.ny,
+                                              self.nbuf, b, blk.data_ptr(), _stream()),
+                            "hx_tag_blocks")
+                    allowed = RG.allowed_blocks(lv.boxes, lv.ny, lv.nx, b, self.nest)
+                    new = RG.grids_from_blocks(blk.bool().cpu().numpy(), allowed, b, 2, self.eff,
+                                               self.mgs)
+                else:
+                    valid = lv.interior(lv.mask).bool().cpu().numpy()
+                    new = RG.new_grids(tags.bool().cpu().numpy(), valid, 2, self.bf, self.mgs,
+                                       self.nbuf, self.eff, self.nest)
             if not new:
                 del self.levels[k + 1:]
                 break
@@ -634,6 +652,70 @@ class AmrHierarchy:
             self.step(cfl)
             k += 1
         return k
+
+    # -- native run (hx_amr_run) -------------------------------------------
+    def _publish(self) -> None:
+        """Write the level table hx_amr_run reads (current state in u[0])."""
+        for l, lv in enumerate(self.levels):
+            t = self._table[l]
+            t.nx, t.ny, t.npatch, t.cur = lv.nx, lv.ny, len(lv.patches), 0
+            t.patches = lv.patch_arr.ctypes.data_as(C.POINTER(C.c_int32))
+            t.u[0], t.u[1] = lv.U_new.data_ptr(), lv.U_old.data_ptr()
+            t.terrain, t.mask, t.regs = lv.terrain.data_ptr(), lv.mask.data_ptr(), lv.regbuf.data_ptr()
+            t.dx, t.dy, t.t_new, t.t_old = lv.dx, lv.dy, lv.t_new, lv.t_old
+            t.istep, t.last_regrid = self.istep[l], self.last_regrid[l]
+        self._nlev.value = len(self.levels)
+
+    def _absorb(self) -> None:
+        """Take back buffer roles, clocks and step counts from the table."""
+        for l, lv in enumerate(self.levels):
+            t = self._table[l]
+            if t.cur == 1:
+                lv.swap()
+            lv.t_new, lv.t_old = t.t_new, t.t_old
+            self.istep[l], self.last_regrid[l] = t.istep, t.last_regrid
+
+    def run_native(self, t_end: float, cfl: float, max_steps: int = 2 ** 31 - 1) -> np.ndarray:
+        """Coarse steps until t_end (or max_steps) in C++ (hx_amr_run): the same
+        kernels and order as step(); the host is called back only to regrid.
+        Returns the coarse dt of each step."""
+        self._table = (L.HxAmrLevel * (self.max_level + 1))()
+        self._nlev = C.c_int(len(self.levels))
+        err = []
+
+        def regrid(user, lev, time):
+            try:
+                self._absorb()
+                self._make_grids(lev, time)
+                self._publish()
+                return 0
+            except Exception as e:          # reported after the run
+                err.append(e)
+                return 1
+
+        cb = L.HX_AMR_REGRID_FN(regrid)
+        self._publish()
+        n = min(max_steps, 10 ** 7)
+        hist = np.zeros(max(n, 1) if n < 100000 else 100000)
+        tm = C.c_double(self.time)
+        done = C.c_int32()
+        lo = (C.c_int32 * 2)(*self.bc_lo)
+        hi = (C.c_int32 * 2)(*self.bc_hi)
+        rc = L.load().hx_amr_run(self.ctx, self.solver, _DT[self.dtype], self._table,
+                                 C.byref(self._nlev), self.max_level, lo, hi, int(self.subcycle),
+                                 int(self.do_reflux), int(self.regrid_int), cb, None, float(cfl),
+                                 float(t_end), int(min(n, len(hist))), C.byref(tm),
+                                 hist.ctypes.data_as(C.POINTER(C.c_double)), C.byref(done),
+                                 _stream())
+        self._absorb()
+        self.time = tm.value
+        self.iteration += done.value
+        if err:
+            raise err[0]
+        if rc == L.HX_E_DT_COLLAPSE:
+            raise L.DtCollapse(rc, f"hx_amr_run (completed {done.value} steps)")
+        L.check(rc, "hx_amr_run")
+        return hist[:done.value]
 
     # -- views -------------------------------------------------------------
     def state(self, lev: int) -> torch.Tensor:

@@ -1620,3 +1620,342 @@ extern "C" int hx_boxes_dt(hx_ctx* c, int solver, int nbox, const hx_patch* boxe
     if (max_vel_out) *max_vel_out = (sl.flags ? vmax : -1.7976931348623157e308);
     return (est < 1.e-12) ? HX_E_DT_COLLAPSE : HX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Native AMR run: the recursive step of AmrMeshState::TimeStepWithSubcycling
+// (src/state/AmrMeshState_Physics.cpp:50-147) with ComputeDt (:13-45) over
+// levels held whole in device memory, launched from C++.  The host is called
+// back only to regrid.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct AmrRun {
+    hx_ctx* c;
+    int solver, dtype;
+    hx_amr_level* lv;
+    int* nlev;
+    int max_level, subcycle, reflux, regrid_int;
+    int32_t bclo[2], bchi[2];
+    hx_amr_regrid_fn regrid;
+    void* user;
+    double dt[HX_AMR_MAX_LEVELS];
+    CflSlot* slots;     // device, one per level
+    hipStream_t st;
+};
+
+size_t amr_es(int dtype) { return dtype == HX_F64 ? 8 : 4; }
+
+// hx_patch of a rectangle of a level (i0, j0, w, h; w = nx, h = ny and i0 = j0
+// = 0 for the whole level) on buffer b: sides on the domain boundary keep the
+// physical rule, the others read stored ghosts.
+hx_patch amr_patch(const AmrRun& r, int l, int b, int i0, int j0, int w, int h)
+{
+    const hx_amr_level& L = r.lv[l];
+    const size_t es = amr_es(r.dtype);
+    const int64_t pitch = L.nx + 4;
+    const size_t plane = (size_t)(L.ny + 4) * pitch * es;
+    const size_t off = ((size_t)(j0 + 2) * pitch + (i0 + 2)) * es;
+    hx_patch p;
+    std::memset(&p, 0, sizeof(p));
+    p.nx = w; p.ny = h; p.ng = 2; p.dtype = r.dtype; p.pitch = pitch;
+    char* u = (char*)L.u[b];
+    p.h = u + off; p.hu = u + plane + off; p.hv = u + 2 * plane + off;
+    p.eta = nullptr;
+    p.z = (const char*)L.terrain + off;
+    p.n = (const char*)L.terrain + plane + off;
+    p.dx = L.dx; p.dy = L.dy;
+    p.bc_lo[0] = i0 == 0 ? r.bclo[0] : HX_BC_GHOST;
+    p.bc_lo[1] = j0 == 0 ? r.bclo[1] : HX_BC_GHOST;
+    p.bc_hi[0] = i0 + w == L.nx ? r.bchi[0] : HX_BC_GHOST;
+    p.bc_hi[1] = j0 + h == L.ny ? r.bchi[1] : HX_BC_GHOST;
+    return p;
+}
+
+hx_patch amr_level_patch(const AmrRun& r, int l, int b)
+{
+    hx_patch p = amr_patch(r, l, b, 0, 0, r.lv[l].nx, r.lv[l].ny);
+    p.ng = 0;
+    for (int d = 0; d < 2; ++d) { p.bc_lo[d] = r.bclo[d]; p.bc_hi[d] = r.bchi[d]; }
+    return p;
+}
+
+bool almost_equal(double x, double y)
+{
+    const double m = std::max(std::max(std::fabs(x), std::fabs(y)), 1e-300);
+    return std::fabs(x - y) <= 1e-12 * m;
+}
+
+template <typename R>
+int amr_fill(AmrRun& r, int l, double t)
+{
+    hx_amr_level &C = r.lv[l - 1], &F = r.lv[l];
+    int bo, bn;
+    double alpha = 0.0;
+    if (almost_equal(t, C.t_new)) { bo = bn = C.cur; }
+    else if (almost_equal(t, C.t_old)) { bo = bn = C.cur ^ 1; }
+    else { bo = C.cur ^ 1; bn = C.cur; alpha = (t - C.t_old) / (C.t_new - C.t_old); }
+    const hx_patch po = amr_level_patch(r, l - 1, bo), pn = amr_level_patch(r, l - 1, bn);
+    for (int k = 0; k < F.npatch; ++k) {
+        const int32_t* b = F.patches + 4 * k;
+        const hx_patch fp = amr_patch(r, l, F.cur, b[0], b[1], b[2], b[3]);
+        const uint8_t* skip = F.mask + (size_t)(b[1] + 2) * (F.nx + 4) + (b[0] + 2);
+        const int rc = fillfine_t<R>(&po, &pn, alpha, b[0] / 2, b[1] / 2, &fp, 0, (void*)fp.h,
+                                     (void*)fp.hu, (void*)fp.hv, r.st, nullptr, skip, F.nx + 4);
+        if (rc) return rc;
+    }
+    return HX_OK;
+}
+
+int amr_reg_offset(const hx_amr_level& F, int k)
+{
+    int off = 0;
+    for (int q = 0; q < k; ++q) off += 6 * (F.patches[4 * q + 2] / 2 + F.patches[4 * q + 3] / 2);
+    return off;
+}
+
+template <typename R>
+int amr_advance(AmrRun& r, int l, double t, double dt)
+{
+    hx_amr_level& L = r.lv[l];
+    int rc;
+    if (l > 0 && (rc = amr_fill<R>(r, l, t))) return rc;
+    if (r.reflux && l < *r.nlev - 1) {            // CrseInit for every patch of level l + 1
+        hx_amr_level& F = r.lv[l + 1];
+        const hx_patch cp = amr_level_patch(r, l, L.cur);
+        for (int k = 0; k < F.npatch; ++k) {
+            const int32_t* b = F.patches + 4 * k;
+            FaceReq q{};
+            q.pair = 1; q.accumulate = 0; q.box_fine = 0;
+            q.ci0 = b[0] / 2; q.cj0 = b[1] / 2; q.ncx = b[2] / 2; q.ncy = b[3] / 2;
+            q.nfaces = 2 * (q.ncx + q.ncy);
+            q.sx = -dt * L.dy; q.sy = -dt * L.dx;
+            R* reg = (R*)F.regs + amr_reg_offset(F, k);
+            if ((rc = launch_faces(r.c, r.solver, &cp, q, reg, r.st))) return rc;
+        }
+    }
+    if (r.reflux && l > 0) {                      // FineAdd of this level's patches
+        for (int k = 0; k < L.npatch; ++k) {
+            const int32_t* b = L.patches + 4 * k;
+            const hx_patch fp = amr_patch(r, l, L.cur, b[0], b[1], b[2], b[3]);
+            FaceReq q{};
+            q.pair = 2; q.accumulate = 1; q.box_fine = 1;
+            q.ci0 = b[0] / 2; q.cj0 = b[1] / 2; q.ncx = b[2] / 2; q.ncy = b[3] / 2;
+            q.nfaces = 4 * (q.ncx + q.ncy);
+            q.sx = dt * L.dy; q.sy = dt * L.dx;
+            R* reg = (R*)L.regs + amr_reg_offset(L, k);
+            if ((rc = launch_faces(r.c, r.solver, &fp, q, reg, r.st))) return rc;
+        }
+    }
+    L.cur ^= 1;
+    for (int k = 0; k < L.npatch; ++k) {
+        const int32_t* b = L.patches + 4 * k;
+        const hx_patch in = amr_patch(r, l, L.cur ^ 1, b[0], b[1], b[2], b[3]);
+        const hx_patch out = amr_patch(r, l, L.cur, b[0], b[1], b[2], b[3]);
+        StepArgs<R> a{};
+        a.p = view_of<R>(&in);
+        a.ho = (R*)out.h; a.huo = (R*)out.hu; a.hvo = (R*)out.hv;
+        a.opitch = out.pitch;
+        a.dt_host = dt;
+        fill_geometry(a, r.solver, r.c->rows[r.solver][r.dtype], in.dx, in.dy);
+        if ((rc = launch(r.c, r.solver, r.dtype, a, r.st))) return rc;
+    }
+    return HX_OK;
+}
+
+template <typename R>
+int amr_reflux_avgdown(AmrRun& r, int l)
+{
+    hx_amr_level &L = r.lv[l], &F = r.lv[l + 1];
+    const hx_patch cp = amr_level_patch(r, l, L.cur);
+    int rc;
+    if (r.reflux)
+        for (int orient = 0; orient < 4; ++orient)
+            for (int k = 0; k < F.npatch; ++k) {
+                const int32_t* b = F.patches + 4 * k;
+                const R* reg = (const R*)F.regs + amr_reg_offset(F, k);
+                if ((rc = reflux_t<R>(&cp, (void*)cp.h, (void*)cp.hu, (void*)cp.hv, b[0] / 2, b[1] / 2,
+                                      b[2] / 2, b[3] / 2, reg, r.st, nullptr, orient)))
+                    return rc;
+            }
+    for (int k = 0; k < F.npatch; ++k) {
+        const int32_t* b = F.patches + 4 * k;
+        const hx_patch fp = amr_patch(r, l + 1, F.cur, b[0], b[1], b[2], b[3]);
+        if ((rc = avgdown_t<R>(&fp, b[0] / 2, b[1] / 2, &cp, (void*)cp.h, (void*)cp.hu,
+                               (void*)cp.hv, r.st)))
+            return rc;
+    }
+    return HX_OK;
+}
+
+template <typename R>
+int amr_time_step(AmrRun& r, int l, double time)
+{
+    int rc;
+    hx_amr_level* L = &r.lv[l];
+    if (r.regrid_int > 0 && l < r.max_level && L->istep > L->last_regrid &&
+        L->istep % r.regrid_int == 0 && r.regrid) {
+        const int old_n = *r.nlev;
+        if ((rc = r.regrid(r.user, l, time))) return rc;
+        for (int k = l; k < *r.nlev; ++k) r.lv[k].last_regrid = r.lv[k].istep;
+        for (int k = old_n; k < *r.nlev; ++k) r.dt[k] = r.dt[k - 1] / (r.subcycle ? 2.0 : 1.0);
+        L = &r.lv[l];
+    }
+    L->t_old = L->t_new;
+    L->t_new += r.dt[l];
+    if ((rc = amr_advance<R>(r, l, L->t_old, r.dt[l]))) return rc;
+    L->istep += 1;
+    if (l < *r.nlev - 1) {
+        const int nsub = r.subcycle ? 2 : 1;
+        for (int i = 1; i <= nsub; ++i)
+            if ((rc = amr_time_step<R>(r, l + 1, time + (i - 1) * r.dt[l + 1]))) return rc;
+        if ((rc = amr_reflux_avgdown<R>(r, l))) return rc;
+    }
+    return HX_OK;
+}
+
+// BaseSolver::compute_dt at ratio 2 and the lockstep rule: every level's
+// patches reduce into that level's slot, one synchronisation.
+template <typename R>
+int amr_compute_dt(AmrRun& r, double cfl, int* collapsed)
+{
+    const int n = *r.nlev;
+    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, r.st, r.slots, n, (int*)nullptr);
+    for (int l = 0; l < n; ++l) {
+        const hx_amr_level& L = r.lv[l];
+        for (int k = 0; k < L.npatch; ++k) {
+            const int32_t* b = L.patches + 4 * k;
+            const hx_patch p = amr_patch(r, l, L.cur, b[0], b[1], b[2], b[3]);
+            const int rc = launch_cfl<R>(&p, r.slots + l, r.st);
+            if (rc) return rc;
+        }
+    }
+    CflSlot s[HX_AMR_MAX_LEVELS];
+    HX_TRY(hipMemcpyAsync(s, r.slots, sizeof(CflSlot) * n, hipMemcpyDeviceToHost, r.st));
+    HX_TRY(hipStreamSynchronize(r.st));
+    double tmp[HX_AMR_MAX_LEVELS];
+    *collapsed = 0;
+    for (int l = 0; l < n; ++l) {
+        double est;
+        if (sizeof(R) == 8) {
+            est = slot_dt_est<double>(s[l].amax, s[l].flags);
+            tmp[l] = est * cfl;
+        } else {
+            const float e = slot_dt_est<float>(s[l].amax, s[l].flags);
+            est = e;
+            tmp[l] = (double)(e * (float)cfl);
+        }
+        if (est < 1.e-12) *collapsed = 1;
+    }
+    double dt0 = tmp[0], factor = 1.0;
+    for (int l = 1; l < n; ++l) {
+        factor *= 2.0;
+        dt0 = std::min(dt0, tmp[l] * factor);
+    }
+    r.dt[0] = dt0;
+    for (int l = 1; l < n; ++l) r.dt[l] = r.dt[l - 1] / 2.0;
+    if (!r.subcycle) {
+        double m = r.dt[0];
+        for (int l = 1; l < n; ++l) m = std::min(m, r.dt[l]);
+        for (int l = 0; l < n; ++l) r.dt[l] = m;
+    }
+    return HX_OK;
+}
+
+}  // namespace
+
+extern "C" int hx_amr_run(hx_ctx* c, int solver, int dtype, hx_amr_level* levels, int* nlevels,
+                          int max_level, const int32_t* bc_lo, const int32_t* bc_hi, int subcycle,
+                          int do_reflux, int regrid_int, hx_amr_regrid_fn regrid, void* user,
+                          double cfl, double t_end, int max_steps, double* time,
+                          double* dt_hist, int32_t* steps_done, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    if (!levels || !nlevels || *nlevels < 1 || *nlevels > max_level + 1 || !bc_lo || !bc_hi ||
+        max_level + 1 > HX_AMR_MAX_LEVELS || !time || !(cfl > 0.0) || max_steps < 0 ||
+        (dtype != HX_F64 && dtype != HX_F32))
+        return HX_E_BADARG;
+    for (int d = 0; d < 2; ++d)
+        if ((bc_lo[d] != 0 && bc_lo[d] != 1) || (bc_hi[d] != 0 && bc_hi[d] != 1)) return HX_E_BADARG;
+    if (steps_done) *steps_done = 0;
+    HX_TRY(hipSetDevice(c->device));
+    AmrRun r{};
+    r.c = c; r.solver = solver; r.dtype = dtype; r.lv = levels; r.nlev = nlevels;
+    r.max_level = max_level; r.subcycle = subcycle != 0; r.reflux = do_reflux != 0;
+    r.regrid_int = regrid_int; r.regrid = regrid; r.user = user;
+    r.bclo[0] = bc_lo[0]; r.bclo[1] = bc_lo[1]; r.bchi[0] = bc_hi[0]; r.bchi[1] = bc_hi[1];
+    r.st = (hipStream_t)stream;
+    HX_TRY(hipMalloc((void**)&r.slots, sizeof(CflSlot) * HX_AMR_MAX_LEVELS));
+    int rc = HX_OK, k = 0;
+    while (*time < t_end && k < max_steps) {
+        int collapsed = 0;
+        rc = (dtype == HX_F64) ? amr_compute_dt<double>(r, cfl, &collapsed)
+                               : amr_compute_dt<float>(r, cfl, &collapsed);
+        if (rc) break;
+        if (collapsed) { rc = HX_E_DT_COLLAPSE; break; }
+        if (dt_hist) dt_hist[k] = r.dt[0];
+        rc = (dtype == HX_F64) ? amr_time_step<double>(r, 0, *time)
+                               : amr_time_step<float>(r, 0, *time);
+        if (rc) break;
+        *time = r.lv[0].t_new;
+        ++k;
+    }
+    if (hipStreamSynchronize(r.st) != hipSuccess && !rc) rc = HX_E_HIP;
+    (void)hipFree(r.slots);
+    if (steps_done) *steps_done = k;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Regrid tag blocks: block (bi, bj) of b x b cells is set when one of its
+// valid cells lies within nbuf cells (max norm) of a tagged valid cell --
+// the tags grown by n_error_buf, kept on the level's valid cells and
+// coarsened by the blocking factor in one pass, so only the block array
+// travels to the host.
+// ---------------------------------------------------------------------------
+namespace {
+__global__ __launch_bounds__(256) void tag_blocks_kernel(const uint8_t* __restrict__ tags,
+                                                         int64_t tpitch,
+                                                         const uint8_t* __restrict__ valid,
+                                                         int64_t vpitch, int nx, int ny, int nbuf,
+                                                         int b, uint8_t* __restrict__ blocks)
+{
+    const int nbx = nx / b, nby = ny / b;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nbx * nby) return;
+    const int bi = q % nbx, bj = q / nbx;
+    uint8_t hit = 0;
+    for (int j = bj * b; j < bj * b + b && !hit; ++j)
+        for (int i = bi * b; i < bi * b + b && !hit; ++i) {
+            if (!valid[(int64_t)j * vpitch + i]) continue;
+            for (int dj = -nbuf; dj <= nbuf && !hit; ++dj) {
+                const int y = j + dj;
+                if (y < 0 || y >= ny) continue;
+                for (int di = -nbuf; di <= nbuf; ++di) {
+                    const int x = i + di;
+                    if (x < 0 || x >= nx) continue;
+                    if (tags[(int64_t)y * tpitch + x] && valid[(int64_t)y * vpitch + x]) {
+                        hit = 1;
+                        break;
+                    }
+                }
+            }
+        }
+    blocks[q] = hit;
+}
+}  // namespace
+
+extern "C" int hx_tag_blocks(hx_ctx* c, const uint8_t* tags, int64_t tag_pitch,
+                             const uint8_t* valid, int64_t valid_pitch, int nx, int ny, int nbuf,
+                             int b, uint8_t* blocks, void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (!tags || !valid || !blocks || nx < 1 || ny < 1 || nbuf < 0 || b < 1 || nx % b || ny % b ||
+        tag_pitch < nx || valid_pitch < nx)
+        return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    const int n = (nx / b) * (ny / b);
+    hipLaunchKernelGGL(tag_blocks_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       tags, tag_pitch, valid, valid_pitch, nx, ny, nbuf, b, blocks);
+    return hip_status(hipGetLastError());
+}

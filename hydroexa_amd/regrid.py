@@ -238,15 +238,33 @@ def new_grids(tags: np.ndarray, valid: np.ndarray, ratio: int, blocking_factor: 
     ny, nx = t.shape
     if nx % b or ny % b:
         raise ValueError(f"level extents {nx}x{ny} are not multiples of blocking_factor/ratio = {b}")
-    tb = coarsen_tags(t, b)
     vb_all = valid.reshape(ny // b, b, nx // b, b).all(axis=(1, 3))
     if 1 <= nest <= b and (vb_all == coarsen_tags(valid, b)).all():
-        # valid is a union of whole blocks: a block is nested when it and its
-        # eight neighbours (outside the level counts) are valid
         allowed = erode(vb_all, 1)
     else:
         allowed = erode(valid, nest).reshape(ny // b, b, nx // b, b).all(axis=(1, 3))
-    tb &= allowed
+    return grids_from_blocks(coarsen_tags(t, b), allowed, b, ratio, grid_eff, max_grid_size)
+
+
+def allowed_blocks(boxes: Sequence[Box], ny: int, nx: int, b: int, nest: int) -> np.ndarray:
+    """Nesting domain on the block grid of a level whose valid region is the
+    union of boxes aligned to b: a block is allowed when it and its eight
+    neighbours are valid (outside the level counts as valid); requires
+    1 <= nest <= b."""
+    if not 1 <= nest <= b:
+        raise ValueError("nest must lie in [1, b] for the block-grid nesting test")
+    vb = np.zeros((ny // b, nx // b), bool)
+    for (i0, j0, w, h) in boxes:
+        if i0 % b or j0 % b or w % b or h % b:
+            raise ValueError("boxes are not aligned to the block size")
+        vb[j0 // b:(j0 + h) // b, i0 // b:(i0 + w) // b] = True
+    return erode(vb, 1)
+
+
+def grids_from_blocks(tb: np.ndarray, allowed: np.ndarray, b: int, ratio: int, grid_eff: float,
+                      max_grid_size: int) -> List[Box]:
+    """Steps 3-5 of the module docstring from tags already grown and blocked."""
+    tb = tb & allowed
     boxes = intersect_nesting(cluster(tb, grid_eff), allowed, tb)
     fine = [((i0 * b) * ratio, (j0 * b) * ratio, (w * b) * ratio, (h * b) * ratio)
             for (i0, j0, w, h) in boxes]

@@ -314,11 +314,70 @@ int hx_reflux_side(hx_ctx *ctx, const hx_patch *crse, void *h, void *hu, void *h
                    int cj0, int ncx, int ncy, const void *reg, int orient, void *stream);
 
 /*
+ * Regrid tag blocks (device in and out): blocks[(j/b) * (nx/b) + i/b] = 1 when
+ * a valid cell of the b x b block lies within nbuf cells (max norm) of a
+ * cell that is tagged and valid; tags: hx_tag_level's bytes (tag_pitch per
+ * row), valid: the level's mask at its (0,0) (valid_pitch per row).  The
+ * n_error_buf growth and blocking-factor coarsening of AMReX's
+ * MakeNewGrids in one pass.
+ */
+int hx_tag_blocks(hx_ctx *ctx, const uint8_t *tags, int64_t tag_pitch, const uint8_t *valid,
+                  int64_t valid_pitch, int nx, int ny, int nbuf, int b, uint8_t *blocks,
+                  void *stream);
+
+/*
  * hx_level_dt over the valid cells of nbox boxes of one level (host array
  * of patches), one reduction and one synchronisation.
  */
 int hx_boxes_dt(hx_ctx *ctx, int solver, int nbox, const hx_patch *boxes, double cfl,
                 double *dt_out, double *max_vel_out, void *stream);
+
+/*
+ * A level of an AMR hierarchy held whole in device memory (hx_amr_run).
+ * Arrays are (ny + 4) x (nx + 4) per component with two cells of padding,
+ * valid cell (0,0) at [2][2], pitch nx + 4; state buffers hold h, hu, hv
+ * planes, terrain z, n planes.  patches: rectangles (even origin and size)
+ * covering exactly the cells the mask marks; regs: the level's flux
+ * registers as a fine level, 6 (w/2 + h/2) values per patch in patch order.
+ */
+#define HX_AMR_MAX_LEVELS 8
+typedef struct hx_amr_level {
+    int32_t nx, ny;
+    int32_t npatch;
+    int32_t cur;                  /* buffer holding the current state (in/out) */
+    const int32_t *patches;       /* host, npatch x {i0, j0, w, h}              */
+    void *u[2];                   /* device state buffers                        */
+    const void *terrain;          /* device                                      */
+    const uint8_t *mask;          /* device, 1 on the cells the level owns       */
+    void *regs;                   /* device                                      */
+    double dx, dy;
+    double t_new, t_old;          /* clocks (in/out)                             */
+    int32_t istep, last_regrid;   /* (in/out)                                    */
+} hx_amr_level;
+
+/* Regrid request from hx_amr_run: level lev's grids are due at time; the
+ * host may change levels lev + 1 .. (patches, masks, registers, new levels
+ * appended, *nlevels) in the arrays it passed.  Non-zero aborts the run. */
+typedef int (*hx_amr_regrid_fn)(void *user, int lev, double time);
+
+/*
+ * Coarse steps of an AMR hierarchy until *time reaches t_end or max_steps
+ * steps, launched from C++: per step ComputeDt (BaseSolver::compute_dt at
+ * ratio 2, lockstep sharing the finest dt; src/state/AmrMeshState_Physics.cpp:13-45)
+ * with one synchronisation, then TimeStepWithSubcycling (:50-147): regrid
+ * requests every regrid_int steps of a level, ghost rings from the coarser
+ * level at the substep's time (clock-ratio blend), side-selected flux
+ * registers (faces generated on the device), the fused advance per patch,
+ * reflux in AMReX's side order and average_down -- the same kernels and
+ * order as hydroexa_amd.amr.AmrHierarchy.step.  dt_hist (host, optional):
+ * the coarse dt of each step.  Returns HX_E_DT_COLLAPSE when a level's
+ * estimate falls below 1e-12 (before that step).
+ */
+int hx_amr_run(hx_ctx *ctx, int solver, int dtype, hx_amr_level *levels, int *nlevels,
+               int max_level, const int32_t *bc_lo, const int32_t *bc_hi, int subcycle,
+               int do_reflux, int regrid_int, hx_amr_regrid_fn regrid, void *user, double cfl,
+               double t_end, int max_steps, double *time, double *dt_hist, int32_t *steps_done,
+               void *stream);
 
 /*
  * nsteps coupled steps of one coarse level and one fine box at ratio 2,

@@ -17,6 +17,7 @@ from scipy.optimize import brentq
 
 from oracle import amr as A
 from oracle import oracle as O
+from hydroexa_amd import _lib as L
 from hydroexa_amd import set_arith
 from hydroexa_amd.amr import AmrHierarchy
 
@@ -141,7 +142,7 @@ def dambreak_run(max_level, regrid_int, t_end=15.0, nx=512, ny=64):
                      regrid_int=regrid_int, blocking_factor=32, max_grid_size=128,
                      n_error_buf=1, tag_thresh=[(0.0075, 0.0075, 0.0075)],
                      static=lambda lev: state(nx << lev, ny << lev))
-    H.run_until(t_end, 0.5)
+    H.run_native(t_end, 0.5)
     x, h = H.centerline(0.5)
     return H, x, h, l1_weighted(x, h - swashes(H.time, x, hl, hr, x0))
 
@@ -171,3 +172,81 @@ def test_reference_amr_dambreak_accuracy(gpu):
     assert amr50 < base and amr5 < base
     assert amr5 <= 1.1 * fine
     assert amr5 < amr50
+
+
+def _same_state(H1, H2):
+    assert H1.finest == H2.finest
+    for a, b in zip(H1.levels, H2.levels):
+        assert a.boxes == b.boxes
+        m = a.interior(a.mask).bool()
+        assert torch.equal(m, b.interior(b.mask).bool())
+        assert torch.equal(a.interior(a.U_new)[:, m], b.interior(b.U_new)[:, m])
+        assert a.t_new == b.t_new
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("subcycle", [True, False])
+def test_native_run_equals_python_steps_fixed(gpu, case, subcycle):
+    model, (ny, nx), lo, hi, boxes = CASES[case]
+    st = field(ny, nx, seed=9)
+    mk = lambda: AmrHierarchy(model, *st, 1.0, 1.0, lo, hi, max_level=max(boxes), boxes=boxes,
+                              subcycle=subcycle)
+    H1, H2 = mk(), mk()
+    print(f"native vs python: {case} subcycle {subcycle}", flush=True)
+    hist = H1.run_native(1e30, 0.45, max_steps=5)
+    dts = [H2.step(0.45) for _ in range(5)]
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(hist, dts)
+    assert H1.time == H2.time
+    _same_state(H1, H2)
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_native_run_equals_python_steps_with_regrid(gpu, model):
+    ny, nx = 64, 64
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    h = 1.0 + 0.5 * (np.hypot(ii - 24.0, jj - 30.0) < 8)
+    zero = np.zeros((ny, nx))
+    mk = lambda: AmrHierarchy(model, h, zero, zero, zero, zero, 1.0, 1.0, (0, 0), (0, 0),
+                              max_level=2, regrid_int=2, blocking_factor=8, max_grid_size=32,
+                              tag_thresh=[(0.02, 0.0, 0.0), (0.03, 0.0, 0.0)])
+    H1, H2 = mk(), mk()
+    H1.run_native(1e30, 0.45, max_steps=12)
+    for _ in range(12):
+        H2.step(0.45)
+    torch.cuda.synchronize()
+    _same_state(H1, H2)
+    assert H1.istep == H2.istep and H1.last_regrid == H2.last_regrid
+
+
+def test_native_run_reports_dt_collapse(gpu):
+    st = list(field(16, 16, seed=2))
+    st[1] = st[1].copy()
+    st[1][5, 5] = 1e13
+    H = AmrHierarchy("Roe", *st, 1.0, 1.0, (0, 0), (0, 0), max_level=1,
+                     boxes={1: [(8, 8, 12, 12)]})
+    before = H.state(0).clone()
+    with pytest.raises(L.DtCollapse):
+        H.run_native(1e30, 0.45, max_steps=3)
+    assert H.time == 0.0 and torch.equal(H.state(0), before)
+
+
+@pytest.mark.parametrize("nbuf,b", [(0, 4), (1, 8), (2, 4), (3, 16)])
+def test_tag_blocks_match_host_restatement(gpu, nbuf, b):
+    from hydroexa_amd import regrid as RG
+    from hydroexa_amd.solver import context, _stream
+    rng = np.random.default_rng(nbuf * 10 + b)
+    ny, nx = 96, 128
+    tags = rng.random((ny, nx)) < 0.02
+    valid = RG.covers([(0, 0, 64, 96), (64, 32, 48, 48)], ny, nx)
+    vpitch = nx + 4
+    vpad = np.zeros((ny + 4, vpitch), np.uint8)
+    vpad[2:-2, 2:-2] = valid
+    td = torch.as_tensor(tags.astype(np.uint8), device="cuda")
+    vd = torch.as_tensor(vpad, device="cuda")
+    out = torch.empty((ny // b, nx // b), dtype=torch.uint8, device="cuda")
+    L.check(L.load().hx_tag_blocks(context(), td.data_ptr(), nx, vd.data_ptr() + 2 * vpitch + 2,
+                                   vpitch, nx, ny, nbuf, b, out.data_ptr(), _stream()),
+            "hx_tag_blocks")
+    want = RG.coarsen_tags(RG.grow_tags(tags & valid, nbuf) & valid, b)
+    np.testing.assert_array_equal(out.bool().cpu().numpy(), want)
