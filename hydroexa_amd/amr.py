@@ -717,6 +717,47 @@ class AmrHierarchy:
         L.check(rc, "hx_amr_run")
         return hist[:done.value]
 
+    # -- diagnostics and output ----------------------------------------------
+    def level_stats(self, lev: int) -> np.ndarray:
+        """(mass, px, py, KE, min h) over the valid cells of a level: the
+        per-level reduction of AmrMeshState::ComputeMassDiagnostics
+        (src/state/AmrMeshState_Utils.cpp:213-299) summed over every patch
+        (the reference assigns each box's result, keeping its last box)."""
+        lv = self.levels[lev]
+        out = np.zeros(5)
+        out[4] = 1.7976931348623157e308
+        r = np.zeros(5)
+        for b in lv.patches:
+            L.check(L.load().hx_conservation(self.ctx, C.byref(lv.patch(lv.U_new, b)),
+                                             r.ctypes.data_as(C.POINTER(C.c_double)), _stream()),
+                    "hx_conservation")
+            out[:4] += r[:4]
+            out[4] = min(out[4], r[4])
+        return out
+
+    def conservation(self) -> dict:
+        """The reference's totals (sums over levels, so a refined region counts
+        once per level that holds it; src/state/AmrMeshState_Utils.cpp:311-321)
+        and the composite mass and momenta (level 0 after average_down)."""
+        per = [self.level_stats(l) for l in range(self.finest + 1)]
+        tot = np.sum([p[:4] for p in per], axis=0)
+        return {"levels": per, "mass": tot[0], "px": tot[1], "py": tot[2], "KE": tot[3],
+                "min_h": min(p[4] for p in per), "composite_mass": per[0][0],
+                "composite_px": per[0][1], "composite_py": per[0][2]}
+
+    def write_plotfile(self, prefix: str, iteration: int) -> str:
+        """Native AMReX plotfile of every level on its grids (WriteNativePlotfile,
+        src/io/NativePlotFile.cpp:19-97, through hydroexa_amd.plotfile)."""
+        from types import SimpleNamespace
+        from .plotfile import write_native_plotfile
+        views, boxes = [], []
+        for lv in self.levels:
+            views.append(SimpleNamespace(nx=lv.nx, ny=lv.ny, dx=lv.dx, dy=lv.dy,
+                                         U_new=lv.interior(lv.U_new), terrain=lv.interior(lv.terrain)))
+            boxes.append([(i0, j0, i0 + w - 1, j0 + h - 1) for (i0, j0, w, h) in lv.boxes])
+        return write_native_plotfile(prefix, iteration, self.time, views,
+                                     ref_ratio=[2] * self.finest, boxes=boxes)
+
     # -- views -------------------------------------------------------------
     def state(self, lev: int) -> torch.Tensor:
         lv = self.levels[lev]

@@ -250,3 +250,33 @@ def test_tag_blocks_match_host_restatement(gpu, nbuf, b):
             "hx_tag_blocks")
     want = RG.coarsen_tags(RG.grow_tags(tags & valid, nbuf) & valid, b)
     np.testing.assert_array_equal(out.bool().cpu().numpy(), want)
+
+
+def test_hierarchy_conservation_and_plotfile(gpu, tmp_path):
+    from hydroexa_amd.plotfile import read_native_plotfile
+    model, (ny, nx), lo, hi, boxes = CASES["hydro_3lev"]
+    st = field(ny, nx, seed=4)
+    H = AmrHierarchy(model, *st, 1.0, 1.0, lo, hi, max_level=2, boxes=boxes)
+    for l in (1, 0):
+        H._average_down(l)
+    c0 = H.conservation()
+    # per level: the sum over the valid cells (NODATA-free, h > 0 here)
+    for l, lv in enumerate(H.levels):
+        m = lv.interior(lv.mask).bool()
+        hs = lv.interior(lv.U_new)[0][m].double().sum().item() * lv.dx * lv.dy
+        assert abs(c0["levels"][l][0] - hs) <= 1e-13 * hs
+    assert abs(c0["mass"] - sum(p[0] for p in c0["levels"])) <= 1e-15 * c0["mass"]
+    H.run_native(1e30, 0.45, max_steps=3)
+    c1 = H.conservation()
+    assert abs(c1["composite_mass"] - c0["composite_mass"]) <= 1e-13 * c0["composite_mass"]
+    name = H.write_plotfile(str(tmp_path / "plt"), 3)
+    pf = read_native_plotfile(name)
+    assert pf["finest_level"] == 2 and pf["ref_ratio"] == [2, 2]
+    for l, lv in enumerate(H.levels):
+        got = pf["levels"][l]
+        assert sorted(got["boxes"]) == sorted((i0, j0, i0 + w - 1, j0 + h - 1)
+                                             for (i0, j0, w, h) in lv.boxes)
+        U = lv.interior(lv.U_new).double().cpu().numpy()
+        for fab in got["fabs"]:
+            i0, j0, i1, j1 = fab["box"]
+            np.testing.assert_array_equal(fab["data"][:3], U[:, j0:j1 + 1, i0:i1 + 1])
