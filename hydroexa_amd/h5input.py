@@ -156,6 +156,95 @@ def _average_down(fine: torch.Tensor, r: int) -> torch.Tensor:
     return vol * c
 
 
+def _static_layer(path: str):
+    meta = read_metadata(path, "terrain")
+    nx, ny = meta["nx"], meta["ny"]
+    terr = read_components(path, "terrain", 0, 2, 0, 0, nx, ny, NODATA)
+    fluid = read_components(path, "fluid", 0, 3, 0, 0, nx, ny, 0.0)
+    return meta, terr, fluid
+
+
+def level_from_static(meta, terr, fluid, terrain_ref_lev: int, lev: int,
+                      bc_lo: Sequence[int] = (0, 0), bc_hi: Sequence[int] = (0, 0),
+                      device: str = "cuda"):
+    """(h, hu, hv, z, n) of level lev (FP64 device tensors, (ny, nx)) from the
+    static layer of a file read at level terrain_ref_lev:
+    TerrainMapStaticToDynamic + FluidMapStaticToDynamic
+    (src/state/AmrMeshState_AMR.cpp:8-195).  At or below the file's level
+    the terrain is masked-averaged and eta_s = h + z (z at NODATA) and the
+    momenta are averaged; one level finer than the file the terrain is
+    piecewise constant and eta_s and the momenta are interpolated by
+    cell_cons_interp (the library's ghost-fill kernel over the whole level).
+    Either way h = max(0, eta - z) on the level's own bed.  Two or more
+    levels finer than the file (ratios of 4 and up, where AMReX's min/max
+    limiter can bind) are not supported."""
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=device)
+    zs, ns = T(terr[0]), T(terr[1])
+    hs, qxs, qys = T(fluid[0]), T(fluid[1]), T(fluid[2])
+    eta_s = torch.where(ns == NODATA, zs, hs + zs)       # eta = z at NODATA
+    if lev <= terrain_ref_lev:
+        r = 2 ** (terrain_ref_lev - lev)
+        if r == 1:
+            z, n, eta, qx, qy = zs, ns, eta_s, qxs, qys
+        else:
+            t, _ = masked_average_down(torch.stack([zs, ns]).contiguous(), (r, r), meta["dx"],
+                                       meta["dy"], NODATA)
+            z, n = t[0], t[1]
+            qx, qy = _average_down(qxs, r), _average_down(qys, r)
+            eta = _average_down(eta_s, r)
+        x = eta - z
+        h = torch.where(x > 0, x, torch.zeros_like(x))     # max(0, eta - z)
+        return h, qx, qy, z, n
+    if lev != terrain_ref_lev + 1:
+        raise ValueError("levels more than one above the file's resolution are not supported")
+    import ctypes as C
+    from .amr import DenseLevel, _pad_edges, _pc_refine
+    from .solver import Level, context
+    ny, nx = zs.shape
+    # the interpolation reads h + z as the free surface: h = 0 at NODATA
+    # makes that eta_s exactly (z there), the file's h elsewhere
+    hsrc = torch.where(ns == NODATA, torch.zeros_like(hs), hs)
+    src = Level(hsrc, qxs, qys, zs, ns, meta["dx"], meta["dy"], bc_lo, bc_hi)
+    fl = DenseLevel(2 * nx, 2 * ny, meta["dx"] / 2, meta["dy"] / 2, bc_lo, bc_hi, torch.float64)
+    zc = torch.stack([zs, ns]).double().cpu().numpy()
+    fl.terrain.copy_(torch.as_tensor(np.stack([_pad_edges(_pc_refine(zc[k])) for k in range(2)]),
+                                     device=device))
+    fl.set_boxes([(0, 0, 2 * nx, 2 * ny)])
+    cp = src.patch()
+    bp = fl.patch(fl.U_new, (0, 0, 2 * nx, 2 * ny))
+    L.check(L.load().hx_fill_fine_region(
+        context(), C.byref(cp), C.byref(cp), 0.0, 2, 0, 0, C.byref(bp), 1, None, fl.pitch,
+        fl.addr(fl.U_new, 0, 0, 0), fl.addr(fl.U_new, 1, 0, 0), fl.addr(fl.U_new, 2, 0, 0),
+        C.c_void_p(torch.cuda.current_stream().cuda_stream)), "hx_fill_fine_region")
+    U, Tr = fl.interior(fl.U_new), fl.interior(fl.terrain)
+    return U[0].clone(), U[1].clone(), U[2].clone(), Tr[0].clone(), Tr[1].clone()
+
+
+def hierarchy_from_file(path: str, model: str, terrain_ref_lev: int = 0, max_level: int = 1,
+                        blocking_factor: int = 8, bc_lo: Sequence[int] = (0, 0),
+                        bc_hi: Sequence[int] = (0, 0), **kw):
+    """An AmrHierarchy initialised from an input file the way the reference
+    initialises a run: level 0 and every new level from the static layer
+    (MakeNewLevelFromScratch), the geometry contract checked first.  kw goes
+    to AmrHierarchy (regrid_int, tag_thresh, subcycle, ...)."""
+    from .amr import AmrHierarchy
+    meta, terr, fluid = _static_layer(path)
+    geo = check_geometry(meta, terrain_ref_lev, blocking_factor)
+    lv0 = [a.double().cpu().numpy() for a in level_from_static(meta, terr, fluid,
+                                                                 terrain_ref_lev, 0, bc_lo, bc_hi)]
+    cache = {}
+
+    def static(lev):
+        if lev not in cache:
+            cache[lev] = [a.double().cpu().numpy() for a in
+                          level_from_static(meta, terr, fluid, terrain_ref_lev, lev, bc_lo, bc_hi)]
+        return cache[lev]
+
+    H = AmrHierarchy(model, *lv0, geo["level0_dx"], geo["level0_dy"], bc_lo, bc_hi,
+                     max_level=max_level, blocking_factor=blocking_factor, static=static, **kw)
+    return H, geo
+
+
 def level0_from_file(path: str, terrain_ref_lev: int = 0, blocking_factor: int = 8,
                      bc_lo: Sequence[int] = (0, 0), bc_hi: Sequence[int] = (0, 0),
                      dtype: torch.dtype = torch.float64, device: str = "cuda"):
