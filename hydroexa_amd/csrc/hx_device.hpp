@@ -20,7 +20,7 @@
 //       * multiplying by 1/dx when dx is a power of two (an exact scaling).
 //   HX_FAST 1 (fast): the same expressions with divisions that share a
 //     denominator turned into one Newton-refined reciprocal, and FMA
-//     contraction allowed.  Each quotient is within ~2 ulp; the parity tests
+//     contraction within an expression (-ffp-contract=on).  Each quotient is within ~2 ulp; the parity tests
 //     hold this build to 1e-12 relative L-infinity against the oracle.
 
 namespace HX_NS {
@@ -209,6 +209,50 @@ __device__ __forceinline__ RoeCell<R> roe_cell(R hr, R qx, R qy, R z, R n)
     return c;
 }
 
+// sqrt(g h) and h + z of a cell: stored by the exact build, recomputed at
+// each use by the fast build (one multiply or add instead of two live
+// registers per carried cell).
+template <typename R>
+__device__ __forceinline__ R cell_cg(const RoeCell<R>& c)
+{
+#if HX_FAST
+    return SqrtG<R>::v * c.sq;
+#else
+    return c.cg;
+#endif
+}
+template <typename R>
+__device__ __forceinline__ R cell_ed(const RoeCell<R>& c)
+{
+#if HX_FAST
+    return c.hd + c.z;
+#else
+    return c.ed;
+#endif
+}
+
+// Wavefront votes over the active lanes, straight from the lane mask of the
+// comparison (no round trip through a VGPR).
+__device__ __forceinline__ bool wave_all(bool p)
+{
+    return __builtin_amdgcn_ballot_w64(p) == __builtin_amdgcn_read_exec();
+}
+__device__ __forceinline__ bool wave_any(bool p)
+{
+    return __builtin_amdgcn_ballot_w64(p) != 0ull;
+}
+
+// A face before its source split: the central-minus-upwind flux in the
+// face-normal frame and what the split needs (the bed/friction/positivity
+// share b0 of the first acoustic wave, b2 of the third, their speeds, the
+// Roe tangential velocity).
+template <typename R>
+struct RoeCore {
+    R F0, F1, F2;
+    R b0, b2, l0, l2, vt;
+    bool src;   // wave-uniform: some lane has a non-zero source share
+};
+
 template <typename R>
 struct RoeFace {
     R F0, F1, F2, Sm0, Sm1, Sm2, Sp0, Sp1, Sp2;
@@ -218,7 +262,8 @@ struct RoeFace {
 // DIR 0: x face, normal (1,0); DIR 1: y face, normal (0,1).  A is the
 // low-side cell, B the high-side cell, ds the face-normal cell size.
 template <typename R, int DIR>
-__device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCell<R>& B, R ds)
+__device__ __forceinline__ void roe_core(const RoeCell<R>& A, const RoeCell<R>& B, R ds,
+                                         RoeCore<R>& o)
 {
     using k = Kc<R>;
     const R hl = A.hd, hr = B.hd;
@@ -247,10 +292,11 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
     const R l1 = ut;
     R l2 = ut + ct;
     {
-        R el = unl - A.cg, er = unr - B.cg;
+        const R cga = cell_cg(A), cgb = cell_cg(B);
+        R el = unl - cga, er = unr - cgb;
         if (el < R(0) && er > R(0)) l0 = Div<R>(er - el)(el * (er - l0));
-        el = unl + A.cg;
-        er = unr + B.cg;
+        el = unl + cga;
+        er = unr + cgb;
         if (el < R(0) && er > R(0)) l2 = Div<R>(er - el)(er * (l2 - el));
     }
 
@@ -276,14 +322,15 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
     // on a lane, tau is likewise skipped.
     const R dz = B.z - A.z;
     const R nt = R(0.5) * (A.n + B.n);
-    const bool flat = __all((dz == R(0)) && (nt == R(0)));
+    const bool flat = wave_all((dz == R(0)) && (nt == R(0)));
     R b0 = R(0);
     if (!flat) {
-        const R dd = B.ed - A.ed;
+        const R eda = cell_ed(A), edb = cell_ed(B);
+        const R dd = edb - eda;
         const R thrust_a = -k::g * ht * dz;
         R dzt;
-        if ((dz >= R(0)) && (A.ed < B.z)) dzt = hl;
-        else if ((dz < R(0)) && (B.ed < A.z)) dzt = hr;
+        if ((dz >= R(0)) && (eda < B.z)) dzt = hl;
+        else if ((dz < R(0)) && (edb < A.z)) dzt = hr;
         else dzt = dz;
         const R hup = (dz >= R(0)) ? hl : hr;
         const R thrust_b = -k::g * (hup - R(0.5) * fabs(dzt)) * dzt;
@@ -357,29 +404,44 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
     F2 -= w1 * ct;
     F0 -= w2; F1 -= w2 * l2; F2 -= w2 * vt;
 
-    RoeFace<R> o;
-    // Upwind source split (Roe.H:281-289), without branches: each acoustic
-    // wave's share goes to S- when its speed is positive and to S+ when
-    // negative; the contact wave carries none.  Adding a selected 0 in place
-    // of a skipped term leaves every sum as the reference's.
-    o.src = __any(b0 != R(0));
-    R Sm0 = R(0), Sm1 = R(0), Sm2 = R(0), Sp0 = R(0), Sp1 = R(0), Sp2 = R(0);
-    if (o.src) {
-        const R k0b = b0 * l0, k0c = b0 * vt, k2b = b2 * l2, k2c = b2 * vt;
-        const bool p0 = l0 > R(0), n0 = l0 < R(0), p2 = l2 > R(0), n2 = l2 < R(0);
-        Sm0 = (p0 ? b0 : R(0)) + (p2 ? b2 : R(0));
-        Sm1 = (p0 ? k0b : R(0)) + (p2 ? k2b : R(0));
-        Sm2 = (p0 ? k0c : R(0)) + (p2 ? k2c : R(0));
-        Sp0 = (n0 ? b0 : R(0)) + (n2 ? b2 : R(0));
-        Sp1 = (n0 ? k0b : R(0)) + (n2 ? k2b : R(0));
-        Sp2 = (n0 ? k0c : R(0)) + (n2 ? k2c : R(0));
-    }
+    o.F0 = F0; o.F1 = F1; o.F2 = F2;
+    o.b0 = b0; o.b2 = b2; o.l0 = l0; o.l2 = l2; o.vt = vt;
+    o.src = wave_any(b0 != R(0));
+}
 
-    o.F0 = F0; o.Sm0 = Sm0; o.Sp0 = Sp0;
+// Upwind source split (Roe.H:281-289), without branches: each acoustic
+// wave's share goes to S- when its speed is positive and to S+ when
+// negative; the contact wave carries none.  Adding a selected 0 in place of
+// a skipped term leaves every sum as the reference's.  Face-normal frame.
+template <typename R>
+__device__ __forceinline__ void src_split(const RoeCore<R>& f, R& Sm0, R& Sm1, R& Sm2, R& Sp0,
+                                          R& Sp1, R& Sp2)
+{
+    const R k0b = f.b0 * f.l0, k0c = f.b0 * f.vt, k2b = f.b2 * f.l2, k2c = f.b2 * f.vt;
+    const bool p0 = f.l0 > R(0), n0 = f.l0 < R(0), p2 = f.l2 > R(0), n2 = f.l2 < R(0);
+    Sm0 = (p0 ? f.b0 : R(0)) + (p2 ? f.b2 : R(0));
+    Sm1 = (p0 ? k0b : R(0)) + (p2 ? k2b : R(0));
+    Sm2 = (p0 ? k0c : R(0)) + (p2 ? k2c : R(0));
+    Sp0 = (n0 ? f.b0 : R(0)) + (n2 ? f.b2 : R(0));
+    Sp1 = (n0 ? k0b : R(0)) + (n2 ? k2b : R(0));
+    Sp2 = (n0 ? k0c : R(0)) + (n2 ? k2c : R(0));
+}
+
+// The face with its split sources, rotated back to the x-y frame.
+template <typename R, int DIR>
+__device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCell<R>& B, R ds)
+{
+    RoeCore<R> f;
+    roe_core<R, DIR>(A, B, ds, f);
+    RoeFace<R> o;
+    o.src = f.src;
+    R Sm0 = R(0), Sm1 = R(0), Sm2 = R(0), Sp0 = R(0), Sp1 = R(0), Sp2 = R(0);
+    if (f.src) src_split(f, Sm0, Sm1, Sm2, Sp0, Sp1, Sp2);
+    o.F0 = f.F0; o.Sm0 = Sm0; o.Sp0 = Sp0;
     if (DIR == 0) {
-        o.F1 = F1; o.F2 = F2; o.Sm1 = Sm1; o.Sm2 = Sm2; o.Sp1 = Sp1; o.Sp2 = Sp2;
+        o.F1 = f.F1; o.F2 = f.F2; o.Sm1 = Sm1; o.Sm2 = Sm2; o.Sp1 = Sp1; o.Sp2 = Sp2;
     } else {   // rotate back with n = (0,1): (a, b) -> (-b, a)
-        o.F1 = -F2; o.F2 = F1; o.Sm1 = -Sm2; o.Sm2 = Sm1; o.Sp1 = -Sp2; o.Sp2 = Sp1;
+        o.F1 = -f.F2; o.F2 = f.F1; o.Sm1 = -Sm2; o.Sm2 = Sm1; o.Sp1 = -Sp2; o.Sp2 = Sp1;
     }
     return o;
 }

@@ -3,6 +3,8 @@
 // geometry and the fused-advance launcher (defined in hx_kernels.hip).
 #pragma once
 
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -72,6 +74,10 @@ inline void fill_geometry(StepArgs<R>& a, int solver, int rows, double dx, doubl
     a.rdy = (R)(1.0 / dy);
     a.dx_pow2 = is_pow2(dx) && is_pow2(dy) ? 1 : 0;
     const int cols = (solver == HX_SOLVER_ROE) ? kRoeCols : kHydroCols;
+    // a strip is addressed as base + 32-bit byte offsets: rows + 4 rows of
+    // either pitch must span less than 4 GiB (check_patch bounds the pitch)
+    const int64_t span = std::max<int64_t>(a.p.pitch, a.opitch) * (int64_t)sizeof(R);
+    rows = (int)std::max<int64_t>(1, std::min<int64_t>(rows, ((int64_t)1 << 32) / span - 4));
     a.rows = rows;
     a.nsx = (a.p.nx + cols - 1) / cols;
     const int nsy = (a.p.ny + rows - 1) / rows;

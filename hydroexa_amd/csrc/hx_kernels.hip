@@ -508,8 +508,9 @@ using namespace hx::host;
 // allocator leaves room for 3 waves per SIMD.
 static int default_rows(int solver, int dtype)
 {
-    if (dtype == HX_F32) return 16;
-    return (solver == HX_SOLVER_ROE) ? 32 : 16;
+    (void)solver;
+    (void)dtype;
+    return 16;
 }
 
 static int default_occ(int solver, int dtype)
@@ -608,6 +609,8 @@ int hx::host::check_patch(const hx_patch* in, int need_ghost)
     }
     const int64_t gx = (in->bc_lo[0] == HX_BC_GHOST ? in->ng : 0) + (in->bc_hi[0] == HX_BC_GHOST ? in->ng : 0);
     if (in->pitch < (int64_t)in->nx + gx) return HX_E_BADARG;
+    // the advance addresses 5 rows of a strip with 32-bit byte offsets
+    if (in->pitch > ((int64_t)1 << 26)) return HX_E_BADARG;
     return HX_OK;
 }
 
@@ -971,6 +974,9 @@ static int ens_upload(hx_ensemble* e)
     for (int t = 0; t < e->nscen; ++t) {
         const int s = order[t];
         const hx_patch* in = &e->patches[s];
+        // strips are addressed with 32-bit byte offsets over rows + 4 rows
+        if ((int64_t)(e->rows + 4) * in->pitch * (int64_t)sizeof(R) >= ((int64_t)1 << 32))
+            return HX_E_BADARG;
         ScenDesc<R>& x = d[t];
         std::memset(&x, 0, sizeof(x));
         x.p = view_of<R>(in);
