@@ -38,15 +38,15 @@ template <typename R> __device__ __forceinline__ R smax(R a, R b) { return (a < 
 template <typename R> __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
 
 #if HX_FAST
-// Reciprocal from the hardware estimate and Newton steps r <- r + r(1 - b r)
-// (two for FP64, one for FP32).  Callers only pass finite, non-zero b.
+// Reciprocal from the hardware estimate r (relative error ~2^-23) and one
+// third-order step: 1/b = r / (1 - e) = r (1 + e + e^2 + ...) with
+// e = 1 - b r, truncated after e^2 (the e^3 term is below 2^-66).  FP32 takes
+// one Newton step r + r e.  Callers only pass finite, non-zero b.
 __device__ __forceinline__ double rcp(double b)
 {
-    double r = __builtin_amdgcn_rcp(b);
-    double e = fma(-b, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-b, r, 1.0);
-    return fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(b);
+    const double e = fma(-b, r, 1.0);
+    return fma(r, fma(e, e, e), r);
 }
 __device__ __forceinline__ float rcp(float b)
 {
@@ -61,21 +61,20 @@ struct Div {
     __device__ __forceinline__ R operator()(R a) const { return a * r; }
 };
 
-// sqrt(x) and 1/sqrt(x) together for finite x > 0: hardware estimate, then
-// Goldschmidt steps g <- g + g e, h <- h + h e with e = 1/2 - g h (two for
-// FP64, one for FP32), g -> sqrt(x), 2h -> 1/sqrt(x).
+// sqrt(x) and 1/sqrt(x) together for finite x > 0.  FP64: the hardware
+// estimate y (relative error ~2^-23) and one third-order step,
+// 1/sqrt(x) = y (1 - d)^(-1/2) = y (1 + d/2 + 3 d^2/8 + ...) with
+// d = 1 - x y^2 (the d^3 term is below 2^-64), and sqrt(x) = (x y) times the
+// same factor.  FP32: one Goldschmidt step g <- g + g e, h <- h + h e with
+// e = 1/2 - g h, g -> sqrt(x), 2h -> 1/sqrt(x).
 __device__ __forceinline__ void sqrt_rsq(double x, double& sq, double& rs)
 {
     const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = 0.5 * y;
-    double e = fma(-g, h, 0.5);
-    g = fma(g, e, g);
-    h = fma(h, e, h);
-    e = fma(-g, h, 0.5);
-    g = fma(g, e, g);
-    h = fma(h, e, h);
-    sq = g;
-    rs = 2.0 * h;
+    const double t = x * y;
+    const double d = fma(-t, y, 1.0);
+    const double c = d * fma(d, 0.375, 0.5);
+    rs = fma(y, c, y);
+    sq = fma(t, c, t);
 }
 __device__ __forceinline__ void sqrt_rsq(float x, float& sq, float& rs)
 {
