@@ -408,6 +408,91 @@ __device__ __forceinline__ void roe_core(const RoeCell<R>& A, const RoeCell<R>& 
     o.src = wave_any(b0 != R(0));
 }
 
+#if HX_FAST
+// The common face in one pass (fast build).  On a lane where the entropy fix
+// does not apply, the bed is flat and smooth and the positivity guard does
+// not fire, the face is the central flux minus the upwind wave sum with no
+// source share:
+//   F0 = (S0 - q0 - q2) / 2,  F1 = (S1 - q0 l0 - q2 l2) / 2,
+//   F2 = (S2 - vt (q0 + q2) - |ut| (dhv - vt dh)) / 2
+// with S the summed physical fluxes and qk = |lk| ak (the contact wave's
+// w1 ct = |ut| a1 ct / 2 with a1 ct = dhv - vt dh, so 1/ct drops out).
+// Returns whether that holds on this lane; F is computed unless no lane of
+// the wavefront has a flat, smooth bed (then every lane returns false).  A
+// zero-sum roughness pair with non-zero members only takes the general path.
+template <typename R, int DIR>
+__device__ __forceinline__ bool roe_quick(const RoeCell<R>& A, const RoeCell<R>& B, R& F0, R& F1,
+                                          R& F2)
+{
+    using k = Kc<R>;
+    const bool rough = (B.z != A.z) || (A.n + B.n != R(0));
+    if (wave_all(rough)) return false;
+    const R hl = A.hd, hr = B.hd;
+    const R unl = (DIR == 0) ? A.u : A.v;
+    const R utl = (DIR == 0) ? A.v : -A.u;
+    const R unr = (DIR == 0) ? B.u : B.v;
+    const R utr = (DIR == 0) ? B.v : -B.u;
+
+    // g max((hl + hr)/2, TOLDRY): scaling by g and halving commute with the
+    // max and with rounding
+    const R ght = fmax((hl + hr) * (R(0.5) * k::g), k::g * k::toldry);
+    R ct, ict;
+    sqrt_rsq(ght, ct, ict);
+    const R qh = R(0.5) * ict;
+    const R rden = qh * fma(-k::tol12, qh, R(1));
+    const R isq = rcp(A.sq + B.sq + k::tol12);
+    const R ut = (unl * A.sq + unr * B.sq) * isq;
+    const R vt = (utl * A.sq + utr * B.sq) * isq;
+    const R l0 = ut - ct, l2 = ut + ct;
+    const R cga = cell_cg(A), cgb = cell_cg(B);
+    const bool fix0 = (unl - cga < R(0)) && (unr - cgb > R(0));
+    const bool fix2 = (unl + cga < R(0)) && (unr + cgb > R(0));
+    const R mL = hl * unl, mR = hr * unr;
+    const R dh = hr - hl;
+    const R dhu = mR - mL;
+    const R a0 = (l2 * dh - dhu) * rden;
+    const R a2 = (dhu - l0 * dh) * rden;
+    const R h1s = hl + a0, h3s = hr - a2;
+    const bool guard = ((h1s < -k::toldry) && (h3s > k::toldry)) ||
+                       ((h1s > k::toldry) && (h3s < -k::toldry));
+    const R q0 = fabs(l0) * a0, q2 = fabs(l2) * a2;
+    const R q02 = q0 + q2;
+    const R dhv = hr * utr - hl * utl;
+    const R S1 = mL * unl + mR * unr + (R(0.5) * k::g) * (hl * hl + hr * hr);
+    const R S2 = mL * utl + mR * utr;
+    F0 = R(0.5) * ((mL + mR) - q02);
+    F1 = R(0.5) * (S1 - q0 * l0 - q2 * l2);
+    F2 = R(0.5) * (S2 - vt * q02 - fabs(ut) * (dhv - vt * dh));
+    return !(fix0 || fix2 || rough || guard);
+}
+
+// The face as the fast build's advance evaluates it, lane by lane: the
+// one-pass form where it applies, roe_core elsewhere -- so the value of a
+// face does not depend on how faces are grouped into wavefronts (the flux
+// registers re-evaluate faces in another grouping).  Returns true when every
+// lane took the one-pass form (o.F holds the fluxes, no source share);
+// otherwise o is the merged core with o.src set as roe_core sets it.
+template <typename R, int DIR>
+__device__ __forceinline__ bool roe_core_adv(const RoeCell<R>& A, const RoeCell<R>& B, R ds,
+                                             RoeCore<R>& o)
+{
+    const bool ok = roe_quick<R, DIR>(A, B, o.F0, o.F1, o.F2);
+    if (wave_all(ok)) return true;
+    RoeCore<R> g;
+    roe_core<R, DIR>(A, B, ds, g);
+    o.F0 = ok ? o.F0 : g.F0;
+    o.F1 = ok ? o.F1 : g.F1;
+    o.F2 = ok ? o.F2 : g.F2;
+    o.b0 = ok ? R(0) : g.b0;
+    o.b2 = ok ? R(0) : g.b2;
+    o.l0 = g.l0;
+    o.l2 = g.l2;
+    o.vt = g.vt;
+    o.src = wave_any(o.b0 != R(0));
+    return false;
+}
+#endif
+
 // Upwind source split (Roe.H:281-289), without branches: each acoustic
 // wave's share goes to S- when its speed is positive and to S+ when
 // negative; the contact wave carries none.  Adding a selected 0 in place of
@@ -443,6 +528,30 @@ __device__ __forceinline__ RoeFace<R> roe_face(const RoeCell<R>& A, const RoeCel
         o.F1 = -f.F2; o.F2 = f.F1; o.Sm1 = -Sm2; o.Sm2 = Sm1; o.Sp1 = -Sp2; o.Sp2 = Sp1;
     }
     return o;
+}
+
+// The face with its split sources as the advance evaluates it (the one-pass
+// form where it applies in the fast build; roe_face in the exact build).
+template <typename R, int DIR>
+__device__ __forceinline__ RoeFace<R> roe_face_adv(const RoeCell<R>& A, const RoeCell<R>& B, R ds)
+{
+#if HX_FAST
+    RoeCore<R> f;
+    const bool quick = roe_core_adv<R, DIR>(A, B, ds, f);
+    RoeFace<R> o;
+    o.src = !quick && f.src;
+    R Sm0 = R(0), Sm1 = R(0), Sm2 = R(0), Sp0 = R(0), Sp1 = R(0), Sp2 = R(0);
+    if (o.src) src_split(f, Sm0, Sm1, Sm2, Sp0, Sp1, Sp2);
+    o.F0 = f.F0; o.Sm0 = Sm0; o.Sp0 = Sp0;
+    if (DIR == 0) {
+        o.F1 = f.F1; o.F2 = f.F2; o.Sm1 = Sm1; o.Sm2 = Sm2; o.Sp1 = Sp1; o.Sp2 = Sp2;
+    } else {
+        o.F1 = -f.F2; o.F2 = f.F1; o.Sm1 = -Sm2; o.Sm2 = Sm1; o.Sp1 = -Sp2; o.Sp2 = Sp1;
+    }
+    return o;
+#else
+    return roe_face<R, DIR>(A, B, ds);
+#endif
 }
 
 // ---------------------------------------------------------------------------

@@ -21,11 +21,42 @@ struct hx_ctx {
     uint32_t* pflags;    // prune flags
     int32_t* pboxes;
     int pcap;
-    int rows[2][2];      // [solver][dtype]
-    int occ[2][2];       // [solver][dtype] minimum waves per SIMD requested
+    int rows[2][2];      // [solver][dtype]; 0: the default of the arithmetic build
+    int occ[2][2];       // [solver][dtype] minimum waves per SIMD requested; 0: default
     int arith;           // HX_ARITH_EXACT or HX_ARITH_FAST
     double* cons;        // conservation partials (kConsBlocks x 9) + result (5)
 };
+
+namespace hx {
+namespace host {
+
+// Launch shape of the fused advance for (solver, dtype) under the context's
+// arithmetic build: the value set by hx_set_tuning, or the measured default
+// (one MI355X, 4096^2 levels: 16 rows and 3 waves per SIMD for FP64; FP32
+// plain Roe in the fast build marches 32 rows).
+inline int default_rows(int solver, int dtype, int arith)
+{
+    return (dtype == HX_F32 && solver == HX_SOLVER_ROE && arith == HX_ARITH_FAST) ? 32 : 16;
+}
+inline int default_occ(int solver, int dtype)
+{
+    (void)solver;
+    return (dtype == HX_F64) ? 3 : 1;
+}
+inline int eff_rows(const hx_ctx* c, int solver, int dtype)
+{
+    const int r = c->rows[solver][dtype];
+    return r ? r : default_rows(solver, dtype, c->arith);
+}
+inline int eff_occ(const hx_ctx* c, int solver, int dtype)
+{
+    const int o = c->occ[solver][dtype];
+    return o ? o : default_occ(solver, dtype);
+}
+
+}  // namespace host
+}  // namespace hx
+
 
 #define HX_TRY(x)                                  \
     do {                                           \

@@ -506,19 +506,6 @@ using namespace hx::host;
 // (4096^2 plain Roe and 2048^2 hydrostatic workloads): short strips keep
 // enough wavefronts in flight; the FP64 kernels run best when the register
 // allocator leaves room for 3 waves per SIMD.
-static int default_rows(int solver, int dtype)
-{
-    (void)solver;
-    (void)dtype;
-    return 16;
-}
-
-static int default_occ(int solver, int dtype)
-{
-    (void)solver;
-    return (dtype == HX_F64) ? 3 : 1;
-}
-
 extern "C" int hx_abi_version(void) { return HX_ABI_VERSION; }
 
 extern "C" const char* hx_status_string(int s)
@@ -552,8 +539,8 @@ extern "C" int hx_ctx_create(hx_ctx** out, int device)
     }
     for (int s = 0; s < 2; ++s)
         for (int d = 0; d < 2; ++d) {
-            c->rows[s][d] = default_rows(s, d);
-            c->occ[s][d] = default_occ(s, d);
+            c->rows[s][d] = 0;
+            c->occ[s][d] = 0;
         }
     c->arith = HX_ARITH_EXACT;
     hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, 0, c->slots, 4, c->err);
@@ -583,8 +570,8 @@ extern "C" int hx_set_tuning(hx_ctx* c, int solver, int dtype, int rows, int occ
     bool known = (occupancy == 0);
     for (int v = 0; v < kOccVariants; ++v) known = known || (occupancy == kOcc[v]);
     if (!known) return HX_E_BADARG;
-    c->rows[solver][dtype] = rows ? rows : default_rows(solver, dtype);
-    c->occ[solver][dtype] = occupancy ? occupancy : default_occ(solver, dtype);
+    c->rows[solver][dtype] = rows;
+    c->occ[solver][dtype] = occupancy;
     return HX_OK;
 }
 
@@ -617,7 +604,7 @@ int hx::host::check_patch(const hx_patch* in, int need_ghost)
 template <typename R>
 static int launch(const hx_ctx* c, int solver, int dtype, const StepArgs<R>& a, hipStream_t st)
 {
-    const int occ = c->occ[solver][dtype];
+    const int occ = eff_occ(c, solver, dtype);
     const int rc = (c->arith == HX_ARITH_FAST) ? fast::launch_advance<R>(solver, occ, a, st)
                                                : exact::launch_advance<R>(solver, occ, a, st);
     return rc ? HX_E_HIP : HX_OK;
@@ -634,7 +621,7 @@ static int advance_t(hx_ctx* c, int solver, const hx_patch* in, double dt, void*
     a.hvo = (R*)hvo;
     a.opitch = opitch;
     a.dt_host = dt;
-    fill_geometry(a, solver, c->rows[solver][in->dtype], in->dx, in->dy);
+    fill_geometry(a, solver, eff_rows(c, solver, in->dtype), in->dx, in->dy);
     return launch(c, solver, in->dtype, a, st);
 }
 
@@ -672,7 +659,7 @@ int hx::host::launch_step_f64(const hx_ctx* c, int solver, const StepArgs<double
 
 int hx::host::launch_adjpost_f64(const hx_ctx* c, const StepArgs<double>& a, hipStream_t st)
 {
-    const int occ = c->occ[HX_SOLVER_HYDROSTATIC_ROE][HX_F64];
+    const int occ = eff_occ(c, HX_SOLVER_HYDROSTATIC_ROE, HX_F64);
     const int rc = (c->arith == HX_ARITH_FAST) ? fast::launch_adjpost(occ, a, st)
                                                : exact::launch_adjpost(occ, a, st);
     return rc ? HX_E_HIP : HX_OK;
@@ -874,7 +861,7 @@ static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
     a.err = c->err;
     a.dt_hist = c->hist;
     a.vel_hist = c->hist + c->hist_cap;
-    fill_geometry(a, solver, c->rows[solver][in->dtype], in->dx, in->dy);
+    fill_geometry(a, solver, eff_rows(c, solver, in->dtype), in->dx, in->dy);
     // reset slots and the error flag, then reduce the initial state's CFL
     hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots, 3, c->err);
     int rc = launch_cfl<R>(in, c->slots, st);
@@ -1047,8 +1034,8 @@ extern "C" int hx_ensemble_create(hx_ctx* c, int solver, int dtype, int nscen, c
     e->solver = solver;
     e->dtype = dtype;
     e->nscen = nscen;
-    e->rows = c->rows[solver][dtype];
-    e->occ = c->occ[solver][dtype];
+    e->rows = eff_rows(c, solver, dtype);
+    e->occ = eff_occ(c, solver, dtype);
     e->persistent = 0;
     e->patches.assign(scen, scen + nscen);
     e->alt.assign(alt, alt + 3 * nscen);
@@ -1090,7 +1077,7 @@ extern "C" int hx_ensemble_set_tuning(hx_ensemble* e, int rows, int occupancy, i
     if (!known || rows < 0 || rows > 1 << 20 || (persistent != 0 && persistent != 1))
         return HX_E_BADARG;
     HX_TRY(hipSetDevice(e->ctx->device));
-    e->rows = rows ? rows : default_rows(e->solver, e->dtype);
+    e->rows = rows ? rows : default_rows(e->solver, e->dtype, e->ctx->arith);
     e->occ = occupancy ? occupancy : default_occ(e->solver, e->dtype);
     e->persistent = persistent;
     return ens_upload_any(e);
@@ -1414,14 +1401,14 @@ static int two_level_t(hx_ctx* c, int solver, const hx_patch* crse, void* const*
     ac.dt_dev = dt;
     ac.sk_i0 = ci0 - 1; ac.sk_i1 = ci0 + ncx + 1;
     ac.sk_j0 = cj0 - 1; ac.sk_j1 = cj0 + ncy + 1;
-    fill_geometry(ac, solver, c->rows[solver][crse->dtype], crse->dx, crse->dy);
+    fill_geometry(ac, solver, eff_rows(c, solver, crse->dtype), crse->dx, crse->dy);
     StepArgs<R> af{};
     af.p = view_of<R>(fine);
     af.opitch = fine->pitch;
     af.cfl = cfl;
     af.err = err;
     af.dt_dev = dt + 1;
-    fill_geometry(af, solver, c->rows[solver][fine->dtype], fine->dx, fine->dy);
+    fill_geometry(af, solver, eff_rows(c, solver, fine->dtype), fine->dx, fine->dy);
     FaceReq qc{};
     qc.nfaces = 2 * (ncx + ncy); qc.pair = 1; qc.accumulate = 0; qc.box_fine = 0;
     qc.ci0 = ci0; qc.cj0 = cj0; qc.ncx = ncx; qc.ncy = ncy;
@@ -1762,7 +1749,7 @@ int amr_advance(AmrRun& r, int l, double t, double dt)
         a.ho = (R*)out.h; a.huo = (R*)out.hu; a.hvo = (R*)out.hv;
         a.opitch = out.pitch;
         a.dt_host = dt;
-        fill_geometry(a, r.solver, r.c->rows[r.solver][r.dtype], in.dx, in.dy);
+        fill_geometry(a, r.solver, eff_rows(r.c, r.solver, r.dtype), in.dx, in.dy);
         if ((rc = launch(r.c, r.solver, r.dtype, a, r.st))) return rc;
     }
     return HX_OK;

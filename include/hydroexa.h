@@ -180,9 +180,10 @@ int hx_run_level(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
 
 /*
  * Launch-shape control for the fused advance (the autotuner's knobs): rows
- * marched per wavefront (0 = default) and the minimum waves per SIMD the
- * register allocator must leave room for (0 = default; 1, 3 or 4).  No
- * reference counterpart (AMReX picks launch shapes itself).
+ * marched per wavefront and the minimum waves per SIMD the register
+ * allocator must leave room for (1, 3 or 4); 0 selects the measured default
+ * of the arithmetic build in use when the advance is launched.  No reference
+ * counterpart (AMReX picks launch shapes itself).
  */
 int hx_set_tuning(hx_ctx *ctx, int solver, int dtype, int rows, int occupancy);
 
