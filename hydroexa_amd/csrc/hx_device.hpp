@@ -758,8 +758,22 @@ __device__ __forceinline__ unsigned long long dbits(double x)
     return (unsigned long long)__double_as_longlong(x);
 }
 
-// Reduce a wavefront's accumulator and fold one block into the slot with
-// one atomic per quantity.  Every thread of the block must call this.
+// Fold one partial (max a, max velocity, flags) into a slot.  The slot only
+// grows while partials fold into it, so it is read at device scope first and
+// the atomics that cannot change it are skipped: same-address device-scope
+// atomics serialize, and after the first partials nearly all skip.
+__device__ __forceinline__ void slot_fold(CflSlot* slot, double a, double v, unsigned f)
+{
+    const unsigned long long ca = __hip_atomic_load(&slot->amax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long cv = __hip_atomic_load(&slot->vmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned cf = __hip_atomic_load(&slot->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (dbits(a) > ca) atomicMax(&slot->amax, dbits(a));
+    if (dbits(v) > cv) atomicMax(&slot->vmax, dbits(v));
+    if ((cf | f) != cf) atomicOr(&slot->flags, f);
+}
+
+// Reduce a wavefront's accumulator and fold one block into the slot.  Every
+// thread of the block must call this.
 template <typename R>
 __device__ __forceinline__ void cfl_commit(CflAcc<R> acc, CflSlot* slot)
 {
@@ -784,11 +798,7 @@ __device__ __forceinline__ void cfl_commit(CflAcc<R> acc, CflSlot* slot)
             v = (v < sv[q]) ? sv[q] : v;
             f |= sf[q];
         }
-        if (f) {
-            atomicMax(&slot->amax, dbits(a));
-            atomicMax(&slot->vmax, dbits(v));
-            atomicOr(&slot->flags, f);
-        }
+        if (f) slot_fold(slot, a, v, f);
     }
 }
 

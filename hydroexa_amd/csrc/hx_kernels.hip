@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, Patch
                                                           const uint8_t* __restrict__ skip,
                                                           int64_t spitch)
 {
-    if (err && *(volatile const int*)err) return;
+    if (err && *err) return;
     const int W = nxf + 4, H = nyf + 4;
     const int n = whole ? W * H : W * H - nxf * nyf;
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void reflux_kernel(int ci0, int cj0, int nc
                                                         R* hv, const int* err, int orient,
                                                         int cnx, int cny)
 {
-    if (err && *(volatile const int*)err) return;
+    if (err && *err) return;
     const int q = blockIdx.x * kBlock + threadIdx.x;
     const int nq = 2 * ncy + 2 * ncx;
     if (q >= nq) return;
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void avgdown_state_kernel(int ci0, int cj0,
                                                                int64_t cpitch, R* h, R* hu, R* hv,
                                                                const int* err)
 {
-    if (err && *(volatile const int*)err) return;
+    if (err && *err) return;
     const int q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= ncx * ncy) return;
     const int ic = q % ncx, jc = q / ncx;
