@@ -118,6 +118,40 @@ def traffic_from_profile(name):
     return e.get("bytes_per_launch") if isinstance(e, dict) else None
 
 
+def stream_ceiling(cells, reps=20):
+    """Streaming ceiling of the advance's access pattern: the library's
+    elementwise probe that reads five FP64 planes of `cells` elements and
+    writes three (hx_stream_probe), timed with HIP events on the stream it
+    runs on.  Returns ms per pass and GB/s at 64 B per element."""
+    import ctypes as C
+    import torch
+    from hydroexa_amd import _lib as L
+
+    lib = L.load()
+    ctx = C.c_void_p()
+    L.check(lib.hx_ctx_create(C.byref(ctx), torch.cuda.current_device()), "hx_ctx_create")
+    try:
+        planes = [torch.full((cells,), 0.5 + k, dtype=torch.float64, device="cuda") for k in range(8)]
+        ins = (C.c_void_p * 5)(*[t.data_ptr() for t in planes[:5]])
+        outs = (C.c_void_p * 3)(*[t.data_ptr() for t in planes[5:]])
+        st = torch.cuda.current_stream()
+        sp = C.c_void_p(st.cuda_stream)
+        for _ in range(3):
+            L.check(lib.hx_stream_probe(ctx, cells, ins, outs, sp), "hx_stream_probe")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            lib.hx_stream_probe(ctx, cells, ins, outs, sp)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del planes
+    finally:
+        lib.hx_ctx_destroy(ctx)
+    return {"kernel": "stream5r3w (read 5 FP64 planes, write 3, elementwise)", "cells": cells,
+            "ms": ms, "GBps": 64.0 * cells / (ms / 1e3) / 1e9}
+
+
 def time_conservation(w, dtype, reps=20):
     """Fused conservation reduction (mass, momenta, KE, min h) of the
     workload's level, wall time per call including the host sync of the
@@ -464,6 +498,10 @@ def main():
         "wall_s": wall,
         "final_dt": float(dth[-1]) if len(dth) else None,
     }
+    if args.dtype == "f64":
+        ceil = stream_ceiling(w.cells)
+        ceil["advance_over_ceiling"] = achieved / ceil["GBps"]
+        out["roofline"]["stream_ceiling"] = ceil
     out["diagnostics"] = time_conservation(w, args.dtype)
     if args.extra:
         extra = {}

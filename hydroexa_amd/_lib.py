@@ -43,7 +43,7 @@ EXPORTS = (
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
     "hx_conservation", "hx_face_fluxes", "hx_fill_fine_ghosts", "hx_reflux",
     "hx_average_down_state", "hx_two_level_run", "hx_fill_fine_region", "hx_reflux_side",
-    "hx_boxes_dt", "hx_amr_run", "hx_tag_blocks",
+    "hx_boxes_dt", "hx_amr_run", "hx_tag_blocks", "hx_stream_probe",
 )
 
 
@@ -119,6 +119,7 @@ def load() -> C.CDLL:
                                     C.POINTER(C.c_int64), C.POINTER(vp)]
     L.hx_adjoint_destroy.argtypes = [vp]
     L.hx_conservation.argtypes = [vp, pp, dp, vp]
+    L.hx_stream_probe.argtypes = [vp, C.c_int64, C.POINTER(vp), C.POINTER(vp), vp]
     L.hx_face_fluxes.argtypes = [vp, C.c_int, pp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                  C.c_int, vp, vp]
     L.hx_fill_fine_region.argtypes = [vp, pp, pp, C.c_double, C.c_int, C.c_int, C.c_int, pp,

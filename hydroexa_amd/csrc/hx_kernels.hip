@@ -850,6 +850,46 @@ extern "C" int hx_conservation(hx_ctx* c, const hx_patch* in, double* out, void*
     return HX_OK;
 }
 
+// Streaming ceiling of the advance's access pattern: an elementwise pass that
+// reads five FP64 planes and writes three (x = a + s, y = b s, z = c - s with
+// s = d + e), as coalesced and as simple as a kernel with this traffic can
+// be.  Its time bounds from below what any advance of the same planes can
+// reach on the device; bench.py reports the advance against it.
+__global__ __launch_bounds__(256) void stream5r3w_kernel(const double* __restrict__ a,
+                                                        const double* __restrict__ b,
+                                                        const double* __restrict__ c,
+                                                        const double* __restrict__ d,
+                                                        const double* __restrict__ e,
+                                                        double* __restrict__ x, double* __restrict__ y,
+                                                        double* __restrict__ z, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double sv = d[i] + e[i];
+        x[i] = a[i] + sv;
+        y[i] = b[i] * sv;
+        z[i] = c[i] - sv;
+    }
+}
+
+extern "C" int hx_stream_probe(hx_ctx* c, int64_t n, const void* const* in, void* const* out,
+                               void* stream)
+{
+    if (!c) return HX_E_STATE;
+    if (n <= 0 || !in || !out) return HX_E_BADARG;
+    for (int k = 0; k < 5; ++k)
+        if (!in[k]) return HX_E_BADARG;
+    for (int k = 0; k < 3; ++k)
+        if (!out[k]) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(stream5r3w_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const double*)in[0], (const double*)in[1], (const double*)in[2],
+                       (const double*)in[3], (const double*)in[4], (double*)out[0], (double*)out[1],
+                       (double*)out[2], n);
+    return hipGetLastError() == hipSuccess ? HX_OK : HX_E_HIP;
+}
+
 template <typename R>
 static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu, void* ahv,
                  double cfl, int nsteps, hipStream_t st)

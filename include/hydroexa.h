@@ -162,6 +162,16 @@ int hx_prune_boxes(hx_ctx *ctx, int dtype, int nbox, const int32_t *boxes, int n
 int hx_conservation(hx_ctx *ctx, const hx_patch *level, double *out, void *stream);
 
 /*
+ * Measurement utility with no reference counterpart: the streaming ceiling of
+ * the fused advance's access pattern.  One elementwise pass over n FP64
+ * elements that reads in[0..4] and writes out[0..2]
+ * (out0 = in0 + s, out1 = in1 * s, out2 = in2 - s, s = in3 + in4), enqueued
+ * on stream.  Device pointers; returns HX_E_BADARG for n <= 0 or a null
+ * pointer.
+ */
+int hx_stream_probe(hx_ctx *ctx, int64_t n, const void *const *in, void *const *out, void *stream);
+
+/*
  * Step-driven single-level time loop with the CFL reduction fused into the
  * advance: nsteps x (ComputeDt; AdvanceLevel).  Replaces the loop body of
  * HydroEXA::Compute (src/HydroEXA.cpp:260-271) for one level with a step

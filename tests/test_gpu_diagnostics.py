@@ -73,3 +73,35 @@ def test_monitor_lines_and_lake_at_rest(gpu):
     # lake at rest: mass to round-off, momenta at round-off
     assert abs(rep["error_pct"][0]) < 1e-10
     assert abs(rep["px"]) < 1e-12 and abs(rep["py"]) < 1e-12
+
+
+def test_stream_probe_values_and_arguments(gpu):
+    """The streaming-ceiling probe computes out0 = in0 + s, out1 = in1 s,
+    out2 = in2 - s with s = in3 + in4 on every element (ragged length) and
+    rejects bad arguments."""
+    import ctypes as C
+    from hydroexa_amd import _lib as L
+
+    lib = L.load()
+    ctx = C.c_void_p()
+    L.check(lib.hx_ctx_create(C.byref(ctx), 0), "ctx")
+    try:
+        n = 1_000_003
+        r = np.random.default_rng(3)
+        host = [r.standard_normal(n) for _ in range(5)]
+        ins_t = [torch.tensor(a, device="cuda") for a in host]
+        outs_t = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+        ins = (C.c_void_p * 5)(*[t.data_ptr() for t in ins_t])
+        outs = (C.c_void_p * 3)(*[t.data_ptr() for t in outs_t])
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        L.check(lib.hx_stream_probe(ctx, n, ins, outs, st), "probe")
+        torch.cuda.synchronize()
+        s = host[3] + host[4]
+        np.testing.assert_array_equal(outs_t[0].cpu().numpy(), host[0] + s)
+        np.testing.assert_array_equal(outs_t[1].cpu().numpy(), host[1] * s)
+        np.testing.assert_array_equal(outs_t[2].cpu().numpy(), host[2] - s)
+        assert lib.hx_stream_probe(ctx, 0, ins, outs, st) == L.HX_E_BADARG
+        bad = (C.c_void_p * 5)(*([None] + [t.data_ptr() for t in ins_t[1:]]))
+        assert lib.hx_stream_probe(ctx, n, bad, outs, st) == L.HX_E_BADARG
+    finally:
+        lib.hx_ctx_destroy(ctx)
