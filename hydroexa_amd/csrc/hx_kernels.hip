@@ -853,8 +853,11 @@ extern "C" int hx_conservation(hx_ctx* c, const hx_patch* in, double* out, void*
 // Streaming ceiling of the advance's access pattern: an elementwise pass that
 // reads five FP64 planes and writes three (x = a + s, y = b s, z = c - s with
 // s = d + e), as coalesced and as simple as a kernel with this traffic can
-// be.  Its time bounds from below what any advance of the same planes can
-// reach on the device; bench.py reports the advance against it.
+// be, with the fastest launch measured for it on MI355X (non-temporal loads,
+// 512 workgroups of 256 threads; 1024 and more workgroups, or cached loads,
+// were 3-8 % slower).  Its time bounds from below what an advance of the
+// same planes can reach on the device; bench.py reports the advance
+// against it.
 __global__ __launch_bounds__(256) void stream5r3w_kernel(const double* __restrict__ a,
                                                         const double* __restrict__ b,
                                                         const double* __restrict__ c,
@@ -865,10 +868,10 @@ __global__ __launch_bounds__(256) void stream5r3w_kernel(const double* __restric
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double sv = d[i] + e[i];
-        x[i] = a[i] + sv;
-        y[i] = b[i] * sv;
-        z[i] = c[i] - sv;
+        const double sv = __builtin_nontemporal_load(d + i) + __builtin_nontemporal_load(e + i);
+        x[i] = __builtin_nontemporal_load(a + i) + sv;
+        y[i] = __builtin_nontemporal_load(b + i) * sv;
+        z[i] = __builtin_nontemporal_load(c + i) - sv;
     }
 }
 
@@ -882,7 +885,7 @@ extern "C" int hx_stream_probe(hx_ctx* c, int64_t n, const void* const* in, void
     for (int k = 0; k < 3; ++k)
         if (!out[k]) return HX_E_BADARG;
     HX_TRY(hipSetDevice(c->device));
-    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 512);
     hipLaunchKernelGGL(stream5r3w_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        (const double*)in[0], (const double*)in[1], (const double*)in[2],
                        (const double*)in[3], (const double*)in[4], (double*)out[0], (double*)out[1],
