@@ -627,14 +627,16 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
     const R utl = (DIR == 0) ? vl : -ul;
     const R unr = (DIR == 0) ? ur : vr;
     const R utr = (DIR == 0) ? vr : -ur;
-    const R hmean = R(0.5) * (hi + hj);
-    const R ht = (hmean > k::toldry) ? hmean : k::toldry;
 #if HX_FAST
+    // g max((hi + hj)/2, TOLDRY): scaling by g and halving commute with the
+    // max and with rounding
     R ct, ict;
-    sqrt_rsq(k::g * ht, ct, ict);
+    sqrt_rsq(fmax((hi + hj) * (R(0.5) * k::g), k::g * k::toldry), ct, ict);
     const R qh = R(0.5) * ict;
     const R rden = qh * fma(-k::tol12, qh, R(1));
 #else
+    const R hmean = R(0.5) * (hi + hj);
+    const R ht = (hmean > k::toldry) ? hmean : k::toldry;
     const R ct = sqrt(k::g * ht);
 #endif
     const Div<R> idsq(sql + sqr + k::tol12);
@@ -656,12 +658,24 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
         if (el < R(0) && er > R(0)) l2 = Div<R>(er - el)(er * (l2 - el));
     }
 #if HX_FAST
-    const auto iden = [rden](R a) { return a * rden; };
-    const auto ictd = [ict](R a) { return a * ict; };
+    // central flux minus the upwind wave sum with qk = |lk| ak and the
+    // contact term |ut| a1 ct = |ut| (dhv - vt dh) (1/ct drops out)
+    const R mL = hi * unl, mR = hj * unr;
+    const R dh = hj - hi;
+    const R dhu = mR - mL;
+    const R dhv = hj * utr - hi * utl;
+    const R a0 = (l2 * dh - dhu) * rden;
+    const R a2 = (dhu - l0 * dh) * rden;
+    const R q0 = fabs(l0) * a0, q2 = fabs(l2) * a2;
+    const R q02 = q0 + q2;
+    const R S1 = mL * unl + mR * unr + (R(0.5) * k::g) * (hi * hi + hj * hj);
+    const R S2 = mL * utl + mR * utr;
+    const R f0 = R(0.5) * ((mL + mR) - q02);
+    const R f1 = R(0.5) * (S1 - q0 * l0 - q2 * l2);
+    const R f2 = R(0.5) * (S2 - vt * q02 - fabs(l1) * (dhv - vt * dh));
 #else
     const Div<R> iden(R(2) * ct + k::tol12);
     const Div<R> ictd(ct);
-#endif
     const R dh = hj - hi;
     const R dhu = hj * unr - hi * unl;
     const R dhv = hj * utr - hi * utl;
@@ -678,6 +692,7 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
     f0 -= w0; f1 -= w0 * l0; f2 -= w0 * vt;
     f2 -= w1 * ct;
     f0 -= w2; f1 -= w2 * l2; f2 -= w2 * vt;
+#endif
     F0 = f0;
     if (DIR == 0) { F1 = f1; F2 = f2; }
     else { F1 = -f2; F2 = f1; }
