@@ -629,7 +629,7 @@ extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl,
     StepArgs<double> s{};
     s.p = host::view_of<double>(st);
     s.opitch = a->nx;
-    host::fill_geometry(s, a->solver, host::eff_rows(c, a->solver, HX_F64), a->dx, a->dy);
+    host::fill_geometry(s, a->solver, host::eff_rows(c, a->solver, HX_F64, a->nx, a->ny), a->dx, a->dy);
     if (use_cfl) {
         rc = host::ensure_hist(c, a->nsteps);
         if (rc) return rc;
@@ -765,7 +765,7 @@ extern "C" int hx_adjoint_backward(hx_adjoint* a, const double* g_series, const 
     }
     s.p = host::view_of<double>(&geom);
     s.opitch = a->nx;
-    host::fill_geometry(s, a->solver, host::eff_rows(c, a->solver, HX_F64), a->dx, a->dy);
+    host::fill_geometry(s, a->solver, host::eff_rows(c, a->solver, HX_F64, a->nx, a->ny), a->dx, a->dy);
     for (int cp = a->ncp - 1; cp >= 0; --cp) {
         const int s0 = cp * a->every, s1 = std::min(s0 + a->every, a->nsteps);
         auto state = [&](int t) -> double* {

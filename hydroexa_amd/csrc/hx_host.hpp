@@ -4,6 +4,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdint>
 
 #include <hip/hip_runtime.h>
 
@@ -24,34 +25,56 @@ struct hx_ctx {
     int rows[2][2];      // [solver][dtype]; 0: the default of the arithmetic build
     int occ[2][2];       // [solver][dtype] minimum waves per SIMD requested; 0: default
     int arith;           // HX_ARITH_EXACT or HX_ARITH_FAST
+    int cus;             // compute units of the device
     double* cons;        // conservation partials (kConsBlocks x 9) + result (5)
 };
 
 namespace hx {
 namespace host {
 
-// Launch shape of the fused advance for (solver, dtype) under the context's
-// arithmetic build: the value set by hx_set_tuning, or the measured default
-// (one MI355X, 4096^2 levels: 16 rows and 3 waves per SIMD for FP64; FP32
-// plain Roe in the fast build marches 32 rows).
-inline int default_rows(int solver, int dtype, int arith)
-{
-    return (dtype == HX_F32 && solver == HX_SOLVER_ROE && arith == HX_ARITH_FAST) ? 32 : 16;
-}
+// Launch shape of the fused advance for (solver, dtype): the value set by
+// hx_set_tuning, or the measured default (one MI355X):
+// * 3 waves per SIMD for FP64, 1 requested for FP32;
+// * plain Roe streams HBM and marches 16 rows (14, 18 and 20 rows were
+//   1-3 % slower on the 4096^2 dam break, 24-128 rows 5-30 %, in FP64 and
+//   FP32);
+// * the FP64 hydrostatic march is arithmetic-bound, so a launch ends with a
+//   tail in which fewer wavefronts than resident slots are left and SIMDs
+//   idle.  Its height is the one in [8, 32] that minimizes
+//   (rounds of resident slots) x (rows + 4 prologue rows), the larger on
+//   ties: 32 rows at 4096^2 and 24 at 2048^2 (2-5 % faster than 16), 8 on
+//   levels too small to fill the device once.
 inline int default_occ(int solver, int dtype)
 {
     (void)solver;
     return (dtype == HX_F64) ? 3 : 1;
 }
-inline int eff_rows(const hx_ctx* c, int solver, int dtype)
+inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int occ)
 {
-    const int r = c->rows[solver][dtype];
-    return r ? r : default_rows(solver, dtype, c->arith);
+    if (solver != HX_SOLVER_HYDROSTATIC_ROE || dtype != HX_F64 || nx <= 0 || ny <= 0 ||
+        cus <= 0 || occ <= 0)
+        return 16;
+    const int64_t slots = (int64_t)cus * 4 * occ;
+    const int64_t nsx = (nx + kHydroCols - 1) / kHydroCols;
+    int best = 16;
+    int64_t cost = INT64_MAX;
+    for (int r = 8; r <= 32; ++r) {
+        const int64_t waves = nsx * ((ny + r - 1) / r);
+        const int64_t k = ((waves + slots - 1) / slots) * (r + 4);
+        if (k <= cost) { cost = k; best = r; }
+    }
+    return best;
 }
 inline int eff_occ(const hx_ctx* c, int solver, int dtype)
 {
     const int o = c->occ[solver][dtype];
     return o ? o : default_occ(solver, dtype);
+}
+// nx, ny: the level the launch covers (0 when the launch covers many).
+inline int eff_rows(const hx_ctx* c, int solver, int dtype, int nx, int ny)
+{
+    const int r = c->rows[solver][dtype];
+    return r ? r : default_rows(solver, dtype, nx, ny, c->cus, eff_occ(c, solver, dtype));
 }
 
 }  // namespace host

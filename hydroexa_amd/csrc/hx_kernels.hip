@@ -543,6 +543,8 @@ extern "C" int hx_ctx_create(hx_ctx** out, int device)
             c->occ[s][d] = 0;
         }
     c->arith = HX_ARITH_EXACT;
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        c->cus = 256;
     hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, 0, c->slots, 4, c->err);
     HX_TRY(hipDeviceSynchronize());
     *out = c;
@@ -621,7 +623,7 @@ static int advance_t(hx_ctx* c, int solver, const hx_patch* in, double dt, void*
     a.hvo = (R*)hvo;
     a.opitch = opitch;
     a.dt_host = dt;
-    fill_geometry(a, solver, eff_rows(c, solver, in->dtype), in->dx, in->dy);
+    fill_geometry(a, solver, eff_rows(c, solver, in->dtype, in->nx, in->ny), in->dx, in->dy);
     return launch(c, solver, in->dtype, a, st);
 }
 
@@ -904,7 +906,7 @@ static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
     a.err = c->err;
     a.dt_hist = c->hist;
     a.vel_hist = c->hist + c->hist_cap;
-    fill_geometry(a, solver, eff_rows(c, solver, in->dtype), in->dx, in->dy);
+    fill_geometry(a, solver, eff_rows(c, solver, in->dtype, in->nx, in->ny), in->dx, in->dy);
     // reset slots and the error flag, then reduce the initial state's CFL
     hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots, 3, c->err);
     int rc = launch_cfl<R>(in, c->slots, st);
@@ -1077,7 +1079,7 @@ extern "C" int hx_ensemble_create(hx_ctx* c, int solver, int dtype, int nscen, c
     e->solver = solver;
     e->dtype = dtype;
     e->nscen = nscen;
-    e->rows = eff_rows(c, solver, dtype);
+    e->rows = eff_rows(c, solver, dtype, 0, 0);
     e->occ = eff_occ(c, solver, dtype);
     e->persistent = 0;
     e->patches.assign(scen, scen + nscen);
@@ -1120,7 +1122,7 @@ extern "C" int hx_ensemble_set_tuning(hx_ensemble* e, int rows, int occupancy, i
     if (!known || rows < 0 || rows > 1 << 20 || (persistent != 0 && persistent != 1))
         return HX_E_BADARG;
     HX_TRY(hipSetDevice(e->ctx->device));
-    e->rows = rows ? rows : default_rows(e->solver, e->dtype, e->ctx->arith);
+    e->rows = rows ? rows : default_rows(e->solver, e->dtype, 0, 0, 0, 0);
     e->occ = occupancy ? occupancy : default_occ(e->solver, e->dtype);
     e->persistent = persistent;
     return ens_upload_any(e);
@@ -1444,14 +1446,14 @@ static int two_level_t(hx_ctx* c, int solver, const hx_patch* crse, void* const*
     ac.dt_dev = dt;
     ac.sk_i0 = ci0 - 1; ac.sk_i1 = ci0 + ncx + 1;
     ac.sk_j0 = cj0 - 1; ac.sk_j1 = cj0 + ncy + 1;
-    fill_geometry(ac, solver, eff_rows(c, solver, crse->dtype), crse->dx, crse->dy);
+    fill_geometry(ac, solver, eff_rows(c, solver, crse->dtype, crse->nx, crse->ny), crse->dx, crse->dy);
     StepArgs<R> af{};
     af.p = view_of<R>(fine);
     af.opitch = fine->pitch;
     af.cfl = cfl;
     af.err = err;
     af.dt_dev = dt + 1;
-    fill_geometry(af, solver, eff_rows(c, solver, fine->dtype), fine->dx, fine->dy);
+    fill_geometry(af, solver, eff_rows(c, solver, fine->dtype, fine->nx, fine->ny), fine->dx, fine->dy);
     FaceReq qc{};
     qc.nfaces = 2 * (ncx + ncy); qc.pair = 1; qc.accumulate = 0; qc.box_fine = 0;
     qc.ci0 = ci0; qc.cj0 = cj0; qc.ncx = ncx; qc.ncy = ncy;
@@ -1792,7 +1794,7 @@ int amr_advance(AmrRun& r, int l, double t, double dt)
         a.ho = (R*)out.h; a.huo = (R*)out.hu; a.hvo = (R*)out.hv;
         a.opitch = out.pitch;
         a.dt_host = dt;
-        fill_geometry(a, r.solver, eff_rows(r.c, r.solver, r.dtype), in.dx, in.dy);
+        fill_geometry(a, r.solver, eff_rows(r.c, r.solver, r.dtype, in.nx, in.ny), in.dx, in.dy);
         if ((rc = launch(r.c, r.solver, r.dtype, a, r.st))) return rc;
     }
     return HX_OK;
