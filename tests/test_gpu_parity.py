@@ -405,3 +405,44 @@ def test_strip_height_leaves_results_unchanged(gpu, arith, model):
     finally:
         set_tuning(model, torch.float64, 0, 0)
         set_arith("exact")
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_widest_pitch_shortens_strips(gpu, model):
+    """The advance addresses a strip as base + 32-bit byte offsets, so at the
+    largest accepted pitch (2^26 elements, 512 MiB per FP64 row) the host
+    shortens the strips to 4 rows: the result equals the same level stored
+    densely, bit for bit (10 rows, so strips of 4, 4 and 2 rows)."""
+    import ctypes as C
+    from hydroexa_amd.solver import context, _stream, MODELS
+    ny, nx, pitch = 10, 150, 1 << 26
+    st = random_state(np.random.default_rng(21), ny, nx, 0.05, False, True)
+    lo, hi = (0, 1), (1, 0)
+    dx, dy = 1.0 / nx, 1.0 / ny
+    dt = 0.4 * O.level_dt(*st[:4], dx, dy)[0]
+    ref = gpu_step(model, st, dt, dx, dy, lo, hi)
+    big = torch.empty((5, ny, pitch), dtype=torch.float64, device="cuda")   # 26.8 GB
+    try:
+        for k in range(5):
+            big[k, :, :nx] = torch.as_tensor(st[k], device="cuda")
+        es = 8
+        p = L.HxPatch()
+        p.nx, p.ny, p.ng, p.dtype, p.pitch = nx, ny, 0, L.HX_F64, pitch
+        base = big.data_ptr()
+        p.h, p.hu, p.hv, p.z, p.n = (base + k * ny * pitch * es for k in range(5))
+        p.eta = None
+        p.dx, p.dy = dx, dy
+        p.bc_lo[0], p.bc_lo[1] = lo
+        p.bc_hi[0], p.bc_hi[1] = hi
+        out = torch.zeros((3, ny, nx), dtype=torch.float64, device="cuda")
+        plane = ny * nx * es
+        rc = L.load().hx_advance_level(context(), MODELS[model], C.byref(p), dt, out.data_ptr(),
+                                       out.data_ptr() + plane, out.data_ptr() + 2 * plane, nx,
+                                       _stream())
+        L.check(rc, "hx_advance_level")
+        got = out.cpu().numpy()
+    finally:
+        del big
+        torch.cuda.empty_cache()
+    for k in range(3):
+        np.testing.assert_array_equal(got[k], ref[k])
