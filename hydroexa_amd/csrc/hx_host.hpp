@@ -35,28 +35,34 @@ namespace host {
 // Launch shape of the fused advance for (solver, dtype): the value set by
 // hx_set_tuning, or the measured default (one MI355X):
 // * 3 waves per SIMD for FP64, 1 requested for FP32;
-// * plain Roe streams HBM and marches 16 rows (14, 18 and 20 rows were
-//   1-3 % slower on the 4096^2 dam break, 24-128 rows 5-30 %, in FP64 and
-//   FP32);
-// * the FP64 hydrostatic march is arithmetic-bound, so a launch ends with a
-//   tail in which fewer wavefronts than resident slots are left and SIMDs
-//   idle.  Its height is the one in [8, 32] that minimizes
-//   (rounds of resident slots) x (rows + 4 prologue rows), the larger on
-//   ties: 32 rows at 4096^2 and 24 at 2048^2 (2-5 % faster than 16), 8 on
-//   levels too small to fill the device once.
+// * the strip height from the level size:
+//   - the FP64 hydrostatic march is arithmetic-bound, so a launch ends with
+//     a tail in which fewer wavefronts are left than the device holds
+//     (slots = CUs x 4 SIMDs x the kernel's waves per SIMD) and SIMDs idle.
+//     Its height is the r in [8, 32] that minimizes (rounds of slots) x
+//     (r + 4 prologue rows), the larger on ties: 32 rows at 4096^2, 24 at
+//     2048^2 (2-7 % faster than 16 on the lakes at rest and the cylinder dam
+//     break), 8 on levels that do not fill the device once;
+//   - the other marches stream HBM: 16 rows when that gives at least 3
+//     wavefronts per SIMD (14, 18 and 20 rows were 1-3 % slower on the
+//     4096^2 dam break, 24-128 rows 5-30 %, and 16 rows stay best at 2048^2
+//     in FP64 and FP32), else 6 rows, so a small level gets more, shorter
+//     strips (256^2 to 1024^2 dam breaks: 1.1-1.9x faster than 16 rows).
 inline int default_occ(int solver, int dtype)
 {
     (void)solver;
     return (dtype == HX_F64) ? 3 : 1;
 }
-inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int occ)
+inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd_waves)
 {
-    if (solver != HX_SOLVER_HYDROSTATIC_ROE || dtype != HX_F64 || nx <= 0 || ny <= 0 ||
-        cus <= 0 || occ <= 0)
-        return 16;
-    const int64_t slots = (int64_t)cus * 4 * occ;
-    const int64_t nsx = (nx + kHydroCols - 1) / kHydroCols;
-    int best = 16;
+    if (nx <= 0 || ny <= 0 || cus <= 0) return 16;
+    const bool hydro = (solver == HX_SOLVER_HYDROSTATIC_ROE);
+    const int cols = hydro ? kHydroCols : kRoeCols;
+    const int64_t nsx = (nx + cols - 1) / cols;
+    if (!(hydro && dtype == HX_F64)) return (nsx * ((ny + 15) / 16) >= (int64_t)cus * 12) ? 16 : 6;
+    if (simd_waves <= 0) return 16;
+    const int64_t slots = (int64_t)cus * 4 * simd_waves;
+    int best = 32;
     int64_t cost = INT64_MAX;
     for (int r = 8; r <= 32; ++r) {
         const int64_t waves = nsx * ((ny + r - 1) / r);
@@ -70,11 +76,16 @@ inline int eff_occ(const hx_ctx* c, int solver, int dtype)
     const int o = c->occ[solver][dtype];
     return o ? o : default_occ(solver, dtype);
 }
+// Defined in hx_kernels.hip: waves per SIMD of the advance kernel in use.
+int advance_simd_waves(const hx_ctx* c, int solver, int dtype);
 // nx, ny: the level the launch covers (0 when the launch covers many).
 inline int eff_rows(const hx_ctx* c, int solver, int dtype, int nx, int ny)
 {
     const int r = c->rows[solver][dtype];
-    return r ? r : default_rows(solver, dtype, nx, ny, c->cus, eff_occ(c, solver, dtype));
+    if (r) return r;
+    if (nx <= 0 || ny <= 0) return 16;
+    const bool query = (solver == HX_SOLVER_HYDROSTATIC_ROE && dtype == HX_F64);
+    return default_rows(solver, dtype, nx, ny, c->cus, query ? advance_simd_waves(c, solver, dtype) : 0);
 }
 
 }  // namespace host

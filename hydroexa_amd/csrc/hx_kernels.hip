@@ -603,6 +603,14 @@ int hx::host::check_patch(const hx_patch* in, int need_ghost)
     return HX_OK;
 }
 
+int hx::host::advance_simd_waves(const hx_ctx* c, int solver, int dtype)
+{
+    const int occ = eff_occ(c, solver, dtype);
+    const bool fa = (c->arith == HX_ARITH_FAST);
+    if (dtype == HX_F64) return fa ? fast::simd_waves<double>(solver, occ) : exact::simd_waves<double>(solver, occ);
+    return fa ? fast::simd_waves<float>(solver, occ) : exact::simd_waves<float>(solver, occ);
+}
+
 template <typename R>
 static int launch(const hx_ctx* c, int solver, int dtype, const StepArgs<R>& a, hipStream_t st)
 {
@@ -1122,7 +1130,7 @@ extern "C" int hx_ensemble_set_tuning(hx_ensemble* e, int rows, int occupancy, i
     if (!known || rows < 0 || rows > 1 << 20 || (persistent != 0 && persistent != 1))
         return HX_E_BADARG;
     HX_TRY(hipSetDevice(e->ctx->device));
-    e->rows = rows ? rows : default_rows(e->solver, e->dtype, 0, 0, 0, 0);
+    e->rows = rows ? rows : 16;
     e->occ = occupancy ? occupancy : default_occ(e->solver, e->dtype);
     e->persistent = persistent;
     return ens_upload_any(e);

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cstring>
 
 namespace hx {
@@ -114,6 +115,8 @@ constexpr int kOcc[kOccVariants] = {1, 3, 4};
 namespace exact {
 template <typename R>
 int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
+template <typename R>
+int simd_waves(int solver, int occ);
 int launch_adjpost(int occ, const StepArgs<double>& a, hipStream_t st);
 template <typename R>
 int launch_bflux(int solver, const PatchView<R>& p, R dx, R dy, const FaceReq& q, R* out,
@@ -122,6 +125,8 @@ int launch_bflux(int solver, const PatchView<R>& p, R dx, R dy, const FaceReq& q
 namespace fast {
 template <typename R>
 int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
+template <typename R>
+int simd_waves(int solver, int occ);
 int launch_adjpost(int occ, const StepArgs<double>& a, hipStream_t st);
 template <typename R>
 int launch_bflux(int solver, const PatchView<R>& p, R dx, R dy, const FaceReq& q, R* out,
