@@ -192,8 +192,10 @@ int hx_run_level(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
  * Launch-shape control for the fused advance (the autotuner's knobs): rows
  * marched per wavefront and the minimum waves per SIMD the register
  * allocator must leave room for (1, 3 or 4); 0 selects the measured default
- * of the arithmetic build in use when the advance is launched.  No reference
- * counterpart (AMReX picks launch shapes itself).
+ * when the advance is launched (3 waves per SIMD for FP64; a strip height
+ * chosen from the level's size, the solver and the kernel's occupancy).
+ * Results do not depend on either.  No reference counterpart (AMReX picks
+ * launch shapes itself).
  */
 int hx_set_tuning(hx_ctx *ctx, int solver, int dtype, int rows, int occupancy);
 
