@@ -73,10 +73,30 @@ def cpu_baseline(w, budget_s):
     t0 = time.perf_counter()
     O.run(solver, w.h, w.hu, w.hv, *args, steps, w.bc_lo, w.bc_hi)
     el = time.perf_counter() - t0
-    return {"value": w.cells * steps / el / 1e6, "unit": "Mcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{steps} steps of the {w.name} workload ({w.cells} cells) with the oracle, "
-                      f"{el:.1f} s, OMP threads {threads}"}
+    out = {"value": w.cells * steps / el / 1e6, "unit": "Mcell-updates/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{steps} steps of the {w.name} workload ({w.cells} cells) with the oracle, "
+                     f"{el:.1f} s, OMP threads {threads}"}
+    # the same oracle on one core (two whole steps), and the host's CPU model
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        try:
+            t0 = time.perf_counter()
+            O.run(solver, w.h, w.hu, w.hv, *args, 2, w.bc_lo, w.bc_hi)
+            el1 = time.perf_counter() - t0
+        finally:
+            gomp.omp_set_num_threads(threads)
+        out["one_core"] = {"value": w.cells * 2 / el1 / 1e6, "sample": f"2 steps, {el1:.1f} s, OMP threads 1"}
+    except OSError:
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            out["cpu_model"] = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    return out
 
 
 def time_gpu(w, dtype, steps, warmup, rows, arith="fast", keep=False):
