@@ -301,7 +301,7 @@ def time_adjoint(steps, every=20, arith="fast"):
            "adjoint_Mcups": cells * steps / (bwd_ms / 1e3) / 1e6,
            "forward_plus_adjoint_Mcups": cells * steps / ((fwd_ms + bwd_ms) / 1e3) / 1e6,
            "backward_over_forward": bwd_ms / fwd_ms,
-           "recomputed_steps": steps - ncp,
+           "recomputed_steps": steps,   # hydrostatic: every segment's last step too
            "checkpoint_bytes": ncp * 3 * cells * 8,
            "calibration_iteration_s": it_s, "loss": float(loss.item()),
            "grad_n_ctrl_norm": float(nc.grad.norm().item())}

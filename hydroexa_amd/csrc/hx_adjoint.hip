@@ -6,10 +6,10 @@
 //
 // One adjoint step, lambda^{n+1} -> lambda^n (plus the roughness and bed
 // gradients accumulated), is deterministic (no atomics):
-//   post    (hydrostatic only): the forward's strip march re-evaluates the
-//           flux update of every cell, then applies the reverse of Manning
-//           friction and the dry clamp -> lambda_u (adjoint of the
-//           pre-friction state; hydro_strip<double, true> in hx_advance.inc);
+//   post    (hydrostatic only): per cell, the reverse of Manning friction
+//           and the dry clamp at the step's state after the flux update,
+//           which the segment's forward recomputation stored (StepArgs::uno)
+//           -> lambda_u (adjoint of the pre-friction state);
 //   tile    per tile of cells: the vector-Jacobian product of every face
 //           whose stencil touches the tile (2 cells for plain Roe, 4 for the
 //           hydrostatic face), summed in LDS, then per cell the gather over
@@ -422,6 +422,29 @@ __global__ __launch_bounds__(kThreads) void rtile_kernel(State s, Geo g, double 
 // ---------------------------------------------------------------------------
 // Small kernels
 // ---------------------------------------------------------------------------
+// Reverse of Manning friction and the dry clamp of the hydrostatic update
+// (HydrostaticRoe.cpp:566-603), cell by cell, from the stored state after
+// the flux update: lun = d(output)/d(un)^T lam, gn += d/dn.
+__global__ __launch_bounds__(kThreads) void post_kernel(const double* __restrict__ un,
+                                                        const double* __restrict__ n,
+                                                        const double* __restrict__ lam,
+                                                        double* __restrict__ lun,
+                                                        double* __restrict__ gn, double dt,
+                                                        int64_t N)
+{
+    for (int64_t o = blockIdx.x * (int64_t)kThreads + threadIdx.x; o < N;
+         o += (int64_t)gridDim.x * kThreads) {
+        const double u[3] = {un[o], un[N + o], un[2 * N + o]};
+        const double ob[3] = {lam[o], lam[N + o], lam[2 * N + o]};
+        double ub[3];
+        const double dn = hydro_post_vjp(u, n[o], dt, ob, ub);
+        lun[o] = ub[0];
+        lun[N + o] = ub[1];
+        lun[2 * N + o] = ub[2];
+        gn[o] += dn;
+    }
+}
+
 __global__ void fill_kernel(double* p, int64_t n, double v)
 {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
@@ -472,6 +495,8 @@ struct hx_adjoint {
     int nu;
     double* ckpt;      // ncp states (3N each): the state at steps 0, every, 2 every, ...
     double* seg;       // nseg states: recomputed states inside a segment
+    double* un;        // hydrostatic: every states after the flux update and
+                       // before friction, stored by the segment's recomputation
     double* lam;       // 2 states: adjoint ping-pong
     double* lun;       // 1 state: adjoint before friction (hydrostatic)
     std::vector<double> dts;
@@ -482,7 +507,7 @@ struct hx_adjoint {
 
 static void adj_free(hx_adjoint* a)
 {
-    double* bufs[] = {a->ckpt, a->seg, a->lam, a->lun};
+    double* bufs[] = {a->ckpt, a->seg, a->lam, a->lun, a->un};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (a->gauges) (void)hipFree(a->gauges);
@@ -525,14 +550,17 @@ extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, in
     a->nsteps = nsteps;
     a->every = std::min(ckpt_every, nsteps);
     a->ncp = (nsteps + a->every - 1) / a->every;
-    a->nseg = std::max(a->every - 1, 2);
+    // the hydrostatic recomputation also runs a segment's last step (for its
+    // pre-friction state) and needs a slot for that step's output
+    a->nseg = std::max(a->every - (solver == HX_SOLVER_HYDROSTATIC_ROE ? 0 : 1), 2);
     a->ngauge = ngauge;
     a->N = N;
     const size_t S = sizeof(double) * 3 * (size_t)N;
     bool ok = hipMalloc(&a->ckpt, S * a->ncp) == hipSuccess &&
               hipMalloc(&a->seg, S * a->nseg) == hipSuccess &&
               hipMalloc(&a->lam, 2 * S) == hipSuccess;
-    if (ok && solver == HX_SOLVER_HYDROSTATIC_ROE) ok = hipMalloc(&a->lun, S) == hipSuccess;
+    if (ok && solver == HX_SOLVER_HYDROSTATIC_ROE)
+        ok = hipMalloc(&a->lun, S) == hipSuccess && hipMalloc(&a->un, S * a->every) == hipSuccess;
     if (ok && ngauge > 0) {
         // group gauges by cell (first appearance order) for the seed kernel
         std::vector<int64_t> ucell;
@@ -699,9 +727,10 @@ extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl,
 }
 
 // One adjoint step: lam (adjoint of the step's output) -> lam_new (adjoint
-// of its input U), accumulating the bed and roughness gradients.  sa holds
-// the step geometry (a hydrostatic forward launch shape) for the post pass.
-static int adj_step(hx_adjoint* a, StepArgs<double> sa, const double* U, double dt,
+// of its input U), accumulating the bed and roughness gradients.  un: the
+// step's state after the flux update (hydrostatic), stored by the forward
+// recomputation.
+static int adj_step(hx_adjoint* a, const double* U, const double* un, double dt,
                     const double* lam, double* lam_new, double* gz, double* gn, hipStream_t hs)
 {
     const int64_t N = a->N;
@@ -709,17 +738,9 @@ static int adj_step(hx_adjoint* a, StepArgs<double> sa, const double* U, double 
     Geo g{a->nx, a->ny, a->bc_lo[0], a->bc_lo[1], a->bc_hi[0], a->bc_hi[1]};
     const dim3 tiles((unsigned)((a->nx + kTX - 1) / kTX), (unsigned)((a->ny + kTY - 1) / kTY));
     if (a->solver == HX_SOLVER_HYDROSTATIC_ROE) {
-        sa.p.h = U;
-        sa.p.hu = U + N;
-        sa.p.hv = U + 2 * N;
-        sa.rd = nullptr;
-        sa.acc = nullptr;
-        sa.dt_host = dt;
-        sa.lam = lam;
-        sa.lun = a->lun;
-        sa.gn = gn;
-        const int rc = host::launch_adjpost_f64(a->ctx, sa, hs);
-        if (rc) return rc;
+        const unsigned pb = (unsigned)std::min<int64_t>(nblocks(N), 8192);
+        hipLaunchKernelGGL(post_kernel, dim3(pb), dim3(kThreads), 0, hs, un, (const double*)a->n,
+                           lam, a->lun, gn, dt, N);
         hipLaunchKernelGGL(htile_kernel, tiles, dim3(kThreads), 0, hs, s, g, dt, a->dx, a->dy,
                            (const double*)a->lun, lam_new, gz);
     } else {
@@ -771,12 +792,18 @@ extern "C" int hx_adjoint_backward(hx_adjoint* a, const double* g_series, const 
         auto state = [&](int t) -> double* {
             return (t == s0) ? state_ptr(a, a->ckpt, cp) : state_ptr(a, a->seg, t - s0 - 1);
         };
-        // recompute the segment's states from its checkpoint
-        for (int t = s0; t < s1 - 1; ++t) {
+        // recompute the segment's states from its checkpoint; the
+        // hydrostatic steps also store their states before friction, the
+        // segment's last step included (its output lands in a spare slot)
+        const bool hyd = (a->solver == HX_SOLVER_HYDROSTATIC_ROE);
+        for (int t = s0; t < (hyd ? s1 : s1 - 1); ++t) {
             double* o = state(t + 1);
             set_io(s, state(t), N, o, o + N, o + 2 * N);
             s.dt_host = a->dts[t];
+            s.uno = hyd ? a->un + (size_t)(t - s0) * 3 * N : nullptr;
+            s.uno_plane = N;
             int rc = host::launch_step_f64(c, a->solver, s, hs);
+            s.uno = nullptr;
             if (rc) return rc;
         }
         for (int t = s1 - 1; t >= s0; --t) {
@@ -785,7 +812,8 @@ extern "C" int hx_adjoint_backward(hx_adjoint* a, const double* g_series, const 
                                    a->ucell, a->uptr, a->uidx,
                                    g_series + (size_t)t * a->ngauge, a->nu);
             }
-            int rc = adj_step(a, s, state(t), a->dts[t], lam, lam2, g_z, g_n, hs);
+            const double* un = a->un ? a->un + (size_t)(t - s0) * 3 * N : nullptr;
+            int rc = adj_step(a, state(t), un, a->dts[t], lam, lam2, g_z, g_n, hs);
             if (rc) return rc;
             std::swap(lam, lam2);
         }

@@ -77,11 +77,11 @@ struct StepArgs {
     double* vel_hist;
     int* err;
     int step;
-    // adjoint post pass (hydrostatic, FP64): lambda of the step's output in,
-    // lambda before friction out, d/dn accumulated; null in a forward step
-    const R* lam;
-    R* lun;
-    R* gn;
+    // forward hydrostatic step: when non-null, the state after the flux
+    // update and before friction and the dry clamp is stored here too (the
+    // adjoint's recomputation), three planes uno_plane apart at pitch opitch
+    R* uno = nullptr;
+    int64_t uno_plane = 0;
     // dt of this step in device memory (a multi-level rule computed it);
     // used when non-null, with err checked first
     const double* dt_dev = nullptr;
@@ -117,7 +117,6 @@ template <typename R>
 int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
 template <typename R>
 int simd_waves(int solver, int occ);
-int launch_adjpost(int occ, const StepArgs<double>& a, hipStream_t st);
 template <typename R>
 int launch_bflux(int solver, const PatchView<R>& p, R dx, R dy, const FaceReq& q, R* out,
                  hipStream_t st);
@@ -127,7 +126,6 @@ template <typename R>
 int launch_advance(int solver, int occ, const StepArgs<R>& a, hipStream_t st);
 template <typename R>
 int simd_waves(int solver, int occ);
-int launch_adjpost(int occ, const StepArgs<double>& a, hipStream_t st);
 template <typename R>
 int launch_bflux(int solver, const PatchView<R>& p, R dx, R dy, const FaceReq& q, R* out,
                  hipStream_t st);
