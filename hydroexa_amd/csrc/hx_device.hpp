@@ -148,21 +148,6 @@ __device__ __forceinline__ void map_axis(int i, int n, int blo, int bhi, int& sr
 // momenta with their vector rules, and h is rebuilt on this cell's own
 // (edge) bed -- the reference's FillPatchEta + FillPatch + rebuild sequence
 // (src/solvers/Roe.cpp:315-339, src/solvers/HydrostaticRoe.cpp:477-500).
-// Load of a cell known to lie inside the patch (no boundary mapping).
-template <typename R>
-__device__ __forceinline__ Raw<R> load_in(const PatchView<R>& p, int64_t s)
-{
-    Raw<R> c;
-    c.h = p.h[s];
-    c.z = p.z[s];
-    c.n = p.n[s];
-    const R e = p.eta ? p.eta[s] : (c.h + c.z);
-    c.hr = smax(R(0), e - c.z);
-    c.qx = p.hu[s];
-    c.qy = p.hv[s];
-    return c;
-}
-
 template <typename R>
 __device__ __forceinline__ Raw<R> load_adv(const PatchView<R>& p, int i, int j)
 {
@@ -738,9 +723,28 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
 // One face: NODATA wall, face bed, MUSCL face states, Audusse depths,
 // homogeneous Roe, hydrostatic pressure correction (HydrostaticRoe.cpp:234-396).
 // Only the normal component of FL/FR differs from the Roe flux, by pL/pR.
+// The MUSCL states of a side (eta, u, v extrapolated half a cell towards the
+// face) come in with its bed: the x face takes the left side's from the lane
+// that owns that cell (4 values instead of its whole reconstruction).
+template <typename R>
+struct HSide {
+    R z, eta, u, v;
+};
+
+template <typename R>
+__device__ __forceinline__ HSide<R> side_low(const HydroCell<R>& c)    // seen from the face above
+{
+    return HSide<R>{c.z, c.eta + R(0.5) * c.se, c.u + R(0.5) * c.su, c.v + R(0.5) * c.sv};
+}
+template <typename R>
+__device__ __forceinline__ HSide<R> side_high(const HydroCell<R>& c)   // seen from the face below
+{
+    return HSide<R>{c.z, c.eta - R(0.5) * c.se, c.u - R(0.5) * c.su, c.v - R(0.5) * c.sv};
+}
+
 template <typename R, int DIR>
-__device__ __forceinline__ void hydro_face(const HydroCell<R>& L, const HydroCell<R>& Rc,
-                                           R& F0, R& F1, R& F2, R& pL, R& pR)
+__device__ __forceinline__ void hydro_face_st(const HSide<R>& L, const HSide<R>& Rc,
+                                              R& F0, R& F1, R& F2, R& pL, R& pR)
 {
     using k = Kc<R>;
     if (fabs(L.z + R(9999)) < k::tol10 || fabs(Rc.z + R(9999)) < k::tol10) {
@@ -752,12 +756,9 @@ __device__ __forceinline__ void hydro_face(const HydroCell<R>& L, const HydroCel
 #else
     const R zf = smax(L.z, Rc.z);
 #endif
-    const R etaLf = L.eta + R(0.5) * L.se;
-    const R etaRf = Rc.eta - R(0.5) * Rc.se;
-    const R uLf = L.u + R(0.5) * L.su;
-    const R uRf = Rc.u - R(0.5) * Rc.su;
-    const R vLf = L.v + R(0.5) * L.sv;
-    const R vRf = Rc.v - R(0.5) * Rc.sv;
+    const R etaLf = L.eta, etaRf = Rc.eta;
+    const R uLf = L.u, uRf = Rc.u;
+    const R vLf = L.v, vRf = Rc.v;
     const R hLu = smax(R(0), etaLf - L.z);
     const R hRu = smax(R(0), etaRf - Rc.z);
     const R hLf = smax(R(0), etaLf - zf);
@@ -777,6 +778,13 @@ __device__ __forceinline__ void hydro_face(const HydroCell<R>& L, const HydroCel
     hroe<R, DIR>(hLf, huLf, hvLf, hRf, huRf, hvRf, F0, F1, F2);
     pL = R(0.5) * k::g * (hLu * hLu - hLf * hLf);
     pR = R(0.5) * k::g * (hRu * hRu - hRf * hRf);
+}
+
+template <typename R, int DIR>
+__device__ __forceinline__ void hydro_face(const HydroCell<R>& L, const HydroCell<R>& Rc,
+                                           R& F0, R& F1, R& F2, R& pL, R& pR)
+{
+    hydro_face_st<R, DIR>(side_low(L), side_high(Rc), F0, F1, F2, pL, pR);
 }
 
 // ---------------------------------------------------------------------------
