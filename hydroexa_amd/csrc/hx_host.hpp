@@ -40,7 +40,7 @@ namespace host {
 //     a tail in which fewer wavefronts are left than the device holds
 //     (slots = CUs x 4 SIMDs x the kernel's waves per SIMD) and SIMDs idle.
 //     Its height is the r in [8, 32] that minimizes (rounds of slots) x
-//     (r + 4 prologue rows), the larger on ties: 32 rows at 4096^2, 24 at
+//     (r + 4 prologue rows), the larger on ties: 31 rows at 4096^2, 24 at
 //     2048^2 (2-7 % faster than 16 on the lakes at rest and the cylinder dam
 //     break), 8 on levels that do not fill the device once;
 //   - the other marches stream HBM: 16 rows when that gives at least 3
