@@ -11,6 +11,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -50,7 +51,7 @@ HDF5_ROOT = os.environ.get("HYDROEXA_HDF5", "/opt/conda")
 IO_LIB = os.path.join(LIBDIR, "libhydroexa_io.so")
 
 
-def build_io(verbose: bool = False, libdir: str = LIBDIR) -> str:
+def build_io(verbose: bool = False) -> str:
     """Host-only HDF5 input library (libhydroexa_io.so) with g++ against the
     serial HDF5 C library.  Returns the path, or "" when HDF5 is absent."""
     inc = os.path.join(HDF5_ROOT, "include")
@@ -58,8 +59,8 @@ def build_io(verbose: bool = False, libdir: str = LIBDIR) -> str:
     if not os.path.exists(os.path.join(inc, "hdf5.h")):
         sys.stderr.write(f"HDF5 not found under {HDF5_ROOT}: libhydroexa_io.so not built\n")
         return ""
-    os.makedirs(libdir, exist_ok=True)
-    out = os.path.join(libdir, "libhydroexa_io.so")
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = IO_LIB
     tmp = out + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-static-libstdc++",
            "-static-libgcc",
@@ -76,33 +77,39 @@ def build_io(verbose: bool = False, libdir: str = LIBDIR) -> str:
     return out
 
 
-def build(verbose: bool = False, libdir: str = LIBDIR, defines=()) -> str:
-    """Compile every HIP source and link the shared library (always).  The
-    default target is the in-tree build; libdir/defines build variants."""
-    os.makedirs(libdir, exist_ok=True)
-    lib = os.path.join(libdir, "libhydroexa.so")
-    objs = []
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source (the three translation units in parallel) and
+    link the in-tree shared library, then the HDF5 input library.  Always
+    rebuilds."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    jobs = []
     for src, contract in SOURCES:
-        obj = os.path.join(libdir, os.path.basename(src) + ".o")
-        cmd = [hipcc()] + flags() + [f"-D{d}" for d in defines] + \
-              [f"-ffp-contract={contract}", "-c", src, "-o", obj]
+        obj = os.path.join(LIBDIR, os.path.basename(src) + ".o")
+        cmd = [hipcc()] + flags() + [f"-ffp-contract={contract}", "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        res = subprocess.run(cmd, capture_output=True, text=True)
-        if res.returncode != 0:
-            sys.stderr.write(res.stdout + res.stderr)
-            raise RuntimeError(f"hipcc failed on {os.path.basename(src)}")
-        objs.append(obj)
-    tmp = lib + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp]
+        log = tempfile.TemporaryFile(mode="w+")      # a pipe could fill and stall hipcc
+        jobs.append((src, obj, log, subprocess.Popen(cmd, stdout=log, stderr=subprocess.STDOUT)))
+    failed = []
+    for src, obj, log, proc in jobs:
+        proc.wait()
+        if proc.returncode != 0:
+            log.seek(0)
+            sys.stderr.write(log.read())
+            failed.append(os.path.basename(src))
+        log.close()
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC"] + [o for _, o, _, _ in jobs] + \
+          ["-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("linking libhydroexa.so failed")
-    os.replace(tmp, lib)
-    if libdir == LIBDIR and not defines:
-        build_io(verbose, libdir)
-    return lib
+    os.replace(tmp, LIB)
+    build_io(verbose)
+    return LIB
 
 
 if __name__ == "__main__":

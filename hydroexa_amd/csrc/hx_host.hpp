@@ -13,12 +13,37 @@
 #include "../../include/hydroexa.h"
 #include "hx_types.hpp"
 
+namespace hx {
+// Steps of dt / max|vel| history the context holds; longer runs are done in
+// chunks of this many steps, with one synchronisation per chunk.
+constexpr int kHistCap = 4096;
+struct TwoLevelWork {
+    CflSlot slots[4];   // coarse read/write, fine read/write
+    double dt[2];
+    int err;
+    int pad;
+};
+// Where the last successful hx_run_level / hx_run_level_resume on a context
+// left its rollout: the final state's buffers and geometry, and the slot that
+// holds the CFL estimate of that state (reduced by the last fused step).
+struct RunMark {
+    int valid;
+    int solver, dtype, nx, ny, slot;
+    int bc[4];
+    int64_t pitch;
+    double dx, dy;
+    const void *h, *hu, *hv, *z, *n;
+};
+}  // namespace hx
+
 struct hx_ctx {
     int device;
     hx::CflSlot* slots;  // 4 slots: 3 for the fused loop, 1 for hx_level_dt
     int* err;
-    double* hist;        // 2 * hist_cap doubles (dt, vel)
+    double* hist;        // 2 * hist_cap doubles (dt, vel), allocated with the context
     int hist_cap;
+    hx::TwoLevelWork* tlw;   // hx_two_level_run's slots, dt pair and error flag
+    hx::RunMark mark;
     uint32_t* pflags;    // prune flags
     int32_t* pboxes;
     int pcap;
@@ -130,6 +155,27 @@ inline PatchView<R> view_of(const hx_patch* in)
     return v;
 }
 
+// A strip is addressed as base + 32-bit byte offsets.  The base sits 2 rows
+// and 2 columns before the strip's first cell; the hydrostatic march reads
+// rows up to rows + 3 of that frame and columns up to nx + 1 + (high ghosts)
+// <= pitch + 1 (a stored high ghost makes pitch == nx + ghosts possible), so
+// the largest input offset is (rows + 3) span + (pitch + 1) sizeof(R) with
+// span = max(pitch, opitch) sizeof(R); outputs stay below rows * span.
+// Keeping (rows + 5) span within 4 GiB keeps every offset below 2^32
+// (span >= 2 sizeof(R)).  Both pitches are at most 2^26 elements
+// (check_patch, hx_advance_level), which leaves at least 3 FP64 rows.
+template <typename R>
+inline int64_t max_strip_rows(int64_t pitch, int64_t opitch)
+{
+    const int64_t span = std::max<int64_t>(pitch, opitch) * (int64_t)sizeof(R);
+    return ((int64_t)1 << 32) / span - 5;
+}
+template <typename R>
+inline int clamp_strip_rows(int rows, int64_t pitch, int64_t opitch)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(rows, max_strip_rows<R>(pitch, opitch)));
+}
+
 template <typename R>
 inline void fill_geometry(StepArgs<R>& a, int solver, int rows, double dx, double dy)
 {
@@ -139,10 +185,7 @@ inline void fill_geometry(StepArgs<R>& a, int solver, int rows, double dx, doubl
     a.rdy = (R)(1.0 / dy);
     a.dx_pow2 = is_pow2(dx) && is_pow2(dy) ? 1 : 0;
     const int cols = (solver == HX_SOLVER_ROE) ? kRoeCols : kHydroCols;
-    // a strip is addressed as base + 32-bit byte offsets: rows + 4 rows of
-    // either pitch must span less than 4 GiB (check_patch bounds the pitch)
-    const int64_t span = std::max<int64_t>(a.p.pitch, a.opitch) * (int64_t)sizeof(R);
-    rows = (int)std::max<int64_t>(1, std::min<int64_t>(rows, ((int64_t)1 << 32) / span - 4));
+    rows = clamp_strip_rows<R>(rows, a.p.pitch, a.opitch);
     a.rows = rows;
     a.nsx = (a.p.nx + cols - 1) / cols;
     const int nsy = (a.p.ny + rows - 1) / rows;
@@ -154,7 +197,6 @@ int check_patch(const hx_patch* in, int need_ghost);
 int launch_step_f64(const hx_ctx* c, int solver, const StepArgs<double>& a, hipStream_t st);
 int reduce_cfl_f64(const hx_patch* in, CflSlot* slot, hipStream_t st);
 int clear_slots(CflSlot* s, int n, int* err, hipStream_t st);
-int ensure_hist(hx_ctx* c, int nsteps);
 
 }  // namespace host
 }  // namespace hx

@@ -75,3 +75,81 @@ def test_hydrostatic_fp64_height_minimises_rounds_times_rows(rule):
 def test_unknown_sizes_fall_back_to_16(rule):
     assert rule(ROE, F64, 0, 0) == 16
     assert rule(HYD, F64, 2048, 2048, 256, 0) == 16      # occupancy query failed
+
+
+OFFSETS_SRC = r"""
+#include <cstdio>
+#include <cstdlib>
+#include "hx_host.hpp"
+// Largest byte offsets a strip of the fused advance forms from its base
+// (2 rows and 2 columns before the strip's first cell): input rows up to
+// rows + 3 (the hydrostatic march's second halo row above the strip),
+// input columns up to the last stored column + 2, outputs up to row
+// rows - 1 and column nx - 1.
+template <typename R>
+static void run(int solver, long nx, long pitch, long opitch, int ghost_hi)
+{
+    hx::StepArgs<R> a{};
+    a.p.nx = (int)nx;
+    a.p.ny = 64;
+    a.p.pitch = pitch;
+    a.opitch = opitch;
+    a.p.bh0 = ghost_hi ? HX_BC_GHOST : HX_BC_TRANSMIT;
+    hx::host::fill_geometry(a, solver, 16, 1.0, 1.0);
+    const long sz = (long)sizeof(R);
+    const long in_max = (long)(a.rows + 3) * pitch * sz + (nx - 1 + ghost_hi + 2) * sz;
+    const long out_max = (long)(a.rows - 1) * opitch * sz + (nx - 1) * sz;
+    printf("%d %ld %ld\n", a.rows, in_max, out_max);
+}
+int main(int argc, char** argv)
+{
+    (void)argc;
+    const int f64 = atoi(argv[1]), solver = atoi(argv[2]), ghost_hi = atoi(argv[6]);
+    const long nx = atol(argv[3]), pitch = atol(argv[4]), opitch = atol(argv[5]);
+    if (f64) run<double>(solver, nx, pitch, opitch, ghost_hi);
+    else run<float>(solver, nx, pitch, opitch, ghost_hi);
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def offsets(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not found")
+    d = tmp_path_factory.mktemp("strip_offsets")
+    src, exe = d / "offsets.cpp", d / "offsets"
+    src.write_text(OFFSETS_SRC)
+    subprocess.run([HIPCC, "-std=c++17", f"-I{ROOT}/include", f"-I{ROOT}/hydroexa_amd/csrc",
+                    str(src), "-o", str(exe)], check=True, capture_output=True)
+
+    def run(f64, solver, nx, pitch, opitch, ghost_hi):
+        out = subprocess.run([str(exe), *map(str, (int(f64), solver, nx, pitch, opitch, ghost_hi))],
+                             check=True, capture_output=True, text=True).stdout.split()
+        return tuple(int(v) for v in out)
+    return run
+
+
+WIDEST = 1 << 26      # the largest pitch check_patch and hx_advance_level accept
+
+
+@pytest.mark.parametrize("f64", [True, False])
+@pytest.mark.parametrize("solver", [ROE, HYD])
+@pytest.mark.parametrize("nx,pitch,opitch,ghost_hi", [
+    (WIDEST, WIDEST, WIDEST, 0),              # pitch == nx, physical sides
+    (WIDEST - 2, WIDEST, WIDEST, 2),          # a 2-wide stored high ghost fills the pitch
+    (WIDEST - 2, WIDEST, WIDEST - 2, 2),      # output pitch narrower than the input's
+    (1000, 1000, WIDEST, 0),                  # output pitch is the wide one
+    ((1 << 25) + 7, (1 << 25) + 8, (1 << 25) + 7, 1),
+])
+def test_strip_offsets_stay_below_4_gib(offsets, f64, solver, nx, pitch, opitch, ghost_hi):
+    rows, in_max, out_max = offsets(f64, solver, nx, pitch, opitch, ghost_hi)
+    assert rows >= 1
+    assert in_max < 2 ** 32 and out_max < 2 ** 32, (rows, in_max, out_max)
+
+
+def test_strip_clamp_leaves_ordinary_levels_alone(offsets):
+    assert offsets(True, ROE, 4096, 4096, 4096, 0)[0] == 16
+    assert offsets(True, HYD, 4100, 4104, 4100, 2)[0] == 16
+    # at a pitch of 2^26 FP64 elements three rows fit
+    assert offsets(True, HYD, WIDEST, WIDEST, WIDEST, 0)[0] == 3

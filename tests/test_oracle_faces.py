@@ -140,16 +140,155 @@ def test_roe_dry_dry_face():
     assert np.all(F == 0) and np.all(Sm == 0) and np.all(Sp == 0)
 
 
-def test_roe_entropy_fix_sonic_rarefaction():
-    """Transonic rarefaction (u - c changes sign across the face): the fixed
-    eigenvalue keeps the flux between the two one-sided fluxes instead of the
-    expansion-shock value of the unfixed scheme (Harten-Hyman)."""
-    hl, hr = 1.0, 0.1
-    ul = np.sqrt(G * hl) - 0.5      # u - c < 0 on the left
-    ur = np.sqrt(G * hr) + 0.8      # u - c > 0 on the right
-    F, _, _ = O.roe_face([hl, hl * ul, 0, 0, 0], [hr, hr * ur, 0, 0, 0], 1e-3, 0.01, 1.0, 0.0)
-    fl, fr = phys_flux_x(hl, hl * ul, 0), phys_flux_x(hr, hr * ur, 0)
-    assert min(fl[0], fr[0]) - 1e-9 <= F[0] <= max(fl[0], fr[0]) + 1e-9 or F[0] > 0
+def stationary_jump(h_up, h_dn, sign=1.0):
+    """Two states joined by a stationary Rankine-Hugoniot discontinuity: equal
+    mass flux q and equal momentum flux q^2/h + g h^2/2 on both sides, so
+    q^2 = g h_up h_dn (h_up + h_dn) / 2.  The jump is an eigenvector of the Roe
+    matrix with eigenvalue u~ - c~ = 0 (sign +1) or u~ + c~ = 0 (sign -1)."""
+    q = sign * np.sqrt(G * h_up * h_dn * (h_up + h_dn) / 2.0)
+    return q, [h_up, q, 0.0, 0.0, 0.0], [h_dn, q, 0.0, 0.0, 0.0]
+
+
+def test_roe_keeps_an_admissible_stationary_shock():
+    """A stationary hydraulic jump (shallow fast water into deep slow water)
+    is a Lax shock: no eigenvalue changes sign from negative to positive
+    across it, so the entropy fix must stay off and the Roe flux resolves the
+    jump exactly (F = f_L = f_R).  An entropy-fix test with its sign
+    conditions swapped would fire here."""
+    for hl, hr in ((0.5, 1.0), (0.1, 0.9), (0.3, 2.0)):
+        q, L, R = stationary_jump(hl, hr)
+        F, Sm, Sp = O.roe_face(L, R, 1e-3, 0.01, 1.0, 0.0)
+        np.testing.assert_allclose(F, phys_flux_x(*L[:3]), rtol=1e-11, atol=1e-12)
+        np.testing.assert_allclose(F, phys_flux_x(*R[:3]), rtol=1e-11, atol=1e-12)
+        # the mirrored jump (flow to the left) is admissible too
+        F2, _, _ = O.roe_face([hr, -q, 0, 0, 0], [hl, -q, 0, 0, 0], 1e-3, 0.01, 1.0, 0.0)
+        np.testing.assert_allclose(F2, phys_flux_x(hr, -q, 0), rtol=1e-11, atol=1e-12)
+
+
+def test_roe_entropy_fix_breaks_a_stationary_expansion_shock():
+    """The same stationary discontinuity run backwards (deep slow water into
+    shallow fast water) is an expansion shock: u - c goes from negative on the
+    left to positive on the right.  Without a fix the Roe flux would keep it
+    (F = f_L, since the jump's Roe eigenvalue is 0).  The reference applies
+    Harten-Hyman's fixed eigenvalue l* = e_L (e_R - l) / (e_R - e_L),
+    e = u -/+ sqrt(g h) (kernels/Roe.H:113-127) before the wave strengths, so
+    with l = 0 exactly the strengths are alpha0 = dh and alpha2 = -l* dh / 2c~
+    (wave 1), and the mass flux becomes q - |l*| dh: q + |l*| (h_L - h_R).
+    The mirrored problem (flow to the left, wave 3) gives q - l* (h_R - h_L)."""
+    for hl, hr in ((1.0, 0.5), (0.9, 0.1), (2.0, 0.3)):
+        q, L, R = stationary_jump(hl, hr)
+        el, er = q / hl - np.sqrt(G * hl), q / hr - np.sqrt(G * hr)
+        assert el < 0.0 < er
+        lstar = el * er / (er - el)
+        F, _, _ = O.roe_face(L, R, 1e-3, 0.01, 1.0, 0.0)
+        np.testing.assert_allclose(F[0], q + abs(lstar) * (hl - hr), rtol=1e-10)
+        assert abs(F[0] - q) > 0.05 * q          # the expansion shock is not kept
+        # wave 3: left state shallow and fast to the left, right deep
+        el3, er3 = -q / hr + np.sqrt(G * hr), -q / hl + np.sqrt(G * hl)
+        assert el3 < 0.0 < er3
+        lstar3 = -el3 * er3 / (er3 - el3)
+        F3, _, _ = O.roe_face([hr, -q, 0, 0, 0], [hl, -q, 0, 0, 0], 1e-3, 0.01, 1.0, 0.0)
+        np.testing.assert_allclose(F3[0], -q - lstar3 * (hl - hr), rtol=1e-10)
+
+
+def test_roe_emerged_step_is_a_wall():
+    """Water at rest beside a dry step that rises above its surface, step on
+    the right (dz >= 0, d_i = h_i + z_i < z_j): the depth-limited thrust
+    (dztilde = h_i, kernels/Roe.H:161-163) makes the face a wall.  The wet
+    cell sees exactly its hydrostatic pressure (F + S+ = (0, g h^2 / 2, 0))
+    and the dry cell receives nothing (F - S- = 0).  A centred thrust, a
+    wrong branch test or dztilde = dz leaves a net force."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        h = rng.uniform(0.05, 2.0)
+        zw = rng.uniform(-0.5, 0.5)
+        zs = zw + h + rng.uniform(1e-3, 1.0)          # step crest above the surface
+        n = rng.uniform(0.0, 0.05)
+        wall = [0.0, 0.5 * G * h * h, 0.0]
+        F, Sm, Sp = O.roe_face([h, 0, 0, zw, n], [0, 0, 0, zs, n], 1e-3, 0.01, 1.0, 0.0)
+        np.testing.assert_allclose(F + Sp, wall, rtol=1e-11, atol=1e-12 * G * h * h)
+        np.testing.assert_allclose(F - Sm, 0.0, atol=1e-12 * G * h * h)
+
+
+def test_roe_emerged_step_on_the_left_leaks():
+    """The mirrored step (dz < 0, d_j < z_i) takes dztilde = h_j
+    (kernels/Roe.H:164-165), which is positive although dz is negative, so the
+    thrust -g (h_j - h_j/2) h_j = -g h_j^2 / 2 has the sign of the other
+    branch.  Worked through the Roe face (h~ = h/2, c~ = sqrt(g h / 2),
+    u~ = 0, alpha0 = alpha2 = h/2, beta0 = -c~ h / 2): the wet right cell
+    sees F - S- = (-c~ h, 0, 0) -- no pressure from the wall and a mass flux
+    onto the step -- and the dry left cell F + S+ = (-c~ h, g h^2 / 2, 0).
+    This is the reference's behaviour and the oracle keeps it; dztilde = -h_j
+    (the wall) or dztilde = dz would fail here."""
+    rng = np.random.default_rng(12)
+    for _ in range(200):
+        h = rng.uniform(0.05, 2.0)
+        zw = rng.uniform(-0.5, 0.5)
+        zs = zw + h + rng.uniform(1e-3, 1.0)
+        n = rng.uniform(0.0, 0.05)
+        c = np.sqrt(G * h / 2.0)
+        F, Sm, Sp = O.roe_face([0, 0, 0, zs, n], [h, 0, 0, zw, n], 1e-3, 0.01, 1.0, 0.0)
+        np.testing.assert_allclose(F - Sm, [-c * h, 0.0, 0.0], rtol=1e-11, atol=1e-12 * G * h * h)
+        np.testing.assert_allclose(F + Sp, [-c * h, 0.5 * G * h * h, 0.0], rtol=1e-11,
+                                   atol=1e-12 * G * h * h)
+
+
+def star_depths(hi, ui, hj, uj, zj, n, dx):
+    """The reference's star depths h1* = h_i + alpha0 - beta0/l0 and
+    h3* = h_j - alpha2 + beta2/l2 (kernels/Roe.H:209-210) read back from one
+    face's outputs.  With l0 < 0 < l2 and the entropy fix off, S+ = beta0 e0
+    and S- = beta2 e2 with e = (1, l, v~) (so l = S[1]/S[0]), and the Roe
+    property f_j - f_i = sum l_k alpha_k e_k makes F0 = q_i + l0 alpha0 =
+    q_j - l2 alpha2.  So h1* = h_i + (F0 - q_i - S+0) / l0 and
+    h3* = h_j - (q_j - F0 - S-0) / l2: the depths the guard limits, with
+    whatever beta the face ended with."""
+    F, Sm, Sp = O.roe_face([hi, hi * ui, 0, 0, n], [hj, hj * uj, 0, zj, n], 1e-3, dx, 1.0, 0.0)
+    l0, l2 = Sp[1] / Sp[0], Sm[1] / Sm[0]
+    assert l0 < 0.0 < l2
+    h1 = hi + (F[0] - hi * ui - Sp[0]) / l0
+    h3 = hj - (hj * uj - F[0] - Sm[0]) / l2
+    return h1, h3
+
+
+# (h_i, u_i, h_j, u_j, z_j, n, dx), z_i = 0, same roughness on both sides.
+# Rough diverging or converging flow over a small step, entropy fix off and
+# |l0|, |l2| > 0.1.  GUARD_LEFT: without the guard h1* < -0.8 while h3* > 0.1;
+# GUARD_RIGHT: h3* < -1 while h1* > 0.1, and the 3-wave reaches the cell
+# within the face's local step dx / max|l|; LATE_RIGHT: the same sign pattern
+# but the 3-wave would arrive after 1.2-2.5 local steps, so the guard holds back.
+GUARD_LEFT = [(0.9, -0.7, 1.68, 2.8, 0.09, 0.23, 50.0), (0.23, 1.0, 0.94, 2.4, 0.37, 0.24, 1.0),
+              (0.76, 1.2, 0.13, -1.3, -0.01, 0.27, 50.0)]
+GUARD_RIGHT = [(0.3, 1.9, 0.63, -2.4, 0.1, 0.22, 50.0), (0.91, -2.6, 1.93, -1.7, 0.06, 0.23, 50.0),
+               (0.23, -1.3, 1.12, -2.4, -0.5, 0.27, 50.0)]
+LATE_RIGHT = [(0.31, -1.2, 0.74, -2.5, -0.26, 0.16, 1.0), (0.91, -0.6, 1.92, -3.0, -0.36, 0.11, 50.0),
+              (0.17, 0.7, 1.74, -2.1, -0.27, 0.1, 50.0)]
+
+
+@pytest.mark.parametrize("case", GUARD_LEFT)
+def test_roe_positivity_guard_empties_the_left_cell(case):
+    """Where the source share would drive the left star depth negative while
+    the right one stays wet, the guard limits beta0 to the value that just
+    empties the cell (kernels/Roe.H:228-236): h1* = 0 to round-off."""
+    h1, h3 = star_depths(*case)
+    assert abs(h1) <= 1e-11 * max(case[0], case[2]), h1
+    assert h3 > 1e-6
+
+
+@pytest.mark.parametrize("case", GUARD_RIGHT)
+def test_roe_positivity_guard_empties_the_right_cell(case):
+    """The mirrored branch (kernels/Roe.H:238-246) limits beta2: h3* = 0."""
+    h1, h3 = star_depths(*case)
+    assert abs(h3) <= 1e-11 * max(case[0], case[2]), h3
+    assert h1 > 1e-6
+
+
+@pytest.mark.parametrize("case", LATE_RIGHT)
+def test_roe_positivity_guard_waits_for_waves_beyond_the_local_step(case):
+    """A negative star depth whose wave would not reach the cell within the
+    local step (dt3* >= dx / max|l|) is left alone: the unlimited h3* stays."""
+    h1, h3 = star_depths(*case)
+    assert h3 < -0.2, h3
+    assert h1 > 1e-6
 
 
 # ---------------------------------------------------------- homogeneous Roe
