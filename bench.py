@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -40,10 +41,12 @@ def parse():
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--solver", choices=["Roe", "HydrostaticRoe"], default="Roe")
     ap.add_argument("--rows", type=int, default=0, help="strip rows per wavefront (0 = default)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="budget of the CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="budget of the five timed CPU-baseline samples, per thread count")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extra", action="store_true", help="also time the other workloads")
+    ap.add_argument("--no-extra", dest="extra", action="store_false",
+                    help="skip the other advance workloads (bumpy bed, FP32, lakes at rest)")
+    ap.add_argument("--cpu-worker", nargs=2, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
     ap.add_argument("--only", choices=["ensemble", "adjoint", "two_level", "amr"], default=None,
                     help="run one secondary leg alone and print its JSON (for profiling)")
@@ -58,45 +61,79 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(w, budget_s):
-    """Oracle (FP64 C/OpenMP restatement) on the GPU host: whole steps of the
-    same workload until the budget is spent (at least 2)."""
+def _cpu_worker(n: int, budget_s: float) -> None:
+    """Child process of cpu_baseline (no torch, no HIP): time the oracle on
+    the n x n dam break and print one JSON line.  One warm-up step (which
+    also sizes the samples), then 5 samples of k whole steps each, k chosen
+    so the samples take about budget_s together; the median sample counts."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "hx_workloads", os.path.join(ROOT, "hydroexa_amd", "workloads.py"))
+    W = importlib.util.module_from_spec(spec)
+    sys.modules["hx_workloads"] = W          # dataclasses look their module up
+    spec.loader.exec_module(W)
     import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    solver = O.ROE if w.solver == "Roe" else O.HYDROSTATIC_ROE
+    w = W.dam_break(n, 1)
     args = (w.z, w.n, w.dx, w.dy, w.cfl)
     t0 = time.perf_counter()
-    O.run(solver, w.h, w.hu, w.hv, *args, 2, w.bc_lo, w.bc_hi)      # warm-up and calibration
-    per_step = (time.perf_counter() - t0) / 2
-    steps = int(min(1000, max(2, budget_s / max(per_step, 1e-9))))
-    t0 = time.perf_counter()
-    O.run(solver, w.h, w.hu, w.hv, *args, steps, w.bc_lo, w.bc_hi)
-    el = time.perf_counter() - t0
-    out = {"value": w.cells * steps / el / 1e6, "unit": "Mcell-updates/s", "cores": threads,
-           "kind": "port",
-           "sample": f"{steps} steps of the {w.name} workload ({w.cells} cells) with the oracle, "
-                     f"{el:.1f} s, OMP threads {threads}"}
-    # the same oracle on one core (two whole steps), and the host's CPU model
+    O.run(O.ROE, w.h, w.hu, w.hv, *args, 1, w.bc_lo, w.bc_hi)
+    per_step = time.perf_counter() - t0
+    k = int(max(1, min(200, round(budget_s / 5 / max(per_step, 1e-9)))))
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.run(O.ROE, w.h, w.hu, w.hv, *args, k, w.bc_lo, w.bc_hi)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    print(json.dumps({"value": w.cells * k / med / 1e6, "steps_per_sample": k,
+                      "sample_s": times, "name": w.name, "cells": w.cells}), flush=True)
+
+
+def _cpu_model() -> str:
     try:
-        import ctypes
-        gomp = ctypes.CDLL("libgomp.so.1")
-        gomp.omp_set_num_threads(1)
-        try:
-            t0 = time.perf_counter()
-            O.run(solver, w.h, w.hu, w.hv, *args, 2, w.bc_lo, w.bc_hi)
-            el1 = time.perf_counter() - t0
-        finally:
-            gomp.omp_set_num_threads(threads)
-        out["one_core"] = {"value": w.cells * 2 / el1 / 1e6, "sample": f"2 steps, {el1:.1f} s, OMP threads 1"}
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
         pass
     try:
         with open("/proc/cpuinfo") as f:
-            out["cpu_model"] = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+            return next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
     except (OSError, StopIteration):
-        pass
-    return out
+        return "unknown"
+
+
+def cpu_baseline(n, budget_s):
+    """The oracle (FP64 C/OpenMP restatement) on the host cores, in a child
+    process so that the OpenMP placement takes effect: OMP_NUM_THREADS
+    threads (the host's affinity when unset), OMP_PROC_BIND=close,
+    OMP_PLACES=cores; then the same on one thread.  Median of 5 samples after
+    a warm-up step (the baseline protocol of the measurement plan)."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+
+    def run(nthreads, budget):
+        env = dict(os.environ, OMP_NUM_THREADS=str(nthreads), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        res = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                              str(n), str(budget)], env=env, capture_output=True, text=True,
+                             timeout=600)
+        if res.returncode != 0:
+            raise RuntimeError("CPU baseline worker failed: " + res.stderr[-2000:])
+        return json.loads(res.stdout.strip().splitlines()[-1])
+
+    many = run(threads, budget_s)
+    one = run(1, budget_s)
+    k, k1 = many["steps_per_sample"], one["steps_per_sample"]
+    return {"value": many["value"], "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"median of 5 samples of {k} steps of the {many['name']} workload "
+                      f"({many['cells']} cells) after a 1-step warm-up, oracle with {threads} "
+                      f"OpenMP threads (OMP_PROC_BIND=close, OMP_PLACES=cores); samples "
+                      + ", ".join(f"{t:.2f}" for t in many["sample_s"]) + " s",
+            "one_core": {"value": one["value"], "cores": 1,
+                         "sample": f"median of 5 samples of {k1} steps on one thread; samples "
+                                   + ", ".join(f"{t:.2f}" for t in one["sample_s"]) + " s"},
+            "cpu_model": _cpu_model()}
 
 
 def time_gpu(w, dtype, steps, warmup, rows, arith="fast", keep=False):
@@ -114,7 +151,9 @@ def time_gpu(w, dtype, steps, warmup, rows, arith="fast", keep=False):
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record(st)
-    dth, vh = sv.run(lv, w.cfl, steps)
+    # the timed steps continue the warm-up rollout: the first dt comes from
+    # the estimate the last warm-up step reduced (no standalone CFL pass)
+    dth, vh = sv.run(lv, w.cfl, steps, resume=warmup > 0)
     e1.record(st)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -392,55 +431,52 @@ def time_two_level(steps, arith="fast"):
 
 
 def time_amr_c3(steps, arith="fast"):
-    """The cylinder dam break on an AMR hierarchy (two refined levels at
-    ratio 2): 512^2 base on [0,1]^2, h = 2 inside r = 0.2 around the centre,
-    0.5 outside, bed 0.05 U[0,1) (seeded), Manning 0.03, hydrostatic Roe,
-    transmissive sides, tags where |grad h| > 0.1, regrid every 4 coarse
-    steps, blocking factor 16, subcycled, FP64.  Times `steps` coarse steps
-    (after 4 warm-up steps) on the wall clock, whose timed region ends with a
-    synchronisation, and counts composite cell updates (every level's cells
-    times its substeps)."""
+    """Config C3 on an AMR hierarchy (workloads.circular_dam_break): 512^2
+    base on [0,1]^2, depth 2.0 inside r = 0.2 around the centre and 0.5
+    outside, bed 0.05 U[0,1) (SplitMix64), Manning 0.03, hydrostatic Roe,
+    transmissive sides, two refined levels at ratio 2, tags where
+    |grad h| > 0.1, regrid every 4 steps of a level, blocking factor 16,
+    subcycled, FP64.  Times `steps` coarse steps (after 4 warm-up steps) on
+    the wall clock in calls of 2 coarse steps (level 1 regrids level 2 every
+    2 coarse steps), each ending with a synchronisation, and counts composite
+    cell updates (every level's cells times its substeps) on the grids at the
+    start of each call."""
     import torch
     from hydroexa_amd import set_arith
+    from hydroexa_amd import workloads as W
     from hydroexa_amd.amr import AmrHierarchy
 
     set_arith(arith)
-    n = 512
-    dx = 1.0 / n
-    xc = (np.arange(n) + 0.5) * dx
-    X, Y = np.meshgrid(xc, xc)
-    rng = np.random.default_rng(3)
-    z = 0.05 * rng.uniform(size=(n, n))
-    eta = np.where(np.hypot(X - 0.5, Y - 0.5) < 0.2, 2.0, 0.5)
-    h = np.maximum(eta - z, 0.0)
-    zero = np.zeros_like(h)
-    H = AmrHierarchy("HydrostaticRoe", h, zero, zero, z, np.full_like(h, 0.03), dx, dx,
-                     (1, 1), (1, 1), max_level=2, regrid_int=4, blocking_factor=16,
-                     max_grid_size=128, tag_thresh=[(0.1, 1e300, 1e300)])
-    H.run_native(1e30, 0.45, max_steps=4)
+    w = W.circular_dam_break(512)
+    H = AmrHierarchy(w.solver, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi,
+                     max_level=2, regrid_int=4, blocking_factor=16, max_grid_size=128,
+                     tag_thresh=[(0.1, 1e300, 1e300)])
+    H.run_native(1e30, w.cfl, max_steps=4)
     torch.cuda.synchronize()
-    # composite cell updates: every level's cells times its substeps per
-    # coarse step, counted on the grids of each step (host side, untimed)
     cells = 0
     wall = 0.0
     done = 0
+    boxes_seen = []
     while done < steps:
-        k = min(4, steps - done)            # one regrid interval per call
+        k = min(2, steps - done)
         per = sum(lv.interior(lv.mask).sum().item() * (2 ** l if H.subcycle else 1)
                   for l, lv in enumerate(H.levels))
+        boxes_seen.append([len(lv.boxes) for lv in H.levels])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        H.run_native(1e30, 0.45, max_steps=k)
+        H.run_native(1e30, w.cfl, max_steps=k)
         torch.cuda.synchronize()
         wall += time.perf_counter() - t0
         cells += per * k
         done += k
     set_arith("exact")
-    out = {"workload": "cylinder dam break 512^2 base, max_level 2 at ratio 2, hydrostatic Roe + "
-                       "Manning 0.03, random bed, tags |grad h| > 0.1, regrid_int 4, subcycled, FP64",
+    out = {"workload": "circular dam break (config C3): 512^2 base, max_level 2 at ratio 2, "
+                       "depth 2.0 / 0.5, bed 0.05 U[0,1), hydrostatic Roe + Manning 0.03, "
+                       "transmissive, tags |grad h| > 0.1, regrid_int 4, subcycled, FP64",
            "arith": arith, "coarse_steps": steps, "wall_ms_per_coarse_step": wall / steps * 1e3,
            "composite_cell_updates_per_step": cells / steps,
            "composite_Mcups": cells / wall / 1e6,
+           "boxes_per_level_first": boxes_seen[0] if boxes_seen else None,
            "boxes_per_level": [len(lv.boxes) for lv in H.levels],
            "cells_per_level": [int(lv.interior(lv.mask).sum().item()) for lv in H.levels],
            "time": H.time}
@@ -451,6 +487,9 @@ def time_amr_c3(steps, arith="fast"):
 
 def main():
     args = parse()
+    if args.cpu_worker:
+        _cpu_worker(int(args.cpu_worker[0]), float(args.cpu_worker[1]))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "note only (halo exchange + 8-byte dt min all-reduce)",
@@ -552,8 +591,7 @@ def main():
     if args.amr_steps > 0:
         out["amr_c3"] = time_amr_c3(args.amr_steps, args.arith)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(W.dam_break(args.n, 1) if args.solver == "Roe" else w,
-                                           args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -499,6 +499,7 @@ struct hx_adjoint {
                        // before friction, stored by the segment's recomputation
     double* lam;       // 2 states: adjoint ping-pong
     double* lun;       // 1 state: adjoint before friction (hydrostatic)
+    double* hist;      // 2 nsteps: dt and max|vel| of the forward's CFL steps
     std::vector<double> dts;
     const double* z;
     const double* n;
@@ -507,7 +508,7 @@ struct hx_adjoint {
 
 static void adj_free(hx_adjoint* a)
 {
-    double* bufs[] = {a->ckpt, a->seg, a->lam, a->lun, a->un};
+    double* bufs[] = {a->ckpt, a->seg, a->lam, a->lun, a->un, a->hist};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (a->gauges) (void)hipFree(a->gauges);
@@ -558,7 +559,8 @@ extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, in
     const size_t S = sizeof(double) * 3 * (size_t)N;
     bool ok = hipMalloc(&a->ckpt, S * a->ncp) == hipSuccess &&
               hipMalloc(&a->seg, S * a->nseg) == hipSuccess &&
-              hipMalloc(&a->lam, 2 * S) == hipSuccess;
+              hipMalloc(&a->lam, 2 * S) == hipSuccess &&
+              hipMalloc(&a->hist, sizeof(double) * 2 * (size_t)a->nsteps) == hipSuccess;
     if (ok && solver == HX_SOLVER_HYDROSTATIC_ROE)
         ok = hipMalloc(&a->lun, S) == hipSuccess && hipMalloc(&a->un, S * a->every) == hipSuccess;
     if (ok && ngauge > 0) {
@@ -658,9 +660,8 @@ extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl,
     s.p = host::view_of<double>(st);
     s.opitch = a->nx;
     host::fill_geometry(s, a->solver, host::eff_rows(c, a->solver, HX_F64, a->nx, a->ny), a->dx, a->dy);
+    c->mark.valid = 0;       // the context's slots now follow this rollout
     if (use_cfl) {
-        rc = host::ensure_hist(c, a->nsteps);
-        if (rc) return rc;
         rc = host::clear_slots(c->slots, 3, c->err, hs);
         if (rc) return rc;
         hx_patch p0 = *st;
@@ -671,8 +672,8 @@ extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl,
         if (rc) return rc;
         s.cfl = cfl;
         s.err = c->err;
-        s.dt_hist = c->hist;
-        s.vel_hist = c->hist + c->hist_cap;
+        s.dt_hist = a->hist;
+        s.vel_hist = a->hist + a->nsteps;
     }
     const double* in = c0;
     for (int k = 0; k < a->nsteps; ++k) {
@@ -710,7 +711,7 @@ extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl,
     if (use_cfl) {
         int err = 0;
         HX_TRY(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, hs));
-        HX_TRY(hipMemcpyAsync(a->dts.data(), c->hist, sizeof(double) * a->nsteps,
+        HX_TRY(hipMemcpyAsync(a->dts.data(), a->hist, sizeof(double) * a->nsteps,
                               hipMemcpyDeviceToHost, hs));
         HX_TRY(hipStreamSynchronize(hs));
         if (err) {

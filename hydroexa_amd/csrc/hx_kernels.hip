@@ -532,11 +532,19 @@ extern "C" int hx_ctx_create(hx_ctx** out, int device)
     hx_ctx* c = new (std::nothrow) hx_ctx();
     if (!c) return HX_E_NOMEM;
     c->device = device;
+    c->hist_cap = kHistCap;
     if (hipMalloc(&c->slots, 4 * sizeof(CflSlot)) != hipSuccess ||
-        hipMalloc(&c->err, sizeof(int)) != hipSuccess) {
+        hipMalloc(&c->err, sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->hist, sizeof(double) * 2 * kHistCap) != hipSuccess ||
+        hipMalloc(&c->tlw, sizeof(TwoLevelWork)) != hipSuccess) {
+        hipFree(c->slots);
+        hipFree(c->err);
+        hipFree(c->hist);
+        hipFree(c->tlw);
         delete c;
         return HX_E_NOMEM;
     }
+    c->mark.valid = 0;
     for (int s = 0; s < 2; ++s)
         for (int d = 0; d < 2; ++d) {
             c->rows[s][d] = 0;
@@ -557,7 +565,8 @@ extern "C" int hx_ctx_destroy(hx_ctx* c)
     hipSetDevice(c->device);
     hipFree(c->slots);
     hipFree(c->err);
-    if (c->hist) hipFree(c->hist);
+    hipFree(c->hist);
+    hipFree(c->tlw);
     if (c->pflags) hipFree(c->pflags);
     if (c->pboxes) hipFree(c->pboxes);
     if (c->cons) hipFree(c->cons);
@@ -598,7 +607,8 @@ int hx::host::check_patch(const hx_patch* in, int need_ghost)
     }
     const int64_t gx = (in->bc_lo[0] == HX_BC_GHOST ? in->ng : 0) + (in->bc_hi[0] == HX_BC_GHOST ? in->ng : 0);
     if (in->pitch < (int64_t)in->nx + gx) return HX_E_BADARG;
-    // the advance addresses 5 rows of a strip with 32-bit byte offsets
+    // the advance addresses a strip with 32-bit byte offsets: at this pitch
+    // bound an FP64 strip of up to 3 rows fits (max_strip_rows)
     if (in->pitch > ((int64_t)1 << 26)) return HX_E_BADARG;
     return HX_OK;
 }
@@ -642,7 +652,8 @@ extern "C" int hx_advance_level(hx_ctx* c, int solver, const hx_patch* in, doubl
     if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
     int rc = check_patch(in, solver == HX_SOLVER_ROE ? 1 : 2);
     if (rc) return rc;
-    if (!ho || !huo || !hvo || opitch < in->nx || !(dt >= 0.0)) return HX_E_BADARG;
+    if (!ho || !huo || !hvo || opitch < in->nx || opitch > ((int64_t)1 << 26) || !(dt >= 0.0))
+        return HX_E_BADARG;
     HX_TRY(hipSetDevice(c->device));
     hipStream_t st = (hipStream_t)stream;
     return (in->dtype == HX_F64) ? advance_t<double>(c, solver, in, dt, ho, huo, hvo, opitch, st)
@@ -679,19 +690,6 @@ int hx::host::clear_slots(CflSlot* s, int n, int* err, hipStream_t st)
 }
 
 // Grow the context's dt / max|vel| history to hold nsteps entries.
-int hx::host::ensure_hist(hx_ctx* c, int nsteps)
-{
-    if (nsteps <= c->hist_cap) return HX_OK;
-    if (c->hist) hipFree(c->hist);
-    c->hist = nullptr;
-    c->hist_cap = 0;
-    int cap = 1024;
-    while (cap < nsteps) cap *= 2;
-    if (hipMalloc(&c->hist, sizeof(double) * 2 * cap) != hipSuccess) return HX_E_NOMEM;
-    c->hist_cap = cap;
-    return HX_OK;
-}
-
 extern "C" int hx_level_dt(hx_ctx* c, int solver, const hx_patch* in, double cfl, double* dt_out,
                            double* max_vel_out, void* stream)
 {
@@ -895,9 +893,14 @@ extern "C" int hx_stream_probe(hx_ctx* c, int64_t n, const void* const* in, void
     return hipGetLastError() == hipSuccess ? HX_OK : HX_E_HIP;
 }
 
+// Steps [s0, s1) of a rollout whose step 0 reads slot `base`: step s reads
+// slot (base + s) % 3, reduces its output into the next slot and clears the
+// one after.  History and error indices are relative to s0 (the context's
+// history holds kHistCap steps).  When s0 == 0 and fresh, the slots and the
+// error flag are reset and the initial state's CFL is reduced into `base`.
 template <typename R>
 static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu, void* ahv,
-                 double cfl, int nsteps, hipStream_t st)
+                 double cfl, int s0, int s1, int base, bool fresh, hipStream_t st)
 {
     StepArgs<R> a{};
     a.p = view_of<R>(in);
@@ -907,13 +910,14 @@ static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
     a.dt_hist = c->hist;
     a.vel_hist = c->hist + c->hist_cap;
     fill_geometry(a, solver, eff_rows(c, solver, in->dtype, in->nx, in->ny), in->dx, in->dy);
-    // reset slots and the error flag, then reduce the initial state's CFL
-    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots, 3, c->err);
-    int rc = launch_cfl<R>(in, c->slots, st);
-    if (rc) return rc;
+    if (s0 == 0 && fresh) {
+        hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots, 3, c->err);
+        const int rc = launch_cfl<R>(in, c->slots + base, st);
+        if (rc) return rc;
+    }
     R* bufA[3] = {(R*)in->h, (R*)in->hu, (R*)in->hv};
     R* bufB[3] = {(R*)ah, (R*)ahu, (R*)ahv};
-    for (int s = 0; s < nsteps; ++s) {
+    for (int s = s0; s < s1; ++s) {
         R** src = (s & 1) ? bufB : bufA;
         R** dst = (s & 1) ? bufA : bufB;
         a.p.h = src[0];
@@ -922,19 +926,52 @@ static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
         a.ho = dst[0];
         a.huo = dst[1];
         a.hvo = dst[2];
-        a.rd = c->slots + (s % 3);
-        a.acc = c->slots + ((s + 1) % 3);
-        a.clr = c->slots + ((s + 2) % 3);
-        a.step = s;
-        rc = launch(c, solver, in->dtype, a, st);
+        a.rd = c->slots + ((base + s) % 3);
+        a.acc = c->slots + ((base + s + 1) % 3);
+        a.clr = c->slots + ((base + s + 2) % 3);
+        a.step = s - s0;
+        const int rc = launch(c, solver, in->dtype, a, st);
         if (rc) return rc;
     }
     return HX_OK;
 }
 
-extern "C" int hx_run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
-                            void* ahv, double cfl, int nsteps, double* dt_hist, double* vel_hist,
-                            int32_t* steps_done, void* stream)
+static void set_mark(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu, void* ahv,
+                     int nsteps, int base)
+{
+    RunMark& m = c->mark;
+    m.valid = 1;
+    m.solver = solver;
+    m.dtype = in->dtype;
+    m.nx = in->nx;
+    m.ny = in->ny;
+    m.pitch = in->pitch;
+    m.dx = in->dx;
+    m.dy = in->dy;
+    m.bc[0] = in->bc_lo[0]; m.bc[1] = in->bc_lo[1];
+    m.bc[2] = in->bc_hi[0]; m.bc[3] = in->bc_hi[1];
+    const bool odd = nsteps & 1;
+    m.h = odd ? ah : in->h;
+    m.hu = odd ? ahu : in->hu;
+    m.hv = odd ? ahv : in->hv;
+    m.z = in->z;
+    m.n = in->n;
+    m.slot = (base + nsteps) % 3;
+}
+
+static bool mark_matches(const hx_ctx* c, int solver, const hx_patch* in)
+{
+    const RunMark& m = c->mark;
+    return m.valid && m.solver == solver && m.dtype == in->dtype && m.nx == in->nx &&
+           m.ny == in->ny && m.pitch == in->pitch && m.dx == in->dx && m.dy == in->dy &&
+           m.bc[0] == in->bc_lo[0] && m.bc[1] == in->bc_lo[1] && m.bc[2] == in->bc_hi[0] &&
+           m.bc[3] == in->bc_hi[1] && m.h == in->h && m.hu == in->hu && m.hv == in->hv &&
+           m.z == in->z && m.n == in->n;
+}
+
+static int run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu, void* ahv,
+                     double cfl, int nsteps, double* dt_hist, double* vel_hist,
+                     int32_t* steps_done, void* stream, bool resume)
 {
     if (!c) return HX_E_STATE;
     if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
@@ -943,29 +980,53 @@ extern "C" int hx_run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah,
     for (int d = 0; d < 2; ++d)
         if (in->bc_lo[d] == HX_BC_GHOST || in->bc_hi[d] == HX_BC_GHOST) return HX_E_BADARG;
     if (in->eta || !ah || !ahu || !ahv || nsteps < 0 || !(cfl > 0.0)) return HX_E_BADARG;
+    if (ah == in->h || ahu == in->hu || ahv == in->hv) return HX_E_BADARG;
+    if (resume && !mark_matches(c, solver, in)) return HX_E_STATE;
     if (steps_done) *steps_done = 0;
     if (nsteps == 0) return HX_OK;
     HX_TRY(hipSetDevice(c->device));
     hipStream_t st = (hipStream_t)stream;
-    rc = ensure_hist(c, nsteps);
-    if (rc) return rc;
-    rc = (in->dtype == HX_F64) ? run_t<double>(c, solver, in, ah, ahu, ahv, cfl, nsteps, st)
-                               : run_t<float>(c, solver, in, ah, ahu, ahv, cfl, nsteps, st);
-    if (rc) return rc;
-    int err = 0;
-    HX_TRY(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
-    if (dt_hist)
-        HX_TRY(hipMemcpyAsync(dt_hist, c->hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, st));
-    if (vel_hist)
-        HX_TRY(hipMemcpyAsync(vel_hist, c->hist + c->hist_cap, sizeof(double) * nsteps,
-                              hipMemcpyDeviceToHost, st));
-    HX_TRY(hipStreamSynchronize(st));
-    if (err) {
-        if (steps_done) *steps_done = err - 1;
-        return HX_E_DT_COLLAPSE;
+    const int base = resume ? c->mark.slot : 0;
+    c->mark.valid = 0;
+    for (int s0 = 0; s0 < nsteps; s0 += c->hist_cap) {
+        const int s1 = std::min(nsteps, s0 + c->hist_cap);
+        rc = (in->dtype == HX_F64)
+                 ? run_t<double>(c, solver, in, ah, ahu, ahv, cfl, s0, s1, base, !resume, st)
+                 : run_t<float>(c, solver, in, ah, ahu, ahv, cfl, s0, s1, base, !resume, st);
+        if (rc) return rc;
+        int err = 0;
+        HX_TRY(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        if (dt_hist)
+            HX_TRY(hipMemcpyAsync(dt_hist + s0, c->hist, sizeof(double) * (s1 - s0),
+                                  hipMemcpyDeviceToHost, st));
+        if (vel_hist)
+            HX_TRY(hipMemcpyAsync(vel_hist + s0, c->hist + c->hist_cap, sizeof(double) * (s1 - s0),
+                                  hipMemcpyDeviceToHost, st));
+        HX_TRY(hipStreamSynchronize(st));
+        if (err) {
+            if (steps_done) *steps_done = s0 + err - 1;
+            return HX_E_DT_COLLAPSE;
+        }
     }
+    set_mark(c, solver, in, ah, ahu, ahv, nsteps, base);
     if (steps_done) *steps_done = nsteps;
     return HX_OK;
+}
+
+extern "C" int hx_run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
+                            void* ahv, double cfl, int nsteps, double* dt_hist, double* vel_hist,
+                            int32_t* steps_done, void* stream)
+{
+    return run_level(c, solver, in, ah, ahu, ahv, cfl, nsteps, dt_hist, vel_hist, steps_done,
+                     stream, false);
+}
+
+extern "C" int hx_run_level_resume(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
+                                   void* ahv, double cfl, int nsteps, double* dt_hist,
+                                   double* vel_hist, int32_t* steps_done, void* stream)
+{
+    return run_level(c, solver, in, ah, ahu, ahv, cfl, nsteps, dt_hist, vel_hist, steps_done,
+                     stream, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1006,9 +1067,8 @@ static int ens_upload(hx_ensemble* e)
     for (int t = 0; t < e->nscen; ++t) {
         const int s = order[t];
         const hx_patch* in = &e->patches[s];
-        // strips are addressed with 32-bit byte offsets over rows + 4 rows
-        if ((int64_t)(e->rows + 4) * in->pitch * (int64_t)sizeof(R) >= ((int64_t)1 << 32))
-            return HX_E_BADARG;
+        // strips are addressed with 32-bit byte offsets (max_strip_rows)
+        if (e->rows > max_strip_rows<R>(in->pitch, in->pitch)) return HX_E_BADARG;
         ScenDesc<R>& x = d[t];
         std::memset(&x, 0, sizeof(x));
         x.p = view_of<R>(in);
@@ -1413,27 +1473,26 @@ __global__ void tl_dt_kernel(const CflSlot* rc, const CflSlot* rf, CflSlot* wc, 
     wf->amax = 0ull; wf->vmax = 0ull; wf->flags = 0u;
 }
 
-struct TwoLevelWork {
-    CflSlot slots[4];   // coarse read/write, fine read/write
-    double dt[2];
-    int err;
-    int pad;
-};
-
+// Coupled steps [s0, s1) (history and error indices relative to s0; the
+// context's history holds kHistCap steps); fcur: the fine buffer holding the
+// current fine state, carried across chunks.
 template <typename R>
 static int two_level_t(hx_ctx* c, int solver, const hx_patch* crse, void* const* calt,
                        const hx_patch* fine, void* const* falt, int ci0, int cj0, int subcycle,
-                       double cfl, int nsteps, void* reg, TwoLevelWork* wk, double* hist,
-                       hipStream_t st)
+                       double cfl, int s0, int s1, int& fcur, void* reg, TwoLevelWork* wk,
+                       double* hist, hipStream_t st)
 {
     const int ncx = fine->nx / 2, ncy = fine->ny / 2;
     CflSlot* slots = wk->slots;
     double* dt = wk->dt;
     int* err = &wk->err;
-    hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, slots, 4, err);
-    int rc = launch_cfl<R>(crse, slots + 0, st);
-    if (!rc) rc = launch_cfl<R>(fine, slots + 2, st);
-    if (rc) return rc;
+    int rc = HX_OK;
+    if (s0 == 0) {
+        hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, slots, 4, err);
+        rc = launch_cfl<R>(crse, slots + 0, st);
+        if (!rc) rc = launch_cfl<R>(fine, slots + 2, st);
+        if (rc) return rc;
+    }
     void* cb[2][3] = {{(void*)crse->h, (void*)crse->hu, (void*)crse->hv},
                       {calt[0], calt[1], calt[2]}};
     void* fb[2][3] = {{(void*)fine->h, (void*)fine->hu, (void*)fine->hv},
@@ -1462,11 +1521,10 @@ static int two_level_t(hx_ctx* c, int solver, const hx_patch* crse, void* const*
     qf.nfaces = 4 * (ncx + ncy); qf.pair = 2; qf.accumulate = 1; qf.box_fine = 1;
     qf.sx = fine->dy; qf.sy = fine->dx; qf.dt = dt + 1;
     const int nsub = subcycle ? 2 : 1;
-    int fcur = 0;
-    for (int s = 0; s < nsteps; ++s) {
+    for (int s = s0; s < s1; ++s) {
         const int r = s & 1, w = r ^ 1;
         hipLaunchKernelGGL(tl_dt_kernel<R>, dim3(1), dim3(64), 0, st, slots + r, slots + 2 + r,
-                           slots + w, slots + 2 + w, cfl, subcycle, s, dt, hist, err);
+                           slots + w, slots + 2 + w, cfl, subcycle, s - s0, dt, hist, err);
         hx_patch cold = *crse, cnew = *crse;
         cold.h = cb[r][0]; cold.hu = cb[r][1]; cold.hv = cb[r][2];
         cnew.h = cb[w][0]; cnew.hu = cb[w][1]; cnew.hv = cb[w][2];
@@ -1542,34 +1600,27 @@ extern "C" int hx_two_level_run(hx_ctx* c, int solver, const hx_patch* crse, voi
     if (nsteps == 0) return HX_OK;
     HX_TRY(hipSetDevice(c->device));
     hipStream_t st = (hipStream_t)stream;
-    TwoLevelWork* wk = nullptr;
-    double* hist = nullptr;
-    HX_TRY(hipMalloc((void**)&wk, sizeof(TwoLevelWork)));
-    if (hipMalloc((void**)&hist, sizeof(double) * 2 * (size_t)nsteps) != hipSuccess) {
-        (void)hipFree(wk);
-        return HX_E_HIP;
-    }
     void* calt[3] = {ch, chu, chv};
     void* falt[3] = {fh, fhu, fhv};
-    rc = (crse->dtype == HX_F64)
-             ? two_level_t<double>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0, cfl,
-                                   nsteps, reg, wk, hist, st)
-             : two_level_t<float>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0, cfl,
-                                  nsteps, reg, wk, hist, st);
-    int err = 0;
-    if (!rc && hipMemcpyAsync(&err, &wk->err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
-        rc = HX_E_HIP;
-    if (!rc && dt_hist &&
-        hipMemcpyAsync(dt_hist, hist, sizeof(double) * 2 * (size_t)nsteps, hipMemcpyDeviceToHost,
-                       st) != hipSuccess)
-        rc = HX_E_HIP;
-    if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = HX_E_HIP;
-    (void)hipFree(wk);
-    (void)hipFree(hist);
-    if (rc) return rc;
-    if (err) {
-        if (steps_done) *steps_done = err - 1;
-        return HX_E_DT_COLLAPSE;
+    int fcur = 0;
+    for (int s0 = 0; s0 < nsteps; s0 += c->hist_cap) {
+        const int s1 = std::min(nsteps, s0 + c->hist_cap);
+        rc = (crse->dtype == HX_F64)
+                 ? two_level_t<double>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0,
+                                       cfl, s0, s1, fcur, reg, c->tlw, c->hist, st)
+                 : two_level_t<float>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0,
+                                      cfl, s0, s1, fcur, reg, c->tlw, c->hist, st);
+        if (rc) return rc;
+        int err = 0;
+        HX_TRY(hipMemcpyAsync(&err, &c->tlw->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        if (dt_hist)
+            HX_TRY(hipMemcpyAsync(dt_hist + 2 * (size_t)s0, c->hist, sizeof(double) * 2 * (s1 - s0),
+                                  hipMemcpyDeviceToHost, st));
+        HX_TRY(hipStreamSynchronize(st));
+        if (err) {
+            if (steps_done) *steps_done = s0 + err - 1;
+            return HX_E_DT_COLLAPSE;
+        }
     }
     if (steps_done) *steps_done = nsteps;
     return HX_OK;

@@ -172,10 +172,13 @@ class SolverVariant:
         return tags
 
     # -- step-driven loop (HydroEXA::Compute for one level) ----------------
-    def run(self, level: Level, cfl: float, nsteps: int):
+    def run(self, level: Level, cfl: float, nsteps: int, resume: bool = False):
         """nsteps x (ComputeDt; AdvanceLevel) with the CFL reduction fused into
         the advance.  Leaves the final state in U_new; returns numpy arrays of
-        per-step dt and max|vel|."""
+        per-step dt and max|vel|.  resume: continue the previous run of this
+        level (U_new untouched since), whose last step already reduced the
+        first dt (hx_run_level_resume; raises if that run is not the
+        context's last)."""
         kind = self._kind()
         p = level.patch(level.U_new)
         es = level.U_new.element_size()
@@ -184,14 +187,13 @@ class SolverVariant:
         dth = np.zeros(max(nsteps, 1))
         vh = np.zeros(max(nsteps, 1))
         done = C.c_int32()
-        rc = L.load().hx_run_level(context(), kind, C.byref(p), alt, alt + plane, alt + 2 * plane,
-                                   float(cfl), int(nsteps),
-                                   dth.ctypes.data_as(C.POINTER(C.c_double)),
-                                   vh.ctypes.data_as(C.POINTER(C.c_double)), C.byref(done),
-                                   _stream())
+        fn = L.load().hx_run_level_resume if resume else L.load().hx_run_level
+        rc = fn(context(), kind, C.byref(p), alt, alt + plane, alt + 2 * plane, float(cfl),
+                int(nsteps), dth.ctypes.data_as(C.POINTER(C.c_double)),
+                vh.ctypes.data_as(C.POINTER(C.c_double)), C.byref(done), _stream())
         if done.value % 2 == 1:
             level.swap()
-        L.check(rc, f"hx_run_level (completed {done.value} of {nsteps} steps)")
+        L.check(rc, f"{fn.__name__} (completed {done.value} of {nsteps} steps)")
         return dth[:nsteps], vh[:nsteps]
 
 

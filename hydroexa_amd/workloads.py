@@ -97,18 +97,20 @@ def lake_at_rest(n: int = 1024, steps: int = 2000, cfl: float = 0.45, seed: int 
 
 def circular_dam_break(n: int = 512, steps: int = 1000, cfl: float = 0.45, seed: int = SEED,
                        manning: float = 0.03, bed_amp: float = 0.05,
-                       solver: str = "HydrostaticRoe") -> Workload:
-    """Cylinder of radius 0.2 at the centre, h = 2.0 inside and 0.5 outside
-    (on the free surface), a seeded random bed of amplitude ``bed_amp`` and a
-    uniform Manning coefficient; reflective walls."""
+                       solver: str = "HydrostaticRoe", bc: Tuple[int, int] = (1, 1)) -> Workload:
+    """Cylinder of radius 0.2 at the centre of the unit square: depth h = 2.0
+    inside (r < 0.2 at the cell centre) and 0.5 outside, at rest, over a
+    seeded random bed z = bed_amp U[0, 1) (SplitMix64, seed + 1) with a
+    uniform Manning coefficient.  Sides transmissive by default (the
+    reference's circular dam break uses outflow sides,
+    tests/dambreakcircle/inputs); bc = (0, 0) closes the box."""
     X, Y, dx, dy = _centres(n, n)
     z = bed_amp * splitmix64_uniform(seed + 1, n * n).reshape(n, n)
     r2 = (X - 0.5) ** 2 + (Y - 0.5) ** 2
-    eta = np.where(r2 < 0.04, 2.0, 0.5)
-    h = np.maximum(eta - z, 0.0)
+    h = np.where(r2 < 0.04, 2.0, 0.5)
     zero = np.zeros_like(h)
     return Workload(f"circular_dam_break_{n}x{n}_{solver}", solver, h, zero.copy(), zero.copy(),
-                    z, np.full_like(z, manning), dx, dy, (0, 0), (0, 0), cfl, steps)
+                    z, np.full_like(z, manning), dx, dy, tuple(bc), tuple(bc), cfl, steps)
 
 
 def boxes_of(nx: int, ny: int, box: int) -> np.ndarray:

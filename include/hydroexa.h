@@ -183,10 +183,29 @@ int hx_stream_probe(hx_ctx *ctx, int64_t n, const void *const *in, void *const *
  * alt buffers if odd.  dt_hist / vel_hist (host, optional, nsteps each)
  * receive dt and max|vel| per step.  Returns HX_E_DT_COLLAPSE if a step's
  * estimate fell below 1e-12 (*steps_done, host, says how many completed).
+ * Allocates nothing: the context's history holds 4096 steps and a longer
+ * run synchronises once per 4096 steps.
  */
 int hx_run_level(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
                  void *alt_hu, void *alt_hv, double cfl, int nsteps, double *dt_hist,
                  double *vel_hist, int32_t *steps_done, void *stream);
+
+/*
+ * Continue the rollout that the last successful hx_run_level or
+ * hx_run_level_resume on this context finished: the first step's dt comes
+ * from the CFL estimate of the final state that its last fused advance
+ * reduced (the reference's ComputeDt at the top of the next loop iteration,
+ * src/HydroEXA.cpp:260-271), so a run split into calls (output between
+ * them) costs exactly its fused advances and gives the bits of one long
+ * call.  state must name that call's final buffers (the alt buffers after
+ * an odd step count) with the same solver, geometry, terrain and BC codes,
+ * unchanged since; otherwise HX_E_STATE.  hx_adjoint_forward on the same
+ * context ends the rollout (it reuses the slots).  Other arguments and
+ * results as hx_run_level.  No reference counterpart beyond the loop.
+ */
+int hx_run_level_resume(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
+                        void *alt_hu, void *alt_hv, double cfl, int nsteps, double *dt_hist,
+                        double *vel_hist, int32_t *steps_done, void *stream);
 
 /*
  * Launch-shape control for the fused advance (the autotuner's knobs): rows

@@ -244,6 +244,26 @@ def test_two_level_run_equals_stepping(gpu, model, subcycle, dtype):
     assert torch.equal(A_.fine.valid(), B_.fine.valid())
 
 
+def test_two_level_run_longer_than_the_context_history(gpu):
+    """hx_two_level_run does runs longer than the context's 4096-step history
+    in chunks: one call of 4096 + 37 steps equals a call of 4096 then one of
+    37, bit for bit, with the same dt history."""
+    rng = np.random.default_rng(43)
+    st = wet_state(rng, 32, 32)
+    la = Level(*st, 2.0, 2.0, (0, 0), (0, 0))
+    lb = Level(*st, 2.0, 2.0, (0, 0), (0, 0))
+    A_ = TwoLevel("HydrostaticRoe", la, (10, 12, 8, 6))
+    B_ = TwoLevel("HydrostaticRoe", lb, (10, 12, 8, 6))
+    print("two-level chunked run: 4133 steps", flush=True)
+    hist = A_.run(4096 + 37, 0.45)
+    h1 = B_.run(4096, 0.45)
+    h2 = B_.run(37, 0.45)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(hist, np.concatenate([h1, h2]))
+    assert torch.equal(la.U_new, lb.U_new)
+    assert torch.equal(A_.fine.valid(), B_.fine.valid())
+
+
 def test_two_level_run_reports_dt_collapse(gpu):
     rng = np.random.default_rng(42)
     h, hu, hv, z, n = wet_state(rng, 32, 32)

@@ -411,8 +411,8 @@ def test_strip_height_leaves_results_unchanged(gpu, arith, model):
 def test_widest_pitch_shortens_strips(gpu, model):
     """The advance addresses a strip as base + 32-bit byte offsets, so at the
     largest accepted pitch (2^26 elements, 512 MiB per FP64 row) the host
-    shortens the strips to 4 rows: the result equals the same level stored
-    densely, bit for bit (10 rows, so strips of 4, 4 and 2 rows)."""
+    shortens the strips to 3 rows: the result equals the same level stored
+    densely, bit for bit (10 rows, so strips of 3, 3, 3 and 1 rows)."""
     import ctypes as C
     from hydroexa_amd.solver import context, _stream, MODELS
     ny, nx, pitch = 10, 150, 1 << 26
@@ -446,3 +446,99 @@ def test_widest_pitch_shortens_strips(gpu, model):
         torch.cuda.empty_cache()
     for k in range(3):
         np.testing.assert_array_equal(got[k], ref[k])
+
+
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_row_as_wide_as_the_widest_pitch(gpu, model):
+    """nx == pitch == 2^26 (512 MiB per FP64 row; the host keeps 3-row
+    strips so that the last column's halo offset stays below 2^32).  The
+    level's columns left of its last 150 repeat column nx - 150, so those
+    150 columns evolve as a 150-column level whose low x side is
+    transmissive: equal bit for bit."""
+    import ctypes as C
+    from hydroexa_amd.solver import context, _stream, MODELS
+    ny, w, nx = 10, 150, 1 << 26
+    st = random_state(np.random.default_rng(22), ny, w, 0.05, False, True)
+    lo, hi = (1, 1), (1, 0)
+    dx, dy = 1.0 / w, 1.0 / ny
+    dt = 0.4 * O.level_dt(*st[:4], dx, dy)[0]
+    ref = gpu_step(model, st, dt, dx, dy, lo, hi)
+    es = 8
+    big = torch.empty((5, ny, nx), dtype=torch.float64, device="cuda")     # 26.8 GB
+    out = torch.empty((3, ny, nx), dtype=torch.float64, device="cuda")     # 16.1 GB
+    try:
+        for k in range(5):
+            src = torch.as_tensor(st[k], device="cuda")
+            big[k, :, nx - w:] = src
+            big[k, :, :nx - w] = src[:, :1].expand(ny, nx - w)
+        p = L.HxPatch()
+        p.nx, p.ny, p.ng, p.dtype, p.pitch = nx, ny, 0, L.HX_F64, nx
+        base = big.data_ptr()
+        p.h, p.hu, p.hv, p.z, p.n = (base + k * ny * nx * es for k in range(5))
+        p.eta = None
+        p.dx, p.dy = dx, dy
+        p.bc_lo[0], p.bc_lo[1] = lo
+        p.bc_hi[0], p.bc_hi[1] = hi
+        plane = ny * nx * es
+        rc = L.load().hx_advance_level(context(), MODELS[model], C.byref(p), dt, out.data_ptr(),
+                                       out.data_ptr() + plane, out.data_ptr() + 2 * plane, nx,
+                                       _stream())
+        L.check(rc, "hx_advance_level")
+        got = out[:, :, nx - w:].cpu().numpy()
+    finally:
+        del big, out
+        torch.cuda.empty_cache()
+    for k in range(3):
+        np.testing.assert_array_equal(got[k], ref[k])
+
+
+@pytest.mark.parametrize("model,first", [("Roe", 7), ("Roe", 8), ("HydrostaticRoe", 5)])
+def test_resumed_run_equals_one_run(gpu, model, first):
+    """hx_run_level_resume continues from the CFL estimate the previous
+    call's last fused step reduced: first + rest steps in two calls give the
+    bits and dt history of one call of first + rest steps."""
+    w = W.dam_break(96, 1) if model == "Roe" else W.lake_at_rest(96, 1)
+    if model == "HydrostaticRoe":
+        w.h = w.h * np.where(np.arange(96)[None, :] < 40, 1.3, 1.0)   # a moving front
+    rest = 11
+    one = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    d_one, v_one = SolverVariant(model).run(one, w.cfl, first + rest)
+    two = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    sv = SolverVariant(model)
+    d1, v1 = sv.run(two, w.cfl, first)
+    d2, v2 = sv.run(two, w.cfl, rest, resume=True)
+    torch.cuda.synchronize()
+    assert torch.equal(one.U_new, two.U_new)
+    np.testing.assert_array_equal(np.concatenate([d1, d2]), d_one)
+    np.testing.assert_array_equal(np.concatenate([v1, v2]), v_one)
+
+
+def test_resume_requires_the_previous_rollout(gpu):
+    w = W.dam_break(64, 1)
+    a = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    b = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    sv = SolverVariant("Roe")
+    sv.run(a, w.cfl, 3)
+    sv.run(b, w.cfl, 2)
+    with pytest.raises(L.HydroExaError):
+        sv.run(a, w.cfl, 2, resume=True)           # b ran since
+    with pytest.raises(L.HydroExaError):
+        SolverVariant("HydrostaticRoe").run(b, w.cfl, 2, resume=True)   # another solver
+    sv.run(b, w.cfl, 2, resume=True)
+    b.swap()                                        # the caller moved the state
+    with pytest.raises(L.HydroExaError):
+        sv.run(b, w.cfl, 2, resume=True)
+
+
+def test_run_longer_than_the_context_history(gpu):
+    """A run of more steps than the context's 4096-step history is done in
+    chunks (one synchronisation each): the oracle's state and dt history."""
+    w = W.dam_break(24, 1)
+    steps = 4096 + 905
+    ref = O.run(O.ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.cfl, steps, w.bc_lo, w.bc_hi)
+    lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
+    dth, vh = SolverVariant("Roe").run(lv, w.cfl, steps)
+    for k in range(3):
+        assert rel_linf(lv.U_new[k].cpu().numpy(), ref[k]) <= 1e-12
+    np.testing.assert_array_equal(dth, ref[3])
+    np.testing.assert_array_equal(vh, ref[4])
