@@ -36,7 +36,7 @@ HX_ARITH_FAST = 1
 EXPORTS = (
     "hx_ctx_create", "hx_ctx_destroy", "hx_abi_version", "hx_status_string",
     "hx_advance_level", "hx_level_dt", "hx_tag_level", "hx_masked_average_down",
-    "hx_prune_boxes", "hx_run_level", "hx_run_level_resume", "hx_set_tuning", "hx_set_arith",
+    "hx_prune_boxes", "hx_run_level", "hx_run_level_resume", "hx_run_level_noalloc", "hx_set_tuning", "hx_set_arith",
     "hx_ensemble_create", "hx_ensemble_destroy", "hx_ensemble_set_tuning", "hx_ensemble_run",
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
     "hx_conservation", "hx_face_fluxes", "hx_fill_fine_ghosts", "hx_reflux",
@@ -108,6 +108,7 @@ def load() -> C.CDLL:
                                  u8p, u8p, i32p, vp]
     L.hx_run_level.argtypes = [vp, C.c_int, pp, vp, vp, vp, C.c_double, C.c_int, dp, dp, i32p, vp]
     L.hx_run_level_resume.argtypes = L.hx_run_level.argtypes
+    L.hx_run_level_noalloc.argtypes = L.hx_run_level.argtypes
     L.hx_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.hx_set_arith.argtypes = [vp, C.c_int]
     L.hx_ensemble_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, pp, C.POINTER(vp), C.POINTER(vp)]

@@ -44,6 +44,7 @@ struct hx_ctx {
     int hist_cap;
     hx::TwoLevelWork* tlw;   // hx_two_level_run's slots, dt pair and error flag
     hx::RunMark mark;
+    void* scratch;           // this device's static scratch (hx_run_level_noalloc)
     uint32_t* pflags;    // prune flags
     int32_t* pboxes;
     int pcap;

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -502,6 +503,35 @@ __global__ __launch_bounds__(kBlock) void avgdown_state_kernel(int ci0, int cj0,
 using namespace hx;
 using namespace hx::host;
 
+// Device memory the fused single-level loop works in besides the state: the
+// 3 rotating CFL slots, the error flag and a dt / max|vel| history of cap
+// steps (dt at hist[s], max|vel| at hist[cap + s]).
+struct RunScratch {
+    CflSlot* slots;
+    int* err;
+    double* hist;
+    int cap;
+};
+
+// The allocation-free loop's scratch (hx_run_level_noalloc): one copy per
+// device, held until the process ends.  Layout: 3 CflSlots at byte 0, the
+// error flag at byte 256, the history of kNoAllocSteps steps at byte 512.
+// Host code never uses this name as an address: hx_ctx_create resolves the
+// device copy's address with hipGetSymbolAddress once per device.
+constexpr int kNoAllocSteps = 8192;
+constexpr size_t kScratchBytes = 512 + sizeof(double) * 2 * kNoAllocSteps;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch[kMaxDevices];       // device address of scratch, per device
+static std::mutex g_scratch_mu;
+
+static RunScratch scratch_run(void* base)
+{
+    char* b = (char*)base;
+    return RunScratch{(CflSlot*)b, (int*)(b + 256), (double*)(b + 512), kNoAllocSteps};
+}
+static_assert(3 * sizeof(CflSlot) <= 256, "scratch layout");
+
 // Launch-shape defaults from the strip-height / occupancy sweep on MI355X
 // (4096^2 plain Roe and 2048^2 hydrostatic workloads): short strips keep
 // enough wavefronts in flight; the FP64 kernels run best when the register
@@ -545,6 +575,22 @@ extern "C" int hx_ctx_create(hx_ctx** out, int device)
         return HX_E_NOMEM;
     }
     c->mark.valid = 0;
+    if (device >= kMaxDevices) {
+        hx_ctx_destroy(c);
+        return HX_E_BADARG;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch[device]) {
+            void* ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+                hx_ctx_destroy(c);
+                return HX_E_HIP;
+            }
+            g_scratch[device] = ptr;
+        }
+        c->scratch = g_scratch[device];
+    }
     for (int s = 0; s < 2; ++s)
         for (int d = 0; d < 2; ++d) {
             c->rows[s][d] = 0;
@@ -900,19 +946,20 @@ extern "C" int hx_stream_probe(hx_ctx* c, int64_t n, const void* const* in, void
 // error flag are reset and the initial state's CFL is reduced into `base`.
 template <typename R>
 static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu, void* ahv,
-                 double cfl, int s0, int s1, int base, bool fresh, hipStream_t st)
+                 double cfl, int s0, int s1, int base, bool fresh, const RunScratch& sc,
+                 hipStream_t st)
 {
     StepArgs<R> a{};
     a.p = view_of<R>(in);
     a.opitch = in->pitch;
     a.cfl = cfl;
-    a.err = c->err;
-    a.dt_hist = c->hist;
-    a.vel_hist = c->hist + c->hist_cap;
+    a.err = sc.err;
+    a.dt_hist = sc.hist;
+    a.vel_hist = sc.hist + sc.cap;
     fill_geometry(a, solver, eff_rows(c, solver, in->dtype, in->nx, in->ny), in->dx, in->dy);
     if (s0 == 0 && fresh) {
-        hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, c->slots, 3, c->err);
-        const int rc = launch_cfl<R>(in, c->slots + base, st);
+        hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, sc.slots, 3, sc.err);
+        const int rc = launch_cfl<R>(in, sc.slots + base, st);
         if (rc) return rc;
     }
     R* bufA[3] = {(R*)in->h, (R*)in->hu, (R*)in->hv};
@@ -926,9 +973,9 @@ static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
         a.ho = dst[0];
         a.huo = dst[1];
         a.hvo = dst[2];
-        a.rd = c->slots + ((base + s) % 3);
-        a.acc = c->slots + ((base + s + 1) % 3);
-        a.clr = c->slots + ((base + s + 2) % 3);
+        a.rd = sc.slots + ((base + s) % 3);
+        a.acc = sc.slots + ((base + s + 1) % 3);
+        a.clr = sc.slots + ((base + s + 2) % 3);
         a.step = s - s0;
         const int rc = launch(c, solver, in->dtype, a, st);
         if (rc) return rc;
@@ -969,9 +1016,11 @@ static bool mark_matches(const hx_ctx* c, int solver, const hx_patch* in)
            m.z == in->z && m.n == in->n;
 }
 
+// noalloc: the loop works in the device's static scratch instead of the
+// context's buffers (at most kNoAllocSteps steps, no resume mark).
 static int run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu, void* ahv,
                      double cfl, int nsteps, double* dt_hist, double* vel_hist,
-                     int32_t* steps_done, void* stream, bool resume)
+                     int32_t* steps_done, void* stream, bool resume, bool noalloc = false)
 {
     if (!c) return HX_E_STATE;
     if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
@@ -982,25 +1031,29 @@ static int run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* 
     if (in->eta || !ah || !ahu || !ahv || nsteps < 0 || !(cfl > 0.0)) return HX_E_BADARG;
     if (ah == in->h || ahu == in->hu || ahv == in->hv) return HX_E_BADARG;
     if (resume && !mark_matches(c, solver, in)) return HX_E_STATE;
+    if (noalloc && !c->scratch) return HX_E_STATE;
+    if (noalloc && nsteps > kNoAllocSteps) return HX_E_BADARG;
     if (steps_done) *steps_done = 0;
     if (nsteps == 0) return HX_OK;
     HX_TRY(hipSetDevice(c->device));
     hipStream_t st = (hipStream_t)stream;
+    const RunScratch sc = noalloc ? scratch_run(c->scratch)
+                                  : RunScratch{c->slots, c->err, c->hist, c->hist_cap};
     const int base = resume ? c->mark.slot : 0;
-    c->mark.valid = 0;
-    for (int s0 = 0; s0 < nsteps; s0 += c->hist_cap) {
-        const int s1 = std::min(nsteps, s0 + c->hist_cap);
+    if (!noalloc) c->mark.valid = 0;
+    for (int s0 = 0; s0 < nsteps; s0 += sc.cap) {
+        const int s1 = std::min(nsteps, s0 + sc.cap);
         rc = (in->dtype == HX_F64)
-                 ? run_t<double>(c, solver, in, ah, ahu, ahv, cfl, s0, s1, base, !resume, st)
-                 : run_t<float>(c, solver, in, ah, ahu, ahv, cfl, s0, s1, base, !resume, st);
+                 ? run_t<double>(c, solver, in, ah, ahu, ahv, cfl, s0, s1, base, !resume, sc, st)
+                 : run_t<float>(c, solver, in, ah, ahu, ahv, cfl, s0, s1, base, !resume, sc, st);
         if (rc) return rc;
         int err = 0;
-        HX_TRY(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HX_TRY(hipMemcpyAsync(&err, sc.err, sizeof(int), hipMemcpyDeviceToHost, st));
         if (dt_hist)
-            HX_TRY(hipMemcpyAsync(dt_hist + s0, c->hist, sizeof(double) * (s1 - s0),
+            HX_TRY(hipMemcpyAsync(dt_hist + s0, sc.hist, sizeof(double) * (s1 - s0),
                                   hipMemcpyDeviceToHost, st));
         if (vel_hist)
-            HX_TRY(hipMemcpyAsync(vel_hist + s0, c->hist + c->hist_cap, sizeof(double) * (s1 - s0),
+            HX_TRY(hipMemcpyAsync(vel_hist + s0, sc.hist + sc.cap, sizeof(double) * (s1 - s0),
                                   hipMemcpyDeviceToHost, st));
         HX_TRY(hipStreamSynchronize(st));
         if (err) {
@@ -1008,7 +1061,7 @@ static int run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* 
             return HX_E_DT_COLLAPSE;
         }
     }
-    set_mark(c, solver, in, ah, ahu, ahv, nsteps, base);
+    if (!noalloc) set_mark(c, solver, in, ah, ahu, ahv, nsteps, base);
     if (steps_done) *steps_done = nsteps;
     return HX_OK;
 }
@@ -1019,6 +1072,15 @@ extern "C" int hx_run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah,
 {
     return run_level(c, solver, in, ah, ahu, ahv, cfl, nsteps, dt_hist, vel_hist, steps_done,
                      stream, false);
+}
+
+extern "C" int hx_run_level_noalloc(hx_ctx* c, int solver, const hx_patch* in, void* ah,
+                                    void* ahu, void* ahv, double cfl, int nsteps,
+                                    double* dt_hist, double* vel_hist, int32_t* steps_done,
+                                    void* stream)
+{
+    return run_level(c, solver, in, ah, ahu, ahv, cfl, nsteps, dt_hist, vel_hist, steps_done,
+                     stream, false, true);
 }
 
 extern "C" int hx_run_level_resume(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,

@@ -191,6 +191,23 @@ int hx_run_level(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
                  double *vel_hist, int32_t *steps_done, void *stream);
 
 /*
+ * hx_run_level for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc);
+ * it has no counterpart in the reference.  Same arguments, kernels and
+ * results as hx_run_level, but the CFL slots, error flag and dt history live
+ * in the library's static per-device scratch (about 130 KiB, resolved when
+ * the first context on the device is created) instead of the context's
+ * buffers, and nothing is allocated or freed.  At most 8192 steps per call
+ * (HX_E_BADARG beyond).  Every call on one GPU shares that scratch, which
+ * keeps the last call's contents: calls that overlap on one GPU (several
+ * host threads or streams) race.  Does not start or end a rollout that
+ * hx_run_level_resume could continue.
+ */
+int hx_run_level_noalloc(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
+                         void *alt_hu, void *alt_hv, double cfl, int nsteps, double *dt_hist,
+                         double *vel_hist, int32_t *steps_done, void *stream);
+
+/*
  * Continue the rollout that the last successful hx_run_level or
  * hx_run_level_resume on this context finished: the first step's dt comes
  * from the CFL estimate of the final state that its last fused advance
