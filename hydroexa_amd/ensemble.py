@@ -52,7 +52,9 @@ class Ensemble:
 
     def run(self, cfl: float, nsteps: int, graph: bool = True, stream: torch.cuda.Stream = None):
         """nsteps steps of every scenario; returns (dt history [nscen, nsteps],
-        steps done per scenario).  Graph capture needs a non-default stream:
+        steps done per scenario).  A scenario whose dt collapses stops and the
+        others go on; DtCollapse is raised after the run, with .done and
+        .dt_hist (a stopped scenario's state is not defined).  Graph capture needs a non-default stream:
         one is made when none is given.  The final state is left in each
         level's U_new."""
         n = len(self.levels)
@@ -75,6 +77,13 @@ class Ensemble:
         if rc in (L.HX_OK, L.HX_E_DT_COLLAPSE) and nsteps % 2 == 1:
             for lv in self.levels:
                 lv.swap()
+        if rc == L.HX_E_DT_COLLAPSE:
+            # the other scenarios ran every step: the exception carries the
+            # steps done per scenario and the dt history
+            e = L.DtCollapse(rc, "hx_ensemble_run (scenarios "
+                             f"{[int(s) for s in np.nonzero(done < nsteps)[0]]} stopped)")
+            e.done, e.dt_hist = done, dth[:, :nsteps]
+            raise e
         L.check(rc, "hx_ensemble_run")
         return dth[:, :nsteps], done
 

@@ -393,8 +393,20 @@ class AmrOracle:
     average_down, with the C oracle's ghost-fed step per box."""
 
     def __init__(self, solver, h, hu, hv, z, n, dx, dy, bc_lo, bc_hi, boxes, terrain=None,
-                 subcycle=True, do_reflux=True):
+                 subcycle=True, do_reflux=True, regrid_int=0, max_level=None, events=None):
+        """boxes: level -> box list at time 0.  With regrid_int > 0 the grids
+        change where the hierarchy's step regrids (AmrMeshState's step,
+        src/state/AmrMeshState_Physics.cpp:50-147, with AMReX's regrid_int
+        cadence): level l remakes the levels above it at the start of its
+        step when its step count is a positive multiple of regrid_int not yet
+        regridded.  The new boxes are not derived here: events is a list,
+        consumed in order, of (l, time, {level: boxes}) recorded from the run
+        under test; the oracle checks that each event happens where its
+        cadence says and remakes the levels as the hierarchy does."""
         self.solver, self.subcycle, self.do_reflux = solver, subcycle, do_reflux
+        self.regrid_int = regrid_int
+        self.max_level = max(boxes) if max_level is None else max_level
+        self.events = events if events is not None else []
         self.bc_lo, self.bc_hi = tuple(bc_lo), tuple(bc_hi)
         ny, nx = np.asarray(h).shape
         lv = DenseLevelOracle(nx, ny, dx, dy, bc_lo, bc_hi)
@@ -415,9 +427,11 @@ class AmrOracle:
             f.set_boxes(boxes[lev])
             self.levels.append(f)
             self._fill(lev, 0.0, whole=True, skip=None, crse=(c.U, c.U, 0.0))
-        self.dt = [0.0] * len(self.levels)
+        self.dt = [0.0] * (self.max_level + 1)
         self.time = 0.0
-        self.regs = [[] for _ in self.levels]
+        self.regs = [[] for _ in range(self.max_level + 1)]
+        self.istep = [0] * (self.max_level + 1)
+        self.last_regrid = [0] * (self.max_level + 1)
 
     @property
     def finest(self):
@@ -534,15 +548,66 @@ class AmrOracle:
         for l in range(1, self.finest + 1):
             self.dt[l] = self.dt[l - 1] / 2.0
         if not self.subcycle:
-            m = min(self.dt)
-            self.dt = [m] * len(self.dt)
-        return list(self.dt)
+            m = min(self.dt[:self.finest + 1])
+            for l in range(self.finest + 1):
+                self.dt[l] = m
+        return list(self.dt[:self.finest + 1])
+
+    def _remake(self, lbase, time, boxes):
+        """New grids for the levels above lbase (AmrMesh::MakeNewGrids with the
+        boxes given): an existing level keeps its data where the new boxes
+        cover cells it held and takes the conservative interpolation of the
+        level below elsewhere (RemakeLevel), a new level takes piecewise-
+        constant terrain and the interpolation everywhere
+        (MakeNewLevelFromCoarse); a level with no boxes is removed with the
+        levels above it."""
+        for k in range(lbase, self.max_level):
+            if k > self.finest:
+                break
+            new = [tuple(b) for b in boxes.get(k + 1, [])]
+            if not new:
+                del self.levels[k + 1:]
+                break
+            c = self.levels[k]
+            if k + 1 > self.finest:
+                f = DenseLevelOracle(2 * c.nx, 2 * c.ny, c.dx / 2, c.dy / 2, self.bc_lo, self.bc_hi)
+                pad = lambda a: np.pad(a, G, mode="edge")
+                ref = lambda a: np.repeat(np.repeat(a, 2, 0), 2, 1)
+                f.T[:] = [pad(ref(c.interior(c.T[0]))), pad(ref(c.interior(c.T[1])))]
+                f.t_new, f.t_old = time, time - 1.0e200
+                self.levels.append(f)
+                self.dt[k + 1] = self.dt[k] / (2.0 if self.subcycle else 1.0)
+                old = None
+            else:
+                f = self.levels[k + 1]
+                if new == f.boxes:
+                    continue                   # the same grids: nothing to remake
+                old = f.mask.copy()
+            f.set_boxes(new)
+            if old is None:
+                self._fill(k + 1, time, whole=True, skip=None, crse=(c.U, c.U, 0.0))
+            else:
+                self._fill(k + 1, time, whole=True, skip=old, crse=(c.U, c.U, 0.0))
 
     def _time_step(self, lev, time):
+        if (self.regrid_int > 0 and lev < self.max_level and self.istep[lev] > self.last_regrid[lev]
+                and self.istep[lev] % self.regrid_int == 0):
+            if not self.events:
+                raise AssertionError(f"level {lev} regrids at t = {time!r} but no event is left")
+            el, et, eb = self.events.pop(0)
+            if el != lev or et != time:
+                raise AssertionError(f"regrid of level {lev} at t = {time!r}; next event {el} at {et!r}")
+            old_finest = self.finest
+            self._remake(lev, time, eb)
+            for k in range(lev, self.finest + 1):
+                self.last_regrid[k] = self.istep[k]
+            for k in range(old_finest + 1, self.finest + 1):
+                self.dt[k] = self.dt[k - 1] / (2.0 if self.subcycle else 1.0)
         lv = self.levels[lev]
         lv.t_old = lv.t_new
         lv.t_new += self.dt[lev]
         self._advance(lev, lv.t_old, self.dt[lev])
+        self.istep[lev] += 1
         if lev < self.finest:
             nsub = 2 if self.subcycle else 1
             for i in range(1, nsub + 1):
@@ -555,6 +620,7 @@ class AmrOracle:
         if dts is None:
             self.compute_dt(cfl)
         else:
-            self.dt = list(dts)
+            for l, d in enumerate(dts):
+                self.dt[l] = float(d)
         self._time_step(0, self.time)
         self.time = self.levels[0].t_new

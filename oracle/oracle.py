@@ -13,7 +13,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libhx_oracle.so")
+_LIB32_PATH = os.path.join(_HERE, "libhx_oracle_f32.so")
 _lib = None
+_lib32 = None
 
 ROE = 0
 HYDROSTATIC_ROE = 1
@@ -199,6 +201,34 @@ def run(solver, h, hu, hv, z, n, dx, dy, cfl, nsteps, bc_lo=(1, 1), bc_hi=(1, 1)
                        cfl, nsteps, _p(dth), _p(vh))
     if rc != 0:
         raise RuntimeError(f"oracle run failed with status {rc}")
+    return h, hu, hv, dth, vh
+
+
+def run32(solver, h, hu, hv, z, n, dx, dy, cfl, nsteps, bc_lo=(1, 1), bc_hi=(1, 1)):
+    """run() of the single-precision restatement (hx_oracle_f32.c): every
+    operation rounded to float in the oracle's order.  Returns float32
+    (h, hu, hv, dt_hist, vel_hist)."""
+    global _lib32
+    if _lib32 is None:
+        if not os.path.exists(_LIB32_PATH):
+            build(force=True)
+        L = C.CDLL(_LIB32_PATH)
+        fp = C.POINTER(C.c_float)
+        L.hxo_run.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, fp, C.c_float, C.c_float,
+                              _ip, _ip, C.c_float, C.c_int, fp, fp]
+        _lib32 = L
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    h, hu, hv = (f32(a).copy() for a in (h, hu, hv))
+    z, n = f32(z), f32(n)
+    ny, nx = h.shape
+    dth = np.zeros(nsteps, np.float32)
+    vh = np.zeros(nsteps, np.float32)
+    lo, hi = _bc(bc_lo, bc_hi)
+    p = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    rc = _lib32.hxo_run(solver, nx, ny, p(h), p(hu), p(hv), p(z), p(n), dx, dy, lo, hi, cfl,
+                        nsteps, p(dth), p(vh))
+    if rc != 0:
+        raise RuntimeError(f"single-precision oracle run failed with status {rc}")
     return h, hu, hv, dth, vh
 
 

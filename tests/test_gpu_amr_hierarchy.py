@@ -280,3 +280,123 @@ def test_hierarchy_conservation_and_plotfile(gpu, tmp_path):
         for fab in got["fabs"]:
             i0, j0, i1, j1 = fab["box"]
             np.testing.assert_array_equal(fab["data"][:3], U[:, j0:j1 + 1, i0:i1 + 1])
+
+
+# ---------------------------------------------------------------------------
+# Config C3: the circular dam break on a 3-level hierarchy with regridding
+# ---------------------------------------------------------------------------
+from hydroexa_amd import workloads as W  # noqa: E402
+
+
+def c3_hierarchy(w, **kw):
+    args = dict(max_level=2, regrid_int=4, blocking_factor=16, max_grid_size=128,
+                tag_thresh=[(0.1, 1e300, 1e300)])
+    args.update(kw)
+    return AmrHierarchy(w.solver, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi, **args)
+
+
+def record_regrids(H):
+    """Record every regrid of H after it happened: (level, time, the new box
+    lists of the levels above it)."""
+    events = []
+    make = H._make_grids
+
+    def rec(lbase, time, from_scratch=False):
+        make(lbase, time, from_scratch)
+        events.append((lbase, time, {l: list(H.levels[l].boxes) for l in range(lbase + 1, H.finest + 1)}))
+    H._make_grids = rec
+    return events
+
+
+@pytest.mark.parametrize("n,steps", [(256, 9), (512, 5)])
+def test_c3_cylinder_replays_through_the_oracle(gpu, n, steps):
+    """Config C3's cylinder dam break (workloads.circular_dam_break: depth 2.0
+    inside r = 0.2, 0.5 outside, bed 0.05 U[0,1), Manning 0.03, hydrostatic
+    Roe, transmissive sides, tags |grad h| > 0.1, regrid every 4 steps of a
+    level, 2 refined levels) against oracle/amr.py's AmrOracle replaying the
+    hierarchy's own grids: level 1 regrids level 2 every 2 coarse steps and
+    level 0 regrids both every 4.  At a 256^2 base 9 coarse steps cross the
+    regrids at steps 2, 4, 6 and 8 (two level-0 intervals); at the config's
+    512^2 base 5 steps cross those at 2 and 4.  Every level within 1e-12
+    relative L-infinity after every step, the same dts, every event replayed
+    where the cadence puts it."""
+    import time as _t
+    set_arith("exact")
+    w = W.circular_dam_break(n)
+    H = c3_hierarchy(w)
+    boxes0 = {l: list(H.levels[l].boxes) for l in range(1, H.finest + 1)}
+    events = record_regrids(H)
+    R = A.AmrOracle(MODELS[w.solver], w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi,
+                    boxes0, regrid_int=4, max_level=2, events=events)
+    print(f"C3 replay n={n}: initial boxes per level {[len(b) for b in boxes0.values()]}", flush=True)
+    seen = []
+    for it in range(steps):
+        t0 = _t.time()
+        dts = H.compute_dt(w.cfl)
+        want = R.compute_dt(w.cfl)
+        assert all(abs(a - b) <= 1e-12 * b for a, b in zip(dts, want)), (it, dts, want)
+        nev = len(events)
+        H.step(dts=dts)
+        torch.cuda.synchronize()
+        seen += [(e[0], it) for e in events[nev:]] if len(events) > nev else []
+        R.step(dts=dts)
+        assert not events, f"step {it}: {len(events)} regrid event(s) not replayed"
+        assert R.finest == H.finest
+        for l, (lv, ro) in enumerate(zip(H.levels, R.levels)):
+            assert lv.boxes == ro.boxes, (it, l)
+            m = ro.interior(ro.mask)
+            got = lv.interior(lv.U_new).double().cpu().numpy()[:, m]
+            ref = ro.interior(ro.U)[:, m]
+            for k in range(3):
+                err = rel_linf(got[k], ref[k])
+                assert err <= 1e-12, (n, it, l, k, err)
+        print(f"  step {it}: boxes {[len(lv.boxes) for lv in H.levels]} "
+              f"cells {[int(lv.interior(lv.mask).sum().item()) for lv in H.levels]} "
+              f"oracle+check {_t.time() - t0:.1f} s", flush=True)
+    regrid_levels = [e[0] for e in seen]
+    assert 0 in regrid_levels and 1 in regrid_levels, seen
+
+
+def test_c3_cylinder_full_length_properties(gpu):
+    """Config C3 at full size and length: 512^2 base, 2 refined levels,
+    1000 coarse steps through hx_amr_run.  In a closed box (reflective sides,
+    the bed flat on the two cells along the walls so that the walls hold the
+    water: the reference's wall leak on a sloping bed moves mass through them)
+    the composite mass stays to round-off; every level stays finite with
+    h >= 0; the grids are regridded as the front spreads."""
+    set_arith("fast")
+    try:
+        _c3_full_length()
+    finally:
+        set_arith("exact")
+
+
+def _c3_full_length():
+    w = W.circular_dam_break(512, bc=(0, 0))
+    w.z[:2, :] = w.z[-2:, :] = 0.0
+    w.z[:, :2] = w.z[:, -2:] = 0.0
+    H = c3_hierarchy(w)
+    m0 = H.composite_mass()
+    seen = {tuple(len(lv.boxes) for lv in H.levels)}
+    prev = H.state(0).clone()
+    fine_mass = lambda: H.state(2)[0].double().sum().item() * H.levels[2].dx * H.levels[2].dy
+    done = 0
+    while done < 1000:
+        H.run_native(1e30, w.cfl, max_steps=100)
+        done += 100
+        seen.add(tuple(len(lv.boxes) for lv in H.levels))
+        moved = (H.state(0) - prev).abs().max().item()
+        prev = H.state(0).clone()
+        print(f"  level 0 max change over the last 100 steps {moved:.3e}; composite mass "
+              f"{H.composite_mass()!r} (start {m0!r}); finest-level mass {fine_mass()!r}", flush=True)
+        assert moved > 0.0
+        for l, lv in enumerate(H.levels):
+            U = lv.interior(lv.U_new)
+            m = lv.interior(lv.mask).bool()
+            assert torch.isfinite(U[:, m]).all(), (done, l)
+            assert (U[0][m] >= 0).all(), (done, l)
+        drift = abs(H.composite_mass() - m0) / m0
+        print(f"C3 full length: {done} steps, t = {H.time:.5f}, boxes {[len(lv.boxes) for lv in H.levels]}, "
+              f"mass drift {drift:.2e}", flush=True)
+        assert drift < 1e-12
+    assert H.finest == 2 and len(seen) > 1

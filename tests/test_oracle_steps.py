@@ -211,3 +211,25 @@ def test_hydrostatic_dambreak_converges_to_swashes():
             t += dt
         errs[s] = float(np.mean(np.abs(h[0] - swashes_wet_dambreak(t, x, hl, hr, x0))))
     assert errs[O.HYDROSTATIC_ROE] < 0.8 * errs[O.ROE] < 0.8 * 3e-5
+
+
+def test_single_precision_restatement_tracks_the_fp64_oracle():
+    """The single-precision build of the restatement (hx_oracle_f32.c) is the
+    same algorithm rounded to float: on smooth flows it stays within float
+    rounding of the FP64 oracle (a wrong type or literal mapping in the build
+    would not), and it keeps a lake at rest to float round-off."""
+    from hydroexa_amd import workloads as W
+    w = W.dam_break(64, 1)
+    a = O.run32(O.ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.cfl, 40, w.bc_lo, w.bc_hi)
+    b = O.run(O.ROE, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.cfl, 40, w.bc_lo, w.bc_hi)
+    assert a[0].dtype == np.float32
+    assert np.abs(a[0] - b[0]).max() / b[0].max() < 2e-6
+    assert np.abs(a[1] - b[1]).max() / np.abs(b[1]).max() < 2e-5
+    np.testing.assert_allclose(a[3], b[3], rtol=2e-6)
+    # the float dt is the float CFL estimate, not the double one rounded
+    assert not np.array_equal(a[3], b[3].astype(np.float32))
+    lk = W.lake_at_rest(64, 1, nodata_corner=0)
+    c = O.run32(O.HYDROSTATIC_ROE, lk.h, lk.hu, lk.hv, lk.z, lk.n, lk.dx, lk.dy, lk.cfl, 100,
+                lk.bc_lo, lk.bc_hi)
+    assert c[4].max() < 1e-5
+    assert np.abs(c[0] - lk.h.astype(np.float32)).max() < 1e-5
