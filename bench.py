@@ -48,12 +48,15 @@ def parse():
                     help="skip the other advance workloads (bumpy bed, FP32, lakes at rest)")
     ap.add_argument("--cpu-worker", nargs=2, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
-    ap.add_argument("--only", choices=["ensemble", "adjoint", "two_level", "amr"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "adjoint", "adjoint_ensemble", "two_level", "amr"],
+                    default=None,
                     help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="steps of the 2048^2 calibration rollout timed forward + adjoint (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=120,
                     help="timed steps of the batched ensemble and of the per-scenario loop (0: skip)")
+    ap.add_argument("--adjoint-ensemble-steps", type=int, default=100,
+                    help="steps of the batched forward + adjoint leg (0: skip)")
     ap.add_argument("--two-level-steps", type=int, default=100,
                     help="timed coarse steps of the two-level dam break (0: skip)")
     ap.add_argument("--amr-steps", type=int, default=100,
@@ -351,6 +354,75 @@ def time_adjoint(steps, every=20, arith="fast"):
     return out
 
 
+def time_adjoint_ensemble(steps, nscen=128, every=10, arith="fast"):
+    """Batched forward + adjoint (hx_adjoint_batch_*) against the loop that
+    runs each scenario's forward + adjoint on its own (hx_adjoint_*, the same
+    kernels one scenario at a time): workloads.adjoint_ensemble, nscen dam
+    breaks of sides U{128..512} (seeded), hydrostatic Roe + Manning, FP64,
+    `steps` steps with a checkpoint every `every`, 8 gauges per scenario,
+    loss = sum of squares of the gauge series.  Both take the same CFL dt
+    and give the same bits (checked here on every scenario's roughness
+    gradient).  Cell updates of forward + adjoint = cells x steps."""
+    import torch
+    from hydroexa_amd import set_arith
+    from hydroexa_amd import workloads as W
+    from hydroexa_amd.pyhydroexa import Rollout, RolloutBatch
+
+    set_arith(arith)
+    ws, gauges = W.adjoint_ensemble(nscen, 128, 512, steps=steps)
+    cells = sum(w.cells for w in ws)
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device="cuda")
+    U0s = [torch.stack([T(w.h), T(w.hu), T(w.hv)]).contiguous() for w in ws]
+    Ts = [torch.stack([T(w.z), T(w.n)]).contiguous() for w in ws]
+    sc = [dict(nx=w.h.shape[1], ny=w.h.shape[0], dx=w.dx, dy=w.dy, bc_lo=w.bc_lo, bc_hi=w.bc_hi,
+               gauges=g) for w, g in zip(ws, gauges)]
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    rb = RolloutBatch("HydrostaticRoe", sc, steps, ckpt_every=every)
+    for rep in range(2):                      # the first pass warms up
+        ev[0].record(st)
+        series, _ = rb.forward_raw(U0s, Ts)
+        ev[1].record(st)
+        g_state, g_ter = rb.backward_raw([2.0 * x for x in series], None)
+        ev[2].record(st)
+        torch.cuda.synchronize()
+    b_fwd, b_bwd = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    ros = [Rollout("HydrostaticRoe", s["nx"], s["ny"], s["dx"], s["dy"], s["bc_lo"], s["bc_hi"],
+                   steps, s["gauges"], ckpt_every=every) for s in sc]
+    for k in range(min(4, nscen)):            # warm-up of the one-scenario path
+        se, _ = ros[k].forward_raw(U0s[k], Ts[k])
+        ros[k].backward_raw(2.0 * se, None)
+    torch.cuda.synchronize()
+    same = True
+    ev[0].record(st)
+    outs = []
+    for k, ro in enumerate(ros):
+        se, _ = ro.forward_raw(U0s[k], Ts[k])
+        gs1, gt1 = ro.backward_raw(2.0 * se, None)
+        outs.append(gt1)
+    ev[1].record(st)
+    torch.cuda.synchronize()
+    loop_ms = ev[0].elapsed_time(ev[1])
+    for k in range(nscen):
+        same = same and torch.equal(outs[k], g_ter[k])
+    for ro in ros:
+        ro.close()
+    rb.close()
+    set_arith("exact")
+    del ros, rb, outs, g_state, g_ter, U0s, Ts
+    torch.cuda.empty_cache()
+    batch_ms = b_fwd + b_bwd
+    return {"workload": f"batched forward + adjoint: {nscen} dam breaks, sides U{{128..512}} "
+                        f"(seeded), hydrostatic Roe + Manning, FP64, {steps} steps, checkpoint "
+                        f"every {every}, 8 gauges each",
+            "cells": cells, "steps": steps, "arith": arith,
+            "batched_forward_ms": b_fwd, "batched_backward_ms": b_bwd,
+            "batched_Mcups": cells * steps / (batch_ms / 1e3) / 1e6,
+            "per_scenario_loop_ms": loop_ms,
+            "per_scenario_loop_Mcups": cells * steps / (loop_ms / 1e3) / 1e6,
+            "speedup": loop_ms / batch_ms, "gradients_bitwise_equal": bool(same)}
+
+
 def time_two_level(steps, arith="fast"):
     """Two-level coupled step (ratio 2, subcycled): the 4096^2 FP64 dam break
     with the 1024^2 coarse cells around the dam refined (2048^2 fine cells),
@@ -504,6 +576,10 @@ def main():
     if args.only == "adjoint":
         print(json.dumps({"adjoint": time_adjoint(args.adjoint_steps, 20, args.arith)}), flush=True)
         return
+    if args.only == "adjoint_ensemble":
+        print(json.dumps({"adjoint_ensemble": time_adjoint_ensemble(args.adjoint_ensemble_steps,
+                                                                    arith=args.arith)}), flush=True)
+        return
     if args.only == "amr":
         print(json.dumps({"amr_c3": time_amr_c3(args.amr_steps, args.arith)}), flush=True)
         return
@@ -586,6 +662,8 @@ def main():
         out["ensemble"] = time_ensemble(args.ensemble_steps, args.arith)
     if args.adjoint_steps > 0:
         out["adjoint"] = time_adjoint(args.adjoint_steps, 20, args.arith)
+    if args.adjoint_ensemble_steps > 0:
+        out["adjoint_ensemble"] = time_adjoint_ensemble(args.adjoint_ensemble_steps, arith=args.arith)
     if args.two_level_steps > 0:
         out["two_level"] = time_two_level(args.two_level_steps, args.arith)
     if args.amr_steps > 0:

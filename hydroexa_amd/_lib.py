@@ -39,6 +39,7 @@ EXPORTS = (
     "hx_prune_boxes", "hx_run_level", "hx_run_level_resume", "hx_run_level_noalloc", "hx_set_tuning", "hx_set_arith",
     "hx_ensemble_create", "hx_ensemble_destroy", "hx_ensemble_set_tuning", "hx_ensemble_run",
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
+    "hx_adjoint_batch_create", "hx_adjoint_batch_forward", "hx_adjoint_batch_backward",
     "hx_conservation", "hx_face_fluxes", "hx_fill_fine_ghosts", "hx_reflux",
     "hx_average_down_state", "hx_two_level_run", "hx_fill_fine_region", "hx_reflux_side",
     "hx_boxes_dt", "hx_amr_run", "hx_tag_blocks", "hx_stream_probe",
@@ -141,6 +142,13 @@ def load() -> C.CDLL:
     L.hx_average_down_state.argtypes = [vp, pp, C.c_int, C.c_int, C.c_int, pp, vp, vp, vp, vp]
     L.hx_adjoint_forward.argtypes = [vp, pp, C.c_double, dp, vp, vp, vp, vp, i32p, vp]
     L.hx_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.hx_adjoint_batch_create.argtypes = [vp, C.c_int, C.c_int, pp, C.c_int, C.c_int,
+                                          C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_int64)),
+                                          C.POINTER(vp)]
+    L.hx_adjoint_batch_forward.argtypes = [vp, pp, C.c_double, dp, C.POINTER(vp), C.POINTER(vp),
+                                           i32p, vp]
+    L.hx_adjoint_batch_backward.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                            C.POINTER(vp), C.POINTER(vp), vp]
     _lib = L
     return L
 

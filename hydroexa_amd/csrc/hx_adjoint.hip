@@ -120,16 +120,15 @@ constexpr int kTX = 64, kTY = 16;
 // y: a reflective high side folds ghost n+1 into cell n-2, which may be the
 // last cell of the tile before the one holding n-1.
 constexpr int kHPX = kTX + 5, kHPY = kTY + 5;
+typedef double HAcc[4][kHPY][kHPX];
 
-__global__ __launch_bounds__(kThreads) void htile_kernel(State s, Geo g, double dt, double dx,
-                                                         double dy,
-                                                         const double* __restrict__ lun,
-                                                         double* __restrict__ lam_new,
-                                                         double* __restrict__ gz)
+__device__ __forceinline__ void htile_body(HAcc& acc, const State& s, const Geo& g, double dt,
+                                           double dx, double dy, const double* __restrict__ lun,
+                                           double* __restrict__ lam_new, double* __restrict__ gz,
+                                           int bx, int by)
 {
-    __shared__ double acc[4][kHPY][kHPX];
     const int64_t N = (int64_t)g.nx * g.ny;
-    const int i0 = blockIdx.x * kTX, j0 = blockIdx.y * kTY;
+    const int i0 = bx * kTX, j0 = by * kTY;
     const int t = threadIdx.x;
     for (int k = t; k < 4 * kHPY * kHPX; k += kThreads) (&acc[0][0][0])[k] = 0.0;
     __syncthreads();
@@ -272,17 +271,15 @@ __global__ __launch_bounds__(kThreads) void htile_kernel(State s, Geo g, double 
 // Plain Roe: two-cell faces; LDS image covers [i0 - 1, i0 + kTX + 1) (the
 // high ghost n folds into n - 1, which is always in the tile holding it).
 constexpr int kRPX = kTX + 2, kRPY = kTY + 2;
+typedef double RAcc[5][kRPY][kRPX];
 
-__global__ __launch_bounds__(kThreads) void rtile_kernel(State s, Geo g, double dt, double dx,
-                                                         double dy,
-                                                         const double* __restrict__ lam,
-                                                         double* __restrict__ lam_new,
-                                                         double* __restrict__ gz,
-                                                         double* __restrict__ gn)
+__device__ __forceinline__ void rtile_body(RAcc& acc, const State& s, const Geo& g, double dt,
+                                           double dx, double dy, const double* __restrict__ lam,
+                                           double* __restrict__ lam_new, double* __restrict__ gz,
+                                           double* __restrict__ gn, int bx, int by)
 {
-    __shared__ double acc[5][kRPY][kRPX];
     const int64_t N = (int64_t)g.nx * g.ny;
-    const int i0 = blockIdx.x * kTX, j0 = blockIdx.y * kTY;
+    const int i0 = bx * kTX, j0 = by * kTY;
     const int t = threadIdx.x;
     for (int k = t; k < 5 * kRPY * kRPX; k += kThreads) (&acc[0][0][0])[k] = 0.0;
     __syncthreads();
@@ -420,54 +417,182 @@ __global__ __launch_bounds__(kThreads) void rtile_kernel(State s, Geo g, double 
 }
 
 // ---------------------------------------------------------------------------
-// Small kernels
+// Batched launches: every kernel below covers all scenarios of a batch in
+// one grid.  A workgroup (or thread) finds its scenario by binary search
+// over a prefix of work items; scenarios run in lockstep (one step index
+// and one segment for all), each with its own buffers, geometry and dt.
 // ---------------------------------------------------------------------------
+struct Batch {
+    const AdjScen* sc;
+    const int* start;      // [nscen + 1] prefix of this launch's work items
+    int nscen;
+    int every, nsteps;
+    int t, s0;             // step; first step of its segment (backward)
+    int lam_in;            // slot of d.lam holding lambda^{t+1}
+};
+
+__device__ __forceinline__ int find_scen(const Batch& b, int item)
+{
+    int lo = 0, hi = b.nscen - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.start[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// State t of the segment that starts at s0 (its checkpoint or a recomputed
+// state).
+__device__ __forceinline__ const double* seg_state(const AdjScen& d, const Batch& b)
+{
+    const int64_t N3 = 3 * d.N;
+    return (b.t == b.s0) ? d.ckpt + (int64_t)(b.s0 / b.every) * N3
+                         : d.seg + (int64_t)(b.t - b.s0 - 1) * N3;
+}
+
+__device__ __forceinline__ Geo geo_of(const AdjScen& d)
+{
+    return Geo{d.p.nx, d.p.ny, d.p.bl0, d.p.bl1, d.p.bh0, d.p.bh1};
+}
+
+// start: tile prefix
+__global__ __launch_bounds__(kThreads) void htile_kernel(Batch b)
+{
+    __shared__ HAcc acc;
+    const int blk = blockIdx.x;
+    const AdjScen& d = b.sc[find_scen(b, blk)];
+    const int lt = blk - d.tile0;
+    const double* U = seg_state(d, b);
+    const State s{U, U + d.N, U + 2 * d.N, d.p.z, d.p.n};
+    double* lam_new = d.lam + (int64_t)(1 - b.lam_in) * 3 * d.N;
+    htile_body(acc, s, geo_of(d), d.dts[b.t], d.dx, d.dy, d.lun, lam_new, d.gz, lt % d.ntx,
+               lt / d.ntx);
+}
+
+__global__ __launch_bounds__(kThreads) void rtile_kernel(Batch b)
+{
+    __shared__ RAcc acc;
+    const int blk = blockIdx.x;
+    const AdjScen& d = b.sc[find_scen(b, blk)];
+    const int lt = blk - d.tile0;
+    const double* U = seg_state(d, b);
+    const State s{U, U + d.N, U + 2 * d.N, d.p.z, d.p.n};
+    const double* lam = d.lam + (int64_t)b.lam_in * 3 * d.N;
+    double* lam_new = d.lam + (int64_t)(1 - b.lam_in) * 3 * d.N;
+    rtile_body(acc, s, geo_of(d), d.dts[b.t], d.dx, d.dy, lam, lam_new, d.gz, d.gn, lt % d.ntx,
+               lt / d.ntx);
+}
+
+// Cell-wise kernels: start is a prefix of workgroups, kCellBlocks(N) per
+// scenario, each thread striding over its scenario's cells.
+constexpr int kCellsPerThread = 8;
+inline int cell_blocks(int64_t N)
+{
+    return (int)((N + (int64_t)kThreads * kCellsPerThread - 1) / ((int64_t)kThreads * kCellsPerThread));
+}
+
+#define HX_ADJ_CELLS(d, lb, nb)                                                   \
+    const int _blk = blockIdx.x;                                                  \
+    const int _s = find_scen(b, _blk);                                            \
+    const AdjScen& d = b.sc[_s];                                                  \
+    const int lb = _blk - b.start[_s], nb = b.start[_s + 1] - b.start[_s]
+
 // Reverse of Manning friction and the dry clamp of the hydrostatic update
 // (HydrostaticRoe.cpp:566-603), cell by cell, from the stored state after
 // the flux update: lun = d(output)/d(un)^T lam, gn += d/dn.
-__global__ __launch_bounds__(kThreads) void post_kernel(const double* __restrict__ un,
-                                                        const double* __restrict__ n,
-                                                        const double* __restrict__ lam,
-                                                        double* __restrict__ lun,
-                                                        double* __restrict__ gn, double dt,
-                                                        int64_t N)
+__global__ __launch_bounds__(kThreads) void post_kernel(Batch b)
 {
-    for (int64_t o = blockIdx.x * (int64_t)kThreads + threadIdx.x; o < N;
-         o += (int64_t)gridDim.x * kThreads) {
+    HX_ADJ_CELLS(d, lb, nb);
+    const int64_t N = d.N;
+    const double* un = d.un + (int64_t)(b.t - b.s0) * 3 * N;
+    const double* lam = d.lam + (int64_t)b.lam_in * 3 * N;
+    const double dt = d.dts[b.t];
+    for (int64_t o = (int64_t)lb * kThreads + threadIdx.x; o < N; o += (int64_t)nb * kThreads) {
         const double u[3] = {un[o], un[N + o], un[2 * N + o]};
         const double ob[3] = {lam[o], lam[N + o], lam[2 * N + o]};
         double ub[3];
-        const double dn = hydro_post_vjp(u, n[o], dt, ob, ub);
-        lun[o] = ub[0];
-        lun[N + o] = ub[1];
-        lun[2 * N + o] = ub[2];
-        gn[o] += dn;
+        const double dn = hydro_post_vjp(u, d.p.n[o], dt, ob, ub);
+        d.lun[o] = ub[0];
+        d.lun[N + o] = ub[1];
+        d.lun[2 * N + o] = ub[2];
+        d.gn[o] += dn;
     }
 }
 
-__global__ void fill_kernel(double* p, int64_t n, double v)
+// Start of the backward: lambda = the final-state adjoint (or 0), zero
+// gradients.
+__global__ __launch_bounds__(kThreads) void init_kernel(Batch b)
 {
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (int64_t)gridDim.x * blockDim.x)
-        p[k] = v;
+    HX_ADJ_CELLS(d, lb, nb);
+    const int64_t N = d.N;
+    double* lam = d.lam + (int64_t)b.lam_in * 3 * N;
+    for (int64_t o = (int64_t)lb * kThreads + threadIdx.x; o < N; o += (int64_t)nb * kThreads) {
+        for (int k = 0; k < 3; ++k) lam[k * N + o] = d.gfin ? d.gfin[k * N + o] : 0.0;
+        d.gz[o] = 0.0;
+        d.gn[o] = 0.0;
+    }
 }
 
-__global__ void sample_kernel(const double* h, const int64_t* gauges, double* out, int ng)
+// Forward start: the initial state into checkpoint 0.  Backward end:
+// lambda^0 into the caller's state gradient (dir 1).
+__global__ __launch_bounds__(kThreads) void copy_kernel(Batch b, int dir)
+{
+    HX_ADJ_CELLS(d, lb, nb);
+    const int64_t N = d.N;
+    const double* lam = d.lam + (int64_t)b.lam_in * 3 * N;
+    for (int64_t o = (int64_t)lb * kThreads + threadIdx.x; o < N; o += (int64_t)nb * kThreads) {
+        if (dir == 0) {
+            d.ckpt[o] = d.in_h[o];
+            d.ckpt[N + o] = d.in_hu[o];
+            d.ckpt[2 * N + o] = d.in_hv[o];
+        } else {
+            for (int k = 0; k < 3; ++k) d.gstate[k * N + o] = lam[k * N + o];
+        }
+    }
+}
+
+// Depth at the gauges after forward step t; start: gauge prefix, one thread
+// per gauge.
+__global__ void sample_kernel(Batch b, const int64_t* __restrict__ gauges, int total)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < ng) out[g] = h[gauges[g]];
+    if (g >= total) return;
+    const AdjScen& d = b.sc[find_scen(b, g)];
+    const int t = b.t;
+    const int64_t N3 = 3 * d.N;
+    const double* h;
+    if (t == b.nsteps - 1) h = d.out_h;
+    else h = ((t + 1) % b.every == 0) ? d.ckpt + (int64_t)((t + 1) / b.every) * N3
+                                      : d.seg + (int64_t)((t + 1) & 1) * N3;
+    d.series[(int64_t)t * d.ngauge + (g - d.gauge0)] = h[gauges[g]];
 }
 
-// Gauge adjoints into lambda_h: one thread per distinct gauge cell, which
-// sums the series adjoints of its gauges in gauge order (deterministic).
-__global__ void seed_kernel(double* lam_h, const int64_t* ucell, const int* uptr,
-                            const int* uidx, const double* gs, int nu)
+// Gauge adjoints into lambda_h: one thread per distinct gauge cell of a
+// scenario (start: prefix of distinct cells), which sums the series
+// adjoints of its gauges in gauge order (deterministic).
+__global__ void seed_kernel(Batch b, const int64_t* __restrict__ ucell,
+                            const int* __restrict__ uptr, const int* __restrict__ uidx, int total)
 {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nu) return;
+    if (u >= total) return;
+    const AdjScen& d = b.sc[find_scen(b, u)];
+    if (!d.gs) return;
     double acc = 0.0;
-    for (int k = uptr[u]; k < uptr[u + 1]; ++k) acc += gs[uidx[k]];
-    lam_h[ucell[u]] += acc;
+    for (int k = uptr[u]; k < uptr[u + 1]; ++k) acc += d.gs[(int64_t)b.t * d.ngauge + uidx[k]];
+    d.lam[(int64_t)b.lam_in * 3 * d.N + ucell[u]] += acc;
+}
+
+__global__ void clear_kernel(AdjScen* sc, int nscen)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscen) return;
+    for (int k = 0; k < 3; ++k) {
+        sc[s].slots[k].amax = 0ull;
+        sc[s].slots[k].vmax = 0ull;
+        sc[s].slots[k].flags = 0u;
+        sc[s].slots[k].pad = 0u;
+    }
+    *sc[s].err = 0;
 }
 
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -483,38 +608,189 @@ using namespace hx::adj;
 
 struct hx_adjoint {
     hx_ctx* ctx;
-    int solver, nx, ny;
-    double dx, dy;
-    int bc_lo[2], bc_hi[2];
-    int nsteps, every, ncp, nseg, ngauge;
-    int64_t N;
-    int64_t* gauges;   // device, ngauge flat cell indices
-    int64_t* ucell;    // device, distinct gauge cells
-    int* uptr;         // device, nu + 1: gauges of distinct cell u are
-    int* uidx;         //   uidx[uptr[u] .. uptr[u+1]) (device, ngauge)
-    int nu;
-    double* ckpt;      // ncp states (3N each): the state at steps 0, every, 2 every, ...
-    double* seg;       // nseg states: recomputed states inside a segment
-    double* un;        // hydrostatic: every states after the flux update and
-                       // before friction, stored by the segment's recomputation
-    double* lam;       // 2 states: adjoint ping-pong
-    double* lun;       // 1 state: adjoint before friction (hydrostatic)
-    double* hist;      // 2 nsteps: dt and max|vel| of the forward's CFL steps
-    std::vector<double> dts;
-    const double* z;
-    const double* n;
-    int ready;         // a forward completed and the dt schedule is known
+    int solver, nscen;
+    int nsteps, every, ncp, nseg;
+    std::vector<AdjScen> sc;      // host copy of the scenario table
+    std::vector<int64_t> N;       // cells per scenario
+    std::vector<int> ngauge;
+    AdjScen* d_sc;                // device scenario table
+    int* d_strip;                 // [nscen + 1] prefixes: strips, tiles,
+    int* d_tile;                  //   cell-wise workgroups, gauges,
+    int* d_cellblk;               //   distinct gauge cells
+    int* d_gauge;
+    int* d_ucellp;
+    int total_strips, total_tiles, total_cellblk, total_gauges, total_u;
+    int64_t* gauges;              // device: every scenario's gauges, scenario by scenario
+    int64_t* ucell;               // device: distinct gauge cells, scenario by scenario
+    int* uptr;                    // [total_u + 1] into uidx
+    int* uidx;                    // local gauge index per entry
+    CflSlot* slots;               // 3 per scenario
+    int* err;                     // 1 per scenario
+    double* dts;                  // [nscen][nsteps]
+    double* vel;                  // [nscen][nsteps]
+    std::vector<double*> bufs;    // every device allocation of the scenarios
+    std::vector<double> hdts;     // host copy of dts
+    int ready;                    // a forward completed and the dt schedules are known
 };
 
 static void adj_free(hx_adjoint* a)
 {
-    double* bufs[] = {a->ckpt, a->seg, a->lam, a->lun, a->un, a->hist};
-    for (double* b : bufs)
-        if (b) (void)hipFree(b);
-    if (a->gauges) (void)hipFree(a->gauges);
-    if (a->ucell) (void)hipFree(a->ucell);
-    if (a->uptr) (void)hipFree(a->uptr);
-    if (a->uidx) (void)hipFree(a->uidx);
+    for (double* b : a->bufs) (void)hipFree(b);
+    void* misc[] = {a->d_sc, a->d_strip, a->d_tile, a->d_cellblk, a->d_gauge, a->d_ucellp,
+                    a->gauges, a->ucell, a->uptr, a->uidx, a->slots, a->err, a->dts, a->vel};
+    for (void* p : misc)
+        if (p) (void)hipFree(p);
+}
+
+template <typename T>
+static bool dev_copy(T** dst, const std::vector<T>& src)
+{
+    if (src.empty()) return true;
+    return hipMalloc((void**)dst, sizeof(T) * src.size()) == hipSuccess &&
+           hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+static bool valid_geom(const hx_patch* geom)
+{
+    if (!geom || geom->nx < 2 || geom->ny < 2 || geom->dtype != HX_F64) return false;
+    if (!(geom->dx > 0.0) || !(geom->dy > 0.0)) return false;
+    for (int d = 0; d < 2; ++d)
+        if ((geom->bc_lo[d] != 0 && geom->bc_lo[d] != 1) ||
+            (geom->bc_hi[d] != 0 && geom->bc_hi[d] != 1))
+            return false;
+    return true;
+}
+
+extern "C" int hx_adjoint_batch_create(hx_ctx* c, int solver, int nscen, const hx_patch* geoms,
+                                       int nsteps, int ckpt_every, const int* ngauge,
+                                       const int64_t* const* gauges, hx_adjoint** out)
+{
+    if (!c) return HX_E_STATE;
+    if (!out) return HX_E_BADARG;
+    *out = nullptr;
+    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
+    if (nscen < 1 || !geoms || nsteps < 1 || ckpt_every < 1 || !ngauge) return HX_E_BADARG;
+    for (int s = 0; s < nscen; ++s) {
+        if (!valid_geom(&geoms[s]) || ngauge[s] < 0 || (ngauge[s] > 0 && (!gauges || !gauges[s])))
+            return HX_E_BADARG;
+        const int64_t N = (int64_t)geoms[s].nx * geoms[s].ny;
+        if (geoms[s].nx > (1 << 26) || N > ((int64_t)1 << 40)) return HX_E_BADARG;
+        for (int g = 0; g < ngauge[s]; ++g)
+            if (gauges[s][g] < 0 || gauges[s][g] >= N) return HX_E_BADARG;
+    }
+    HX_TRY(hipSetDevice(c->device));
+    hx_adjoint* a = new (std::nothrow) hx_adjoint();
+    if (!a) return HX_E_NOMEM;
+    a->ctx = c;
+    a->solver = solver;
+    a->nscen = nscen;
+    a->nsteps = nsteps;
+    a->every = std::min(ckpt_every, nsteps);
+    a->ncp = (nsteps + a->every - 1) / a->every;
+    // the hydrostatic recomputation also runs a segment's last step (for its
+    // pre-friction state) and needs a slot for that step's output
+    const bool hyd = (solver == HX_SOLVER_HYDROSTATIC_ROE);
+    a->nseg = std::max(a->every - (hyd ? 0 : 1), 2);
+    a->sc.assign(nscen, AdjScen{});
+    a->N.resize(nscen);
+    a->ngauge.assign(ngauge, ngauge + nscen);
+    std::vector<int> strip(nscen + 1, 0), tile(nscen + 1, 0), cellb(nscen + 1, 0),
+        gpre(nscen + 1, 0), upre(nscen + 1, 0);
+    std::vector<int64_t> allg, ucell;
+    std::vector<int> uptr(1, 0), uidx;
+    bool ok = true;
+    for (int s = 0; s < nscen && ok; ++s) {
+        const hx_patch& gm = geoms[s];
+        AdjScen& d = a->sc[s];
+        const int64_t N = (int64_t)gm.nx * gm.ny;
+        a->N[s] = N;
+        d.N = N;
+        hx_patch v = gm;
+        v.pitch = gm.nx;
+        v.h = v.hu = v.hv = v.eta = nullptr;
+        v.ng = 0;
+        d.p = host::view_of<double>(&v);
+        StepArgs<double> sa{};
+        sa.p = d.p;
+        sa.opitch = gm.nx;
+        host::fill_geometry(sa, solver, host::eff_rows(c, solver, HX_F64, gm.nx, gm.ny), gm.dx, gm.dy);
+        d.dx = sa.dx; d.dy = sa.dy; d.rdx = sa.rdx; d.rdy = sa.rdy; d.dx_pow2 = sa.dx_pow2;
+        d.rows = sa.rows; d.nsx = sa.nsx; d.nstrips = sa.nstrips;
+        d.strip0 = strip[s];
+        strip[s + 1] = strip[s] + d.nstrips;
+        d.ntx = (gm.nx + kTX - 1) / kTX;
+        d.ntiles = d.ntx * ((gm.ny + kTY - 1) / kTY);
+        d.tile0 = tile[s];
+        tile[s + 1] = tile[s] + d.ntiles;
+        cellb[s + 1] = cellb[s] + cell_blocks(N);
+        d.ngauge = ngauge[s];
+        d.gauge0 = gpre[s];
+        gpre[s + 1] = gpre[s] + ngauge[s];
+        for (int g = 0; g < ngauge[s]; ++g) allg.push_back(gauges[s][g]);
+        // group the scenario's gauges by cell (first appearance order)
+        std::vector<int64_t> uc;
+        std::vector<std::vector<int>> members;
+        for (int g = 0; g < ngauge[s]; ++g) {
+            size_t u = 0;
+            while (u < uc.size() && uc[u] != gauges[s][g]) ++u;
+            if (u == uc.size()) {
+                uc.push_back(gauges[s][g]);
+                members.emplace_back();
+            }
+            members[u].push_back(g);
+        }
+        d.u0 = upre[s];
+        d.nu = (int)uc.size();
+        upre[s + 1] = upre[s] + d.nu;
+        for (size_t u = 0; u < uc.size(); ++u) {
+            ucell.push_back(uc[u]);
+            uidx.insert(uidx.end(), members[u].begin(), members[u].end());
+            uptr.push_back((int)uidx.size());
+        }
+        const size_t S = sizeof(double) * 3 * (size_t)N;
+        auto alloc = [&](double** p, size_t bytes) {
+            if (!ok) return;
+            ok = hipMalloc((void**)p, bytes) == hipSuccess;
+            if (ok) a->bufs.push_back(*p);
+        };
+        alloc(&d.ckpt, S * a->ncp);
+        alloc(&d.seg, S * a->nseg);
+        alloc(&d.lam, 2 * S);
+        if (hyd) {
+            alloc(&d.lun, S);
+            alloc(&d.un, S * a->every);
+        }
+    }
+    a->total_strips = strip[nscen];
+    a->total_tiles = tile[nscen];
+    a->total_cellblk = cellb[nscen];
+    a->total_gauges = gpre[nscen];
+    a->total_u = upre[nscen];
+    ok = ok && hipMalloc(&a->slots, sizeof(CflSlot) * 3 * nscen) == hipSuccess &&
+         hipMalloc(&a->err, sizeof(int) * nscen) == hipSuccess &&
+         hipMalloc(&a->dts, sizeof(double) * (size_t)nscen * nsteps) == hipSuccess &&
+         hipMalloc(&a->vel, sizeof(double) * (size_t)nscen * nsteps) == hipSuccess &&
+         dev_copy(&a->d_strip, strip) && dev_copy(&a->d_tile, tile) &&
+         dev_copy(&a->d_cellblk, cellb) && dev_copy(&a->d_gauge, gpre) &&
+         dev_copy(&a->d_ucellp, upre) && dev_copy(&a->gauges, allg) && dev_copy(&a->ucell, ucell) &&
+         dev_copy(&a->uptr, uptr) && dev_copy(&a->uidx, uidx) &&
+         hipMalloc(&a->d_sc, sizeof(AdjScen) * nscen) == hipSuccess;
+    if (ok)
+        for (int s = 0; s < nscen; ++s) {
+            AdjScen& d = a->sc[s];
+            d.slots = a->slots + 3 * s;
+            d.err = a->err + s;
+            d.dts = a->dts + (size_t)s * nsteps;
+            d.vel = a->vel + (size_t)s * nsteps;
+        }
+    if (!ok) {
+        adj_free(a);
+        delete a;
+        return HX_E_NOMEM;
+    }
+    a->hdts.assign((size_t)nscen * nsteps, 0.0);
+    *out = a;
+    return HX_OK;
 }
 
 extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, int nsteps,
@@ -524,85 +800,8 @@ extern "C" int hx_adjoint_create(hx_ctx* c, int solver, const hx_patch* geom, in
     if (!c) return HX_E_STATE;
     if (!out) return HX_E_BADARG;
     *out = nullptr;
-    if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
-    if (!geom || geom->nx < 2 || geom->ny < 2 || geom->dtype != HX_F64) return HX_E_BADARG;
-    if (!(geom->dx > 0.0) || !(geom->dy > 0.0)) return HX_E_BADARG;
-    for (int d = 0; d < 2; ++d)
-        if ((geom->bc_lo[d] != 0 && geom->bc_lo[d] != 1) ||
-            (geom->bc_hi[d] != 0 && geom->bc_hi[d] != 1))
-            return HX_E_BADARG;
-    if (nsteps < 1 || ckpt_every < 1 || ngauge < 0 || (ngauge > 0 && !gauges)) return HX_E_BADARG;
-    const int64_t N = (int64_t)geom->nx * geom->ny;
-    for (int g = 0; g < ngauge; ++g)
-        if (gauges[g] < 0 || gauges[g] >= N) return HX_E_BADARG;
-    HX_TRY(hipSetDevice(c->device));
-    hx_adjoint* a = new (std::nothrow) hx_adjoint();
-    if (!a) return HX_E_NOMEM;
-    a->ctx = c;
-    a->solver = solver;
-    a->nx = geom->nx;
-    a->ny = geom->ny;
-    a->dx = geom->dx;
-    a->dy = geom->dy;
-    for (int d = 0; d < 2; ++d) {
-        a->bc_lo[d] = geom->bc_lo[d];
-        a->bc_hi[d] = geom->bc_hi[d];
-    }
-    a->nsteps = nsteps;
-    a->every = std::min(ckpt_every, nsteps);
-    a->ncp = (nsteps + a->every - 1) / a->every;
-    // the hydrostatic recomputation also runs a segment's last step (for its
-    // pre-friction state) and needs a slot for that step's output
-    a->nseg = std::max(a->every - (solver == HX_SOLVER_HYDROSTATIC_ROE ? 0 : 1), 2);
-    a->ngauge = ngauge;
-    a->N = N;
-    const size_t S = sizeof(double) * 3 * (size_t)N;
-    bool ok = hipMalloc(&a->ckpt, S * a->ncp) == hipSuccess &&
-              hipMalloc(&a->seg, S * a->nseg) == hipSuccess &&
-              hipMalloc(&a->lam, 2 * S) == hipSuccess &&
-              hipMalloc(&a->hist, sizeof(double) * 2 * (size_t)a->nsteps) == hipSuccess;
-    if (ok && solver == HX_SOLVER_HYDROSTATIC_ROE)
-        ok = hipMalloc(&a->lun, S) == hipSuccess && hipMalloc(&a->un, S * a->every) == hipSuccess;
-    if (ok && ngauge > 0) {
-        // group gauges by cell (first appearance order) for the seed kernel
-        std::vector<int64_t> ucell;
-        std::vector<std::vector<int>> members;
-        for (int gi = 0; gi < ngauge; ++gi) {
-            size_t u = 0;
-            while (u < ucell.size() && ucell[u] != gauges[gi]) ++u;
-            if (u == ucell.size()) {
-                ucell.push_back(gauges[gi]);
-                members.emplace_back();
-            }
-            members[u].push_back(gi);
-        }
-        std::vector<int> uptr(1, 0), uidx;
-        for (auto& m : members) {
-            uidx.insert(uidx.end(), m.begin(), m.end());
-            uptr.push_back((int)uidx.size());
-        }
-        a->nu = (int)ucell.size();
-        ok = hipMalloc(&a->gauges, sizeof(int64_t) * ngauge) == hipSuccess &&
-             hipMemcpy(a->gauges, gauges, sizeof(int64_t) * ngauge, hipMemcpyHostToDevice) ==
-                 hipSuccess &&
-             hipMalloc(&a->ucell, sizeof(int64_t) * a->nu) == hipSuccess &&
-             hipMemcpy(a->ucell, ucell.data(), sizeof(int64_t) * a->nu, hipMemcpyHostToDevice) ==
-                 hipSuccess &&
-             hipMalloc(&a->uptr, sizeof(int) * (a->nu + 1)) == hipSuccess &&
-             hipMemcpy(a->uptr, uptr.data(), sizeof(int) * (a->nu + 1), hipMemcpyHostToDevice) ==
-                 hipSuccess &&
-             hipMalloc(&a->uidx, sizeof(int) * ngauge) == hipSuccess &&
-             hipMemcpy(a->uidx, uidx.data(), sizeof(int) * ngauge, hipMemcpyHostToDevice) ==
-                 hipSuccess;
-    }
-    if (!ok) {
-        adj_free(a);
-        delete a;
-        return HX_E_NOMEM;
-    }
-    a->dts.assign(nsteps, 0.0);
-    *out = a;
-    return HX_OK;
+    if (!valid_geom(geom)) return HX_E_BADARG;
+    return hx_adjoint_batch_create(c, solver, 1, geom, nsteps, ckpt_every, &ngauge, &gauges, out);
 }
 
 extern "C" int hx_adjoint_destroy(hx_adjoint* a)
@@ -614,17 +813,123 @@ extern "C" int hx_adjoint_destroy(hx_adjoint* a)
     return HX_OK;
 }
 
-static double* state_ptr(hx_adjoint* a, double* base, int k) { return base + (size_t)k * 3 * a->N; }
-
-static void set_io(StepArgs<double>& s, const double* in, int64_t N, double* ho, double* huo,
-                   double* hvo)
+static Batch batch_of(hx_adjoint* a, const int* start)
 {
-    s.p.h = in;
-    s.p.hu = in + N;
-    s.p.hv = in + 2 * N;
-    s.ho = ho;
-    s.huo = huo;
-    s.hvo = hvo;
+    Batch b{};
+    b.sc = a->d_sc;
+    b.start = start;
+    b.nscen = a->nscen;
+    b.every = a->every;
+    b.nsteps = a->nsteps;
+    return b;
+}
+
+static int upload_table(hx_adjoint* a, hipStream_t hs)
+{
+    HX_TRY(hipMemcpyAsync(a->d_sc, a->sc.data(), sizeof(AdjScen) * a->nscen, hipMemcpyHostToDevice,
+                          hs));
+    return HX_OK;
+}
+
+extern "C" int hx_adjoint_batch_forward(hx_adjoint* a, const hx_patch* states, double cfl,
+                                        double* dt_sched, double* const* series,
+                                        void* const* finals, int32_t* steps_done, void* stream)
+{
+    if (!a) return HX_E_STATE;
+    a->ready = 0;
+    if (!states || !dt_sched || !finals) return HX_E_BADARG;
+    if (steps_done)
+        for (int s = 0; s < a->nscen; ++s) steps_done[s] = 0;
+    for (int s = 0; s < a->nscen; ++s) {
+        const hx_patch* st = &states[s];
+        const AdjScen& d = a->sc[s];
+        int rc = host::check_patch(st, 0);
+        if (rc) return rc;
+        if (st->dtype != HX_F64 || st->nx != d.p.nx || st->ny != d.p.ny || st->pitch != d.p.nx ||
+            st->eta || st->dx != (double)d.dx || st->dy != (double)d.dy)
+            return HX_E_BADARG;
+        if (st->bc_lo[0] != d.p.bl0 || st->bc_lo[1] != d.p.bl1 || st->bc_hi[0] != d.p.bh0 ||
+            st->bc_hi[1] != d.p.bh1)
+            return HX_E_BADARG;
+        if (!finals[3 * s] || !finals[3 * s + 1] || !finals[3 * s + 2]) return HX_E_BADARG;
+        if (a->ngauge[s] > 0 && (!series || !series[s])) return HX_E_BADARG;
+    }
+    const bool use_cfl = cfl > 0.0;
+    if (!use_cfl)
+        for (size_t k = 0; k < (size_t)a->nscen * a->nsteps; ++k)
+            if (!(dt_sched[k] >= 0.0)) return HX_E_BADARG;
+    hx_ctx* c = a->ctx;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t hs = (hipStream_t)stream;
+    for (int s = 0; s < a->nscen; ++s) {
+        AdjScen& d = a->sc[s];
+        d.p.z = (const double*)states[s].z;
+        d.p.n = (const double*)states[s].n;
+        d.in_h = (const double*)states[s].h;
+        d.in_hu = (const double*)states[s].hu;
+        d.in_hv = (const double*)states[s].hv;
+        d.out_h = (double*)finals[3 * s];
+        d.out_hu = (double*)finals[3 * s + 1];
+        d.out_hv = (double*)finals[3 * s + 2];
+        d.series = (a->ngauge[s] > 0) ? series[s] : nullptr;
+    }
+    int rc = upload_table(a, hs);
+    if (rc) return rc;
+    if (!use_cfl)
+        HX_TRY(hipMemcpyAsync(a->dts, dt_sched, sizeof(double) * a->nscen * a->nsteps,
+                              hipMemcpyHostToDevice, hs));
+    Batch cb = batch_of(a, a->d_cellblk);
+    hipLaunchKernelGGL(copy_kernel, dim3(a->total_cellblk), dim3(kThreads), 0, hs, cb, 0);
+    AdjBatch fb{};
+    fb.sc = a->d_sc;
+    fb.strip_start = a->d_strip;
+    fb.nscen = a->nscen;
+    fb.total = a->total_strips;
+    fb.nsteps = a->nsteps;
+    fb.every = a->every;
+    fb.s0 = -1;
+    fb.use_cfl = use_cfl ? 1 : 0;
+    fb.cfl = cfl;
+    if (use_cfl) {
+        // every scenario's slot 0 <- the CFL of its initial state
+        hipLaunchKernelGGL(clear_kernel, dim3((a->nscen + 63) / 64), dim3(64), 0, hs, a->d_sc,
+                           a->nscen);
+        for (int s = 0; s < a->nscen; ++s) {
+            hx_patch p0 = states[s];
+            rc = host::reduce_cfl_f64(&p0, a->sc[s].slots, hs);
+            if (rc) return rc;
+        }
+    }
+    Batch sb = batch_of(a, a->d_gauge);
+    for (int k = 0; k < a->nsteps; ++k) {
+        fb.t = k;
+        fb.ring = k % 3;
+        rc = host::launch_adj_batch_f64(c, a->solver, fb, hs);
+        if (rc) return rc;
+        if (a->total_gauges > 0) {
+            sb.t = k;
+            hipLaunchKernelGGL(sample_kernel, dim3((a->total_gauges + 63) / 64), dim3(64), 0, hs, sb,
+                               (const int64_t*)a->gauges, a->total_gauges);
+        }
+    }
+    HX_TRY(hipGetLastError());
+    std::vector<int> err(a->nscen, 0);
+    if (use_cfl) {
+        HX_TRY(hipMemcpyAsync(err.data(), a->err, sizeof(int) * a->nscen, hipMemcpyDeviceToHost, hs));
+        HX_TRY(hipMemcpyAsync(a->hdts.data(), a->dts, sizeof(double) * a->nscen * a->nsteps,
+                              hipMemcpyDeviceToHost, hs));
+    }
+    HX_TRY(hipStreamSynchronize(hs));
+    bool collapse = false;
+    for (int s = 0; s < a->nscen; ++s) {
+        if (steps_done) steps_done[s] = err[s] ? err[s] - 1 : a->nsteps;
+        collapse = collapse || err[s];
+    }
+    if (collapse) return HX_E_DT_COLLAPSE;
+    if (use_cfl) std::copy(a->hdts.begin(), a->hdts.end(), dt_sched);
+    else std::copy(dt_sched, dt_sched + (size_t)a->nscen * a->nsteps, a->hdts.begin());
+    a->ready = 1;
+    return HX_OK;
 }
 
 extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl, double* dt_sched,
@@ -632,122 +937,83 @@ extern "C" int hx_adjoint_forward(hx_adjoint* a, const hx_patch* st, double cfl,
                                   int32_t* steps_done, void* stream)
 {
     if (!a) return HX_E_STATE;
-    a->ready = 0;
-    if (steps_done) *steps_done = 0;
-    int rc = host::check_patch(st, 0);
-    if (rc) return rc;
-    if (st->dtype != HX_F64 || st->nx != a->nx || st->ny != a->ny || st->pitch != a->nx ||
-        st->eta || st->dx != a->dx || st->dy != a->dy)
-        return HX_E_BADARG;
-    for (int d = 0; d < 2; ++d)
-        if (st->bc_lo[d] != a->bc_lo[d] || st->bc_hi[d] != a->bc_hi[d]) return HX_E_BADARG;
-    if (!dt_sched || !ho || !huo || !hvo || (a->ngauge > 0 && !series)) return HX_E_BADARG;
-    const bool use_cfl = cfl > 0.0;
-    if (!use_cfl)
-        for (int s = 0; s < a->nsteps; ++s)
-            if (!(dt_sched[s] >= 0.0)) return HX_E_BADARG;
+    if (a->nscen != 1) return HX_E_STATE;
+    if (!st || !ho || !huo || !hvo) return HX_E_BADARG;
+    void* fin[3] = {ho, huo, hvo};
+    return hx_adjoint_batch_forward(a, st, cfl, dt_sched, &series, fin, steps_done, stream);
+}
+
+extern "C" int hx_adjoint_batch_backward(hx_adjoint* a, const double* const* g_series,
+                                         const double* const* g_final, double* const* g_state,
+                                         double* const* g_z, double* const* g_n, void* stream)
+{
+    if (!a) return HX_E_STATE;
+    if (!a->ready) return HX_E_STATE;
+    if (!g_state || !g_z || !g_n) return HX_E_BADARG;
+    for (int s = 0; s < a->nscen; ++s)
+        if (!g_state[s] || !g_z[s] || !g_n[s]) return HX_E_BADARG;
     hx_ctx* c = a->ctx;
     HX_TRY(hipSetDevice(c->device));
     hipStream_t hs = (hipStream_t)stream;
-    const int64_t N = a->N;
-    a->z = (const double*)st->z;
-    a->n = (const double*)st->n;
-    double* c0 = a->ckpt;
-    HX_TRY(hipMemcpyAsync(c0, st->h, sizeof(double) * N, hipMemcpyDeviceToDevice, hs));
-    HX_TRY(hipMemcpyAsync(c0 + N, st->hu, sizeof(double) * N, hipMemcpyDeviceToDevice, hs));
-    HX_TRY(hipMemcpyAsync(c0 + 2 * N, st->hv, sizeof(double) * N, hipMemcpyDeviceToDevice, hs));
-    StepArgs<double> s{};
-    s.p = host::view_of<double>(st);
-    s.opitch = a->nx;
-    host::fill_geometry(s, a->solver, host::eff_rows(c, a->solver, HX_F64, a->nx, a->ny), a->dx, a->dy);
-    c->mark.valid = 0;       // the context's slots now follow this rollout
-    if (use_cfl) {
-        rc = host::clear_slots(c->slots, 3, c->err, hs);
-        if (rc) return rc;
-        hx_patch p0 = *st;
-        p0.h = c0;
-        p0.hu = c0 + N;
-        p0.hv = c0 + 2 * N;
-        rc = host::reduce_cfl_f64(&p0, c->slots, hs);
-        if (rc) return rc;
-        s.cfl = cfl;
-        s.err = c->err;
-        s.dt_hist = a->hist;
-        s.vel_hist = a->hist + a->nsteps;
+    for (int s = 0; s < a->nscen; ++s) {
+        AdjScen& d = a->sc[s];
+        d.gs = (g_series && a->ngauge[s] > 0) ? g_series[s] : nullptr;
+        d.gfin = g_final ? g_final[s] : nullptr;
+        d.gstate = g_state[s];
+        d.gz = g_z[s];
+        d.gn = g_n[s];
     }
-    const double* in = c0;
-    for (int k = 0; k < a->nsteps; ++k) {
-        double *oh, *ohu, *ohv;
-        if (k == a->nsteps - 1) {
-            oh = (double*)ho;
-            ohu = (double*)huo;
-            ohv = (double*)hvo;
-        } else {
-            double* o = ((k + 1) % a->every == 0) ? state_ptr(a, a->ckpt, (k + 1) / a->every)
-                                                  : state_ptr(a, a->seg, (k + 1) & 1);
-            oh = o;
-            ohu = o + N;
-            ohv = o + 2 * N;
+    int rc = upload_table(a, hs);
+    if (rc) return rc;
+    int lam_in = 0;
+    Batch cb = batch_of(a, a->d_cellblk);
+    cb.lam_in = lam_in;
+    hipLaunchKernelGGL(init_kernel, dim3(a->total_cellblk), dim3(kThreads), 0, hs, cb);
+    const bool hyd = (a->solver == HX_SOLVER_HYDROSTATIC_ROE);
+    AdjBatch fb{};
+    fb.sc = a->d_sc;
+    fb.strip_start = a->d_strip;
+    fb.nscen = a->nscen;
+    fb.total = a->total_strips;
+    fb.nsteps = a->nsteps;
+    fb.every = a->every;
+    fb.use_cfl = 0;
+    Batch tb = batch_of(a, a->d_tile), ub = batch_of(a, a->d_ucellp);
+    for (int cp = a->ncp - 1; cp >= 0; --cp) {
+        const int s0 = cp * a->every, s1 = std::min(s0 + a->every, a->nsteps);
+        // recompute the segment's states from its checkpoint; the
+        // hydrostatic steps also store their states before friction, the
+        // segment's last step included (its output lands in a spare slot)
+        fb.s0 = s0;
+        for (int t = s0; t < (hyd ? s1 : s1 - 1); ++t) {
+            fb.t = t;
+            rc = host::launch_adj_batch_f64(c, a->solver, fb, hs);
+            if (rc) return rc;
         }
-        set_io(s, in, N, oh, ohu, ohv);
-        if (use_cfl) {
-            s.rd = c->slots + (k % 3);
-            s.acc = c->slots + ((k + 1) % 3);
-            s.clr = c->slots + ((k + 2) % 3);
-            s.step = k;
-        } else {
-            s.rd = nullptr;
-            s.dt_host = dt_sched[k];
+        for (int t = s1 - 1; t >= s0; --t) {
+            if (g_series && a->total_u > 0) {
+                ub.t = t;
+                ub.lam_in = lam_in;
+                hipLaunchKernelGGL(seed_kernel, dim3((a->total_u + 63) / 64), dim3(64), 0, hs, ub,
+                                   (const int64_t*)a->ucell, (const int*)a->uptr,
+                                   (const int*)a->uidx, a->total_u);
+            }
+            if (hyd) {
+                cb.t = t;
+                cb.s0 = s0;
+                cb.lam_in = lam_in;
+                hipLaunchKernelGGL(post_kernel, dim3(a->total_cellblk), dim3(kThreads), 0, hs, cb);
+            }
+            tb.t = t;
+            tb.s0 = s0;
+            tb.lam_in = lam_in;
+            if (hyd) hipLaunchKernelGGL(htile_kernel, dim3(a->total_tiles), dim3(kThreads), 0, hs, tb);
+            else hipLaunchKernelGGL(rtile_kernel, dim3(a->total_tiles), dim3(kThreads), 0, hs, tb);
+            lam_in ^= 1;
         }
-        rc = host::launch_step_f64(c, a->solver, s, hs);
-        if (rc) return rc;
-        if (a->ngauge > 0) {
-            hipLaunchKernelGGL(sample_kernel, dim3((a->ngauge + 63) / 64), dim3(64), 0, hs, oh,
-                               a->gauges, series + (size_t)k * a->ngauge, a->ngauge);
-            HX_TRY(hipGetLastError());
-        }
-        in = oh;   // oh, ohu, ohv are planes of one state except for the last step
     }
-    if (use_cfl) {
-        int err = 0;
-        HX_TRY(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, hs));
-        HX_TRY(hipMemcpyAsync(a->dts.data(), a->hist, sizeof(double) * a->nsteps,
-                              hipMemcpyDeviceToHost, hs));
-        HX_TRY(hipStreamSynchronize(hs));
-        if (err) {
-            if (steps_done) *steps_done = err - 1;
-            return HX_E_DT_COLLAPSE;
-        }
-        std::copy(a->dts.begin(), a->dts.end(), dt_sched);
-    } else {
-        std::copy(dt_sched, dt_sched + a->nsteps, a->dts.begin());
-    }
-    if (steps_done) *steps_done = a->nsteps;
-    a->ready = 1;
-    return HX_OK;
-}
-
-// One adjoint step: lam (adjoint of the step's output) -> lam_new (adjoint
-// of its input U), accumulating the bed and roughness gradients.  un: the
-// step's state after the flux update (hydrostatic), stored by the forward
-// recomputation.
-static int adj_step(hx_adjoint* a, const double* U, const double* un, double dt,
-                    const double* lam, double* lam_new, double* gz, double* gn, hipStream_t hs)
-{
-    const int64_t N = a->N;
-    State s{U, U + N, U + 2 * N, a->z, a->n};
-    Geo g{a->nx, a->ny, a->bc_lo[0], a->bc_lo[1], a->bc_hi[0], a->bc_hi[1]};
-    const dim3 tiles((unsigned)((a->nx + kTX - 1) / kTX), (unsigned)((a->ny + kTY - 1) / kTY));
-    if (a->solver == HX_SOLVER_HYDROSTATIC_ROE) {
-        const unsigned pb = (unsigned)std::min<int64_t>(nblocks(N), 8192);
-        hipLaunchKernelGGL(post_kernel, dim3(pb), dim3(kThreads), 0, hs, un, (const double*)a->n,
-                           lam, a->lun, gn, dt, N);
-        hipLaunchKernelGGL(htile_kernel, tiles, dim3(kThreads), 0, hs, s, g, dt, a->dx, a->dy,
-                           (const double*)a->lun, lam_new, gz);
-    } else {
-        hipLaunchKernelGGL(rtile_kernel, tiles, dim3(kThreads), 0, hs, s, g, dt, a->dx, a->dy,
-                           lam, lam_new, gz, gn);
-    }
+    cb.lam_in = lam_in;
+    hipLaunchKernelGGL(copy_kernel, dim3(a->total_cellblk), dim3(kThreads), 0, hs, cb, 1);
     return host::hip_status(hipGetLastError());
 }
 
@@ -755,70 +1021,6 @@ extern "C" int hx_adjoint_backward(hx_adjoint* a, const double* g_series, const 
                                    double* g_state, double* g_z, double* g_n, void* stream)
 {
     if (!a) return HX_E_STATE;
-    if (!a->ready) return HX_E_STATE;
-    if (!g_state || !g_z || !g_n) return HX_E_BADARG;
-    hx_ctx* c = a->ctx;
-    HX_TRY(hipSetDevice(c->device));
-    hipStream_t hs = (hipStream_t)stream;
-    const int64_t N = a->N;
-    double* lam = a->lam;
-    double* lam2 = a->lam + 3 * N;
-    const unsigned fb = (unsigned)std::min<int64_t>(nblocks(3 * N), 8192);
-    if (g_final)
-        HX_TRY(hipMemcpyAsync(lam, g_final, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, hs));
-    else
-        hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(kThreads), 0, hs, lam, 3 * N, 0.0);
-    hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(kThreads), 0, hs, g_z, N, 0.0);
-    hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(kThreads), 0, hs, g_n, N, 0.0);
-    HX_TRY(hipGetLastError());
-    StepArgs<double> s{};
-    hx_patch geom{};
-    geom.nx = a->nx;
-    geom.ny = a->ny;
-    geom.dtype = HX_F64;
-    geom.pitch = a->nx;
-    geom.z = a->z;
-    geom.n = a->n;
-    geom.dx = a->dx;
-    geom.dy = a->dy;
-    for (int d = 0; d < 2; ++d) {
-        geom.bc_lo[d] = a->bc_lo[d];
-        geom.bc_hi[d] = a->bc_hi[d];
-    }
-    s.p = host::view_of<double>(&geom);
-    s.opitch = a->nx;
-    host::fill_geometry(s, a->solver, host::eff_rows(c, a->solver, HX_F64, a->nx, a->ny), a->dx, a->dy);
-    for (int cp = a->ncp - 1; cp >= 0; --cp) {
-        const int s0 = cp * a->every, s1 = std::min(s0 + a->every, a->nsteps);
-        auto state = [&](int t) -> double* {
-            return (t == s0) ? state_ptr(a, a->ckpt, cp) : state_ptr(a, a->seg, t - s0 - 1);
-        };
-        // recompute the segment's states from its checkpoint; the
-        // hydrostatic steps also store their states before friction, the
-        // segment's last step included (its output lands in a spare slot)
-        const bool hyd = (a->solver == HX_SOLVER_HYDROSTATIC_ROE);
-        for (int t = s0; t < (hyd ? s1 : s1 - 1); ++t) {
-            double* o = state(t + 1);
-            set_io(s, state(t), N, o, o + N, o + 2 * N);
-            s.dt_host = a->dts[t];
-            s.uno = hyd ? a->un + (size_t)(t - s0) * 3 * N : nullptr;
-            s.uno_plane = N;
-            int rc = host::launch_step_f64(c, a->solver, s, hs);
-            s.uno = nullptr;
-            if (rc) return rc;
-        }
-        for (int t = s1 - 1; t >= s0; --t) {
-            if (g_series && a->ngauge > 0) {
-                hipLaunchKernelGGL(seed_kernel, dim3((a->nu + 63) / 64), dim3(64), 0, hs, lam,
-                                   a->ucell, a->uptr, a->uidx,
-                                   g_series + (size_t)t * a->ngauge, a->nu);
-            }
-            const double* un = a->un ? a->un + (size_t)(t - s0) * 3 * N : nullptr;
-            int rc = adj_step(a, state(t), un, a->dts[t], lam, lam2, g_z, g_n, hs);
-            if (rc) return rc;
-            std::swap(lam, lam2);
-        }
-    }
-    HX_TRY(hipMemcpyAsync(g_state, lam, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, hs));
-    return HX_OK;
+    if (a->nscen != 1) return HX_E_STATE;
+    return hx_adjoint_batch_backward(a, &g_series, &g_final, &g_state, &g_z, &g_n, stream);
 }

@@ -196,6 +196,7 @@ inline void fill_geometry(StepArgs<R>& a, int solver, int rows, double dx, doubl
 // Defined in hx_kernels.hip.
 int check_patch(const hx_patch* in, int need_ghost);
 int launch_step_f64(const hx_ctx* c, int solver, const StepArgs<double>& a, hipStream_t st);
+int launch_adj_batch_f64(const hx_ctx* c, int solver, const AdjBatch& b, hipStream_t st);
 int reduce_cfl_f64(const hx_patch* in, CflSlot* slot, hipStream_t st);
 int clear_slots(CflSlot* s, int n, int* err, hipStream_t st);
 

@@ -724,6 +724,14 @@ int hx::host::launch_step_f64(const hx_ctx* c, int solver, const StepArgs<double
     return launch<double>(c, solver, HX_F64, a, st);
 }
 
+int hx::host::launch_adj_batch_f64(const hx_ctx* c, int solver, const AdjBatch& b, hipStream_t st)
+{
+    const int occ = eff_occ(c, solver, HX_F64);
+    const int rc = (c->arith == HX_ARITH_FAST) ? fast::launch_adj_batch(solver, occ, b, st)
+                                               : exact::launch_adj_batch(solver, occ, b, st);
+    return rc ? HX_E_HIP : HX_OK;
+}
+
 int hx::host::reduce_cfl_f64(const hx_patch* in, CflSlot* slot, hipStream_t st)
 {
     return launch_cfl<double>(in, slot, st);

@@ -106,6 +106,57 @@ struct FaceReq {
     const int* err;     // non-null and set: nothing is done (a stopped run)
 };
 
+// Scenario table of a batched adjoint rollout (FP64): one entry per
+// scenario, with the buffers its forward and its segment recomputations
+// step through.  State k of the forward lives in ckpt[k / every] when k is a
+// multiple of every, else in seg[k & 1]; inside the recomputation of the
+// segment starting at s0, state t > s0 lives in seg[t - s0 - 1] and the
+// hydrostatic step t stores its pre-friction state in un[t - s0].  Each
+// slot is 3 planes of N doubles.
+struct AdjScen {
+    PatchView<double> p;       // geometry, BCs and terrain (h/hu/hv unused)
+    double dx, dy, rdx, rdy;
+    int dx_pow2;
+    int rows, nsx, nstrips, strip0;     // the advance's strips
+    int ntx, ntiles, tile0;             // the adjoint's tiles
+    int ngauge, gauge0, nu, u0;         // gauges and distinct gauge cells
+    int64_t N;
+    double* ckpt;
+    double* seg;
+    double* un;
+    double* lam;               // 2 slots: adjoint ping-pong
+    double* lun;               // 1 slot (hydrostatic)
+    double* out_h;             // the forward's final state (caller's planes)
+    double* out_hu;
+    double* out_hv;
+    double* series;            // the forward's gauge series (caller's), [step][gauge]
+    const double* in_h;        // the forward's initial state (caller's planes)
+    const double* in_hu;
+    const double* in_hv;
+    const double* gs;          // the backward's series adjoint (caller's, optional)
+    const double* gfin;        // the backward's final-state adjoint (caller's, optional)
+    double* gstate;            // the backward's gradients (caller's): initial state,
+    double* gz;                //   bed and roughness
+    double* gn;
+    double* dts;               // [nsteps] dt of every step
+    double* vel;               // [nsteps] max|vel| before every step (CFL mode)
+    CflSlot* slots;            // 3-slot CFL ring
+    int* err;                  // dt collapse: step + 1
+};
+
+// One launch over a batch: step t of the forward (s0 < 0) or of the
+// recomputation of the segment that starts at s0.
+struct AdjBatch {
+    const AdjScen* sc;
+    const int* strip_start;    // [nscen + 1]
+    int nscen, total;
+    int nsteps, every;
+    int t, s0;
+    int use_cfl;               // forward only: dt from the CFL ring, else dts[t]
+    int ring;                  // t modulo 3
+    double cfl;
+};
+
 // Occupancy variants compiled for the advance kernel (minimum waves per
 // SIMD requested from the register allocator).
 constexpr int kOccVariants = 3;
@@ -173,10 +224,12 @@ struct EnsArgs {
 namespace exact {
 template <typename R>
 int launch_ensemble(int solver, int occ, const EnsArgs<R>& e, int grid, hipStream_t st);
+int launch_adj_batch(int solver, int occ, const AdjBatch& b, hipStream_t st);
 }
 namespace fast {
 template <typename R>
 int launch_ensemble(int solver, int occ, const EnsArgs<R>& e, int grid, hipStream_t st);
+int launch_adj_batch(int solver, int occ, const AdjBatch& b, hipStream_t st);
 }
 
 }  // namespace hx

@@ -154,6 +154,144 @@ class Rollout:
             pass
 
 
+class RolloutBatch:
+    """Many FP64 rollouts (scenarios of uneven sizes, each with its own
+    terrain, gauges and CFL dt) advanced and differentiated together:
+    hx_adjoint_batch_* makes one launch per step and per adjoint kernel for
+    the whole batch.  Each scenario's results equal a :class:`Rollout` of
+    that scenario alone, bit for bit.
+
+    scenarios: a list of dicts with nx, ny, dx, dy, bc_lo, bc_hi, gauges; all
+    share model, nsteps, ckpt_every and cfl.  freeze_dt as in Rollout."""
+
+    def __init__(self, model: str, scenarios: Sequence[dict], nsteps: int, ckpt_every: int = 20,
+                 cfl: float = 0.45, freeze_dt: bool = False, device: Optional[int] = None):
+        if model not in MODELS:
+            raise ValueError(f"unknown model {model!r}")
+        if not scenarios:
+            raise ValueError("a batch needs at least one scenario")
+        self.model = model
+        self.nsteps, self.ckpt_every, self.cfl = int(nsteps), int(ckpt_every), float(cfl)
+        self.freeze_dt = bool(freeze_dt)
+        self.dt_schedule = None
+        self.device = torch.cuda.current_device() if device is None else int(device)
+        self.sc = [dict(s) for s in scenarios]
+        n = len(self.sc)
+        geoms = (L.HxPatch * n)()
+        self._gauges = []
+        for k, s in enumerate(self.sc):
+            g = geoms[k]
+            g.nx, g.ny, g.ng, g.dtype = int(s["nx"]), int(s["ny"]), 0, L.HX_F64
+            g.pitch = g.nx
+            g.dx, g.dy = float(s["dx"]), float(s["dy"])
+            g.bc_lo[0], g.bc_lo[1] = s["bc_lo"]
+            g.bc_hi[0], g.bc_hi[1] = s["bc_hi"]
+            self._gauges.append(np.ascontiguousarray(np.asarray(s["gauges"], np.int64).reshape(-1)))
+        ng = (C.c_int * n)(*[a.size for a in self._gauges])
+        gp = (C.POINTER(C.c_int64) * n)(*[a.ctypes.data_as(C.POINTER(C.c_int64)) for a in self._gauges])
+        self._geoms = geoms
+        self._h = C.c_void_p()
+        L.check(L.load().hx_adjoint_batch_create(context(self.device), MODELS[model], n, geoms,
+                                                 self.nsteps, self.ckpt_every, ng, gp,
+                                                 C.byref(self._h)), "hx_adjoint_batch_create")
+        self.last_dt = None
+
+    def forward_raw(self, U0s: Sequence[torch.Tensor], terrains: Sequence[torch.Tensor]):
+        """Per scenario U0 (3, ny, nx) and terrain (2, ny, nx), contiguous FP64
+        device tensors.  Returns (series list [nsteps, ngauge], finals list)."""
+        n = len(self.sc)
+        if len(U0s) != n or len(terrains) != n:
+            raise ValueError("one state and one terrain per scenario")
+        states = (L.HxPatch * n)()
+        series, finals = [], []
+        ser = (C.c_void_p * n)()
+        fin = (C.c_void_p * (3 * n))()
+        for k, (s, U, T) in enumerate(zip(self.sc, U0s, terrains)):
+            ny, nx = int(s["ny"]), int(s["nx"])
+            for name, t, m in (("state", U, 3), ("terrain", T, 2)):
+                if (t.dtype != torch.float64 or not t.is_cuda or tuple(t.shape) != (m, ny, nx)
+                        or not t.is_contiguous()):
+                    raise ValueError(f"scenario {k}: {name} must be a contiguous float64 device "
+                                     f"tensor of shape ({m}, {ny}, {nx})")
+            plane = nx * ny * 8
+            p = states[k]
+            p.nx, p.ny, p.ng, p.dtype, p.pitch = nx, ny, 0, L.HX_F64, nx
+            b, t = U.data_ptr(), T.data_ptr()
+            p.h, p.hu, p.hv, p.eta, p.z, p.n = b, b + plane, b + 2 * plane, None, t, t + plane
+            p.dx, p.dy = float(s["dx"]), float(s["dy"])
+            p.bc_lo[0], p.bc_lo[1] = s["bc_lo"]
+            p.bc_hi[0], p.bc_hi[1] = s["bc_hi"]
+            se = torch.empty((self.nsteps, self._gauges[k].size), dtype=torch.float64, device=U.device)
+            out = torch.empty_like(U)
+            series.append(se)
+            finals.append(out)
+            ser[k] = se.data_ptr() if se.numel() else None
+            o = out.data_ptr()
+            fin[3 * k], fin[3 * k + 1], fin[3 * k + 2] = o, o + plane, o + 2 * plane
+        if self.dt_schedule is not None:
+            sched, cfl = self.dt_schedule.copy(), -1.0
+        else:
+            sched, cfl = np.zeros((n, self.nsteps)), self.cfl
+        done = (C.c_int32 * n)()
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        rc = L.load().hx_adjoint_batch_forward(self._h, states, cfl, sched.ctypes.data_as(_dp), ser,
+                                               fin, done, st)
+        if rc == L.HX_E_DT_COLLAPSE:
+            e = L.DtCollapse(rc, "hx_adjoint_batch_forward")
+            e.done = np.frombuffer(done, dtype=np.int32).copy()
+            raise e
+        L.check(rc, "hx_adjoint_batch_forward")
+        self.last_dt = sched
+        if self.freeze_dt and self.dt_schedule is None:
+            self.dt_schedule = sched.copy()
+        return series, finals
+
+    def backward_raw(self, g_series: Optional[Sequence[Optional[torch.Tensor]]],
+                     g_finals: Optional[Sequence[Optional[torch.Tensor]]]):
+        """Adjoint of the last forward.  Returns (g_state list, g_terrain list)."""
+        n = len(self.sc)
+        dev = torch.device("cuda", self.device)
+        keep = []
+
+        def arr(ts):
+            if ts is None:
+                return None
+            a = (C.c_void_p * n)()
+            for k, t in enumerate(ts):
+                if t is not None:
+                    t = t.to(dev, torch.float64).contiguous()
+                    keep.append(t)
+                    a[k] = t.data_ptr()
+            return a
+        gs, gf = arr(g_series), arr(g_finals)
+        g_state, g_ter = [], []
+        a_st, a_z, a_n = (C.c_void_p * n)(), (C.c_void_p * n)(), (C.c_void_p * n)()
+        for k, s in enumerate(self.sc):
+            ny, nx = int(s["ny"]), int(s["nx"])
+            gst = torch.empty((3, ny, nx), dtype=torch.float64, device=dev)
+            gt = torch.empty((2, ny, nx), dtype=torch.float64, device=dev)
+            g_state.append(gst)
+            g_ter.append(gt)
+            a_st[k] = gst.data_ptr()
+            a_z[k] = gt.data_ptr()
+            a_n[k] = gt.data_ptr() + nx * ny * 8
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        rc = L.load().hx_adjoint_batch_backward(self._h, gs, gf, a_st, a_z, a_n, st)
+        L.check(rc, "hx_adjoint_batch_backward")
+        return g_state, g_ter
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.load().hx_adjoint_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class ShallowWaterRollout(torch.autograd.Function):
     """(h0, hu0, hv0, z, n) -> (gauge series, h, hu, hv) through
     ``rollout.nsteps`` steps; backward = hand-written adjoint with

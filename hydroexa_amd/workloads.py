@@ -148,3 +148,32 @@ def ensemble(nscen: int = 256, side_min: int = 128, side_max: int = 1024, seed: 
                             zero.copy(), dx, dy, (1, 1), (1, 1), cfl, steps,
                             meta={"dam": xd, "ratio": r, "bump": (A, sg, cx, cy)}))
     return out
+
+
+def adjoint_ensemble(nscen: int = 128, side_min: int = 128, side_max: int = 512, seed: int = SEED,
+                     steps: int = 100, cfl: float = 0.45, ngauge: int = 8, manning: float = 0.03):
+    """Scenarios for the batched forward + adjoint: the dam breaks of
+    ensemble() (seeded side, dam position, depth ratio and Gaussian bump)
+    on the hydrostatic solver with a seeded Manning field
+    (manning (1 + 0.5 U[0, 1))), and ngauge seeded gauges per scenario at
+    wet cells within the waves' reach of the dam (cfl / 2 cells per step).
+    Returns (workloads, gauges): gauges[s] holds flat cell indices."""
+    ws = ensemble(nscen, side_min, side_max, seed=seed, steps=steps, cfl=cfl,
+                  solver="HydrostaticRoe")
+    gauges = []
+    for s, w in enumerate(ws):
+        n = w.h.shape[0]
+        u = splitmix64_uniform(seed + 1000 + s, n * n + 2 * 64 * ngauge)
+        w.n = manning * (1.0 + 0.5 * u[:n * n].reshape(n, n))
+        reach = max(4, int(0.5 * cfl * steps))
+        dam = int(w.meta["dam"] * n)
+        g = []
+        k = n * n
+        while len(g) < ngauge and k + 1 < u.size:
+            i = min(n - 1, max(0, dam + int((-1.0 + 2.0 * u[k]) * reach)))
+            j = min(n - 1, int(u[k + 1] * n))
+            k += 2
+            if w.h[j, i] > 0.05 and j * n + i not in g:
+                g.append(j * n + i)
+        gauges.append(np.asarray(g, dtype=np.int64))
+    return ws, gauges

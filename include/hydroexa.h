@@ -502,6 +502,37 @@ int hx_adjoint_forward(hx_adjoint *adj, const hx_patch *state, double cfl, doubl
 int hx_adjoint_backward(hx_adjoint *adj, const double *g_series, const double *g_final,
                         double *g_state, double *g_z, double *g_n, void *stream);
 
+/*
+ * Batched forward + adjoint over nscen independent rollouts (flood
+ * scenarios of uneven sizes): one launch per forward step, per recomputed
+ * step and per adjoint kernel covers every scenario, each with its own
+ * geometry, terrain, gauges, dt schedule and checkpoints.  No reference
+ * counterpart (the reference neither batches nor differentiates).  All
+ * scenarios share the solver, nsteps and ckpt_every, and the results of a
+ * scenario equal the one-scenario object's (hx_adjoint_create is the
+ * nscen = 1 case) bit for bit.
+ *
+ * create: geoms[nscen] as hx_adjoint_create's geom; ngauge[nscen] and
+ * gauges[nscen] (host arrays of flat cell indices).
+ * forward: states[nscen] as hx_adjoint_forward's state; dt_sched (host,
+ * nscen x nsteps, scenario-major) receives (cfl > 0) or gives (cfl <= 0)
+ * the schedules; series[nscen] (device, nsteps x ngauge[s]; entries of
+ * scenarios without gauges may be null); finals[3 nscen] (device planes
+ * h, hu, hv per scenario); steps_done[nscen] (host, optional).  A scenario
+ * whose dt collapses stops and HX_E_DT_COLLAPSE is returned (no backward).
+ * backward: per-scenario arrays of device pointers; g_series and g_final
+ * (the arrays, or their entries) may be null.
+ */
+int hx_adjoint_batch_create(hx_ctx *ctx, int solver, int nscen, const hx_patch *geoms, int nsteps,
+                            int ckpt_every, const int *ngauge, const int64_t *const *gauges,
+                            hx_adjoint **out);
+int hx_adjoint_batch_forward(hx_adjoint *adj, const hx_patch *states, double cfl,
+                             double *dt_sched, double *const *series, void *const *finals,
+                             int32_t *steps_done, void *stream);
+int hx_adjoint_batch_backward(hx_adjoint *adj, const double *const *g_series,
+                              const double *const *g_final, double *const *g_state,
+                              double *const *g_z, double *const *g_n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
