@@ -8,7 +8,12 @@ Parity pinning status per function (the checks live in tests/test_oracle_*.py):
 
 * ``roe_face``           pinned: consistency, upwind (supersonic) limits,
                           mirror/rotation symmetry, lake-at-rest balance,
-                          friction closed form.
+                          friction closed form; branch by branch: the
+                          entropy fix on stationary expansion and
+                          admissible shocks, the positivity guard's star
+                          depths read back from the outputs (both branches
+                          and the late-wave case), the emerged-step thrust
+                          on either side.
 * ``hroe_face``          pinned: consistency, upwind limits, symmetry.
 * ``vanleer_slope``      pinned: van Leer's harmonic-mean form, brute force.
 * ``hydro_face``         pinned: flat-bed reduction to ``hroe_face``,
@@ -37,6 +42,11 @@ Parity pinning status per function (the checks live in tests/test_oracle_*.py):
                           order of AMReX's FillPatchTwoLevels / FluxRegister
                           are restated from its published algorithm: parity
                           unpinned at the last bit (AMReX is absent).
+* ``run32`` (the restatement compiled in single precision, hx_oracle_f32.c):
+                          pinned: within float rounding of ``run`` on smooth
+                          flows, lake at rest kept to float round-off.
+* ``amr.AmrOracle`` regrid replay: pinned by the hierarchy tests it checks
+  (recorded events must fall where the regrid cadence puts them).
 * AMReX ghost-fill internals (``FilccCell``): restated, parity unpinned (no
   reference test or vector covers them).
 """
