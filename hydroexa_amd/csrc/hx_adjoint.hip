@@ -116,106 +116,123 @@ __device__ __forceinline__ void put4(double* c, const PC& p)
 // ---------------------------------------------------------------------------
 constexpr int kTX = 64, kTY = 16;
 
-// Hydrostatic: LDS image covers x in [i0 - 2, i0 + kTX + 3) and likewise in
-// y: a reflective high side folds ghost n+1 into cell n-2, which may be the
-// last cell of the tile before the one holding n-1.
-constexpr int kHPX = kTX + 5, kHPY = kTY + 5;
-typedef double HAcc[4][kHPY][kHPX];
+// Hydrostatic tiles are kHX x kHY cells.  The workgroup first stages every
+// cell its faces read -- rows [j0 - 3, j0 + kHY + 2] and columns
+// [i0 - 3, i0 + kHX + 2], as the step sees them (rebuilt depth, edge
+// terrain, and 1/h, u, v from the momenta with their BC flips) -- once in
+// LDS, so each cell is loaded, mapped and divided once instead of by each of
+// the 8 faces that read it.  x faces
+// accumulate into an image of the tile rows over columns [i0 - 2, i0 + kHX
+// + 3), y faces into one of rows [j0 - 2, j0 + kHY + 3) over the tile
+// columns (a reflective high side folds ghost n+1 into cell n-2, which may
+// be the last cell of the tile before the one holding n-1); x and y faces
+// share one flat item list, since they write different images.
+constexpr int kHX = 32, kHY = 16;
+constexpr int kSX = kHX + 6, kSY = kHY + 6;
+constexpr int kAXX = kHX + 5, kAYY = kHY + 5;
+struct HShared {
+    double cell[5][kSY][kSX];     // he, u, v, z, 1/he (cell_prims)
+    double ax[4][kHY][kAXX];
+    double ay[4][kAYY][kHX];
+};
 
-__device__ __forceinline__ void htile_body(HAcc& acc, const State& s, const Geo& g, double dt,
+__device__ __forceinline__ void htile_body(HShared& sh, const State& s, const Geo& g, double dt,
                                            double dx, double dy, const double* __restrict__ lun,
                                            double* __restrict__ lam_new, double* __restrict__ gz,
                                            int bx, int by)
 {
     const int64_t N = (int64_t)g.nx * g.ny;
-    const int i0 = bx * kTX, j0 = by * kTY;
+    const int i0 = bx * kHX, j0 = by * kHY;
     const int t = threadIdx.x;
-    for (int k = t; k < 4 * kHPY * kHPX; k += kThreads) (&acc[0][0][0])[k] = 0.0;
+    // stage the cells (indices -2 .. n+1 on each axis: the faces read no
+    // further) and clear the images
+    for (int k = t; k < kSY * kSX; k += kThreads) {
+        const int ly = k / kSX, lx = k - ly * kSX;
+        const int i = i0 - 3 + lx, j = j0 - 3 + ly;
+        double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (i >= -2 && i <= g.nx + 1 && j >= -2 && j <= g.ny + 1) {
+            const PC p = pcell(s, g, i, j);
+            v[0] = p.he;
+            v[3] = p.z;
+            cell_prims(p.he, p.hu, p.hv, v[4], v[1], v[2]);
+        }
+        for (int m = 0; m < 5; ++m) sh.cell[m][ly][lx] = v[m];
+    }
+    for (int k = t; k < 4 * kHY * kAXX; k += kThreads) (&sh.ax[0][0][0])[k] = 0.0;
+    for (int k = t; k < 4 * kAYY * kHX; k += kThreads) (&sh.ay[0][0][0])[k] = 0.0;
     __syncthreads();
-    // x faces f in [i0 - 1, i0 + kTX + 2) of the tile rows
-    {
-        const double r = dt / dx;
-        constexpr int nf = (kTX + 3) * kTY;
-        for (int base = 0; base < nf; base += kThreads) {
-            const int q = base + t;
-            double cb[16];
-            bool have = false;
-            int lx0 = 0, ly = 0;
-            if (q < nf) {
-                const int row = q / (kTX + 3), fp = q - row * (kTX + 3);
-                const int j = j0 + row, f = i0 - 1 + fp;
-                if (j < g.ny && f >= 0 && f <= g.nx) {
-                    double c[16];
-                    for (int k = 0; k < 4; ++k) put4(c + 4 * k, pcell(s, g, f - 2 + k, j));
-                    double mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0};
-                    const int64_t cl = (int64_t)j * g.nx + f - 1;
-                    if (f >= 1)
-                        for (int k = 0; k < 3; ++k) mL[k] = -r * lun[k * N + cl];
-                    if (f < g.nx)
-                        for (int k = 0; k < 3; ++k) mR[k] = r * lun[k * N + cl + 1];
-                    hydro_face_vjp(c, 1.0, 0.0, mL, mR, cb);
-                    have = true;
-                    lx0 = fp - 1;        // slot k lands on x index f - 2 + k
-                    ly = row + 2;
-                }
+    // x faces f in [i0 - 1, i0 + kHX + 2) of the tile rows, then y faces f in
+    // [j0 - 1, j0 + kHY + 2) of the tile columns
+    constexpr int nX = (kHX + 3) * kHY, nY = (kHY + 3) * kHX;
+    const double rx = dt / dx, ry = dt / dy;
+    for (int base = 0; base < nX + nY; base += kThreads) {
+        const int q = base + t;
+        // the face's cells and output adjoints, gathered per direction; one
+        // VJP call for both, so a wavefront holding x and y faces runs it once
+        double c[20], mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0}, cb[16];
+        int dir = -1, a0 = 0, a1 = 0;     // image coordinates of slot 0
+        if (q < nX) {
+            const int row = q / (kHX + 3), fp = q - row * (kHX + 3);
+            const int j = j0 + row, f = i0 - 1 + fp;
+            if (j < g.ny && f >= 0 && f <= g.nx) {
+                for (int k = 0; k < 4; ++k)
+                    for (int m = 0; m < 5; ++m) c[5 * k + m] = sh.cell[m][row + 3][fp + k];
+                const int64_t cl = (int64_t)j * g.nx + f - 1;
+                if (f >= 1)
+                    for (int k = 0; k < 3; ++k) mL[k] = -rx * lun[k * N + cl];
+                if (f < g.nx)
+                    for (int k = 0; k < 3; ++k) mR[k] = rx * lun[k * N + cl + 1];
+                dir = 0;
+                a0 = row;                // slot k lands on x index f - 2 + k
+                a1 = fp - 1;
             }
-            for (int k = 0; k < 4; ++k) {
-                const int lx = lx0 + k;
-                if (have && lx >= 0 && lx < kHPX)
-                    for (int m = 0; m < 4; ++m) acc[m][ly][lx] += cb[4 * k + m];
-                __syncthreads();
-            }
-        }
-    }
-    // y faces f in [j0 - 1, j0 + kTY + 2) of the tile columns
-    {
-        const double r = dt / dy;
-        constexpr int nf = (kTY + 3) * kTX;
-        for (int base = 0; base < nf; base += kThreads) {
-            const int q = base + t;
-            double cb[16];
-            bool have = false;
-            int ly0 = 0, lx = 0;
-            if (q < nf) {
-                const int gp = q / kTX, cp = q - gp * kTX;
-                const int i = i0 + cp, f = j0 - 1 + gp;
-                if (i < g.nx && f >= 0 && f <= g.ny) {
-                    double c[16];
-                    for (int k = 0; k < 4; ++k) put4(c + 4 * k, pcell(s, g, i, f - 2 + k));
-                    double mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0};
-                    if (f >= 1)
-                        for (int k = 0; k < 3; ++k) mL[k] = -r * lun[k * N + (int64_t)(f - 1) * g.nx + i];
-                    if (f < g.ny)
-                        for (int k = 0; k < 3; ++k) mR[k] = r * lun[k * N + (int64_t)f * g.nx + i];
-                    hydro_face_vjp(c, 0.0, 1.0, mL, mR, cb);
-                    have = true;
-                    ly0 = gp - 1;
-                    lx = cp + 2;
-                }
-            }
-            for (int k = 0; k < 4; ++k) {
-                const int ly = ly0 + k;
-                if (have && ly >= 0 && ly < kHPY)
-                    for (int m = 0; m < 4; ++m) acc[m][ly][lx] += cb[4 * k + m];
-                __syncthreads();
+        } else if (q < nX + nY) {
+            const int qq = q - nX;
+            const int gp = qq / kHX, cp = qq - gp * kHX;
+            const int i = i0 + cp, f = j0 - 1 + gp;
+            if (i < g.nx && f >= 0 && f <= g.ny) {
+                for (int k = 0; k < 4; ++k)
+                    for (int m = 0; m < 5; ++m) c[5 * k + m] = sh.cell[m][gp + k][cp + 3];
+                if (f >= 1)
+                    for (int k = 0; k < 3; ++k) mL[k] = -ry * lun[k * N + (int64_t)(f - 1) * g.nx + i];
+                if (f < g.ny)
+                    for (int k = 0; k < 3; ++k) mR[k] = ry * lun[k * N + (int64_t)f * g.nx + i];
+                dir = 1;
+                a0 = gp - 1;             // slot k lands on y index f - 2 + k
+                a1 = cp;
             }
         }
+        if (dir >= 0) hydro_face_vjp(c, dir == 0 ? 1.0 : 0.0, dir == 0 ? 0.0 : 1.0, mL, mR, cb);
+        for (int k = 0; k < 4; ++k) {
+            if (dir == 0) {
+                const int lx = a1 + k;
+                if (lx >= 0 && lx < kAXX)
+                    for (int m = 0; m < 4; ++m) sh.ax[m][a0][lx] += cb[4 * k + m];
+            } else if (dir == 1) {
+                const int ly = a0 + k;
+                if (ly >= 0 && ly < kAYY)
+                    for (int m = 0; m < 4; ++m) sh.ay[m][ly][a1] += cb[4 * k + m];
+            }
+            __syncthreads();
+        }
     }
-    // gather
-    for (int k = t; k < kTX * kTY; k += kThreads) {
-        const int ci = k % kTX, cj = k / kTX;
+    // gather: the tile cell's own position (x and y images) and the ghost
+    // positions filled from it (x ghosts in the x image, y ghosts in the y
+    // image), with the ghost-fill and rebuild chain rules; a position's depth
+    // was rebuilt from eta where its staged depth is positive
+    for (int k = t; k < kHX * kHY; k += kThreads) {
+        const int ci = k % kHX, cj = k / kHX;
         const int i = i0 + ci, j = j0 + cj;
         if (i >= g.nx || j >= g.ny) continue;
         const int64_t c = (int64_t)j * g.nx + i;
         double l0 = lun[c], l1 = lun[N + c], l2 = lun[2 * N + c];   // stored-state path
         double eb = 0.0, zb = 0.0;
         {
-            const PC p = pcell(s, g, i, j);
-            const double he = acc[0][cj + 2][ci + 2];
-            l1 += acc[1][cj + 2][ci + 2];
-            l2 += acc[2][cj + 2][ci + 2];
-            zb += acc[3][cj + 2][ci + 2];
-            if (0.0 < p.x) { eb += he; zb -= he; }
+            const double he = sh.ax[0][cj][ci + 2] + sh.ay[0][cj + 2][ci];
+            l1 += sh.ax[1][cj][ci + 2] + sh.ay[1][cj + 2][ci];
+            l2 += sh.ax[2][cj][ci + 2] + sh.ay[2][cj + 2][ci];
+            zb += sh.ax[3][cj][ci + 2] + sh.ay[3][cj + 2][ci];
+            if (0.0 < sh.cell[0][cj + 3][ci + 3]) { eb += he; zb -= he; }
         }
         if (i <= 1 || i >= g.nx - 2) {
             const int gx[4] = {-2, -1, g.nx, g.nx + 1};
@@ -225,17 +242,17 @@ __device__ __forceinline__ void htile_body(HAcc& acc, const State& s, const Geo&
                 map1(gx[u], g.nx, g.bl0, g.bh0, src, edge, flip);
                 if (src != i && edge != i) continue;
                 const int lx = gx[u] - (i0 - 2);
-                const PC p = pcell(s, g, gx[u], j);
-                const double he = acc[0][cj + 2][lx];
+                const double he = sh.ax[0][cj][lx];
+                const bool wet = 0.0 < sh.cell[0][cj + 3][lx + 1];
                 if (src == i) {
-                    const double hu = acc[1][cj + 2][lx];
+                    const double hu = sh.ax[1][cj][lx];
                     l1 += flip ? -hu : hu;
-                    l2 += acc[2][cj + 2][lx];
-                    if (0.0 < p.x) eb += he;
+                    l2 += sh.ax[2][cj][lx];
+                    if (wet) eb += he;
                 }
                 if (edge == i) {
-                    zb += acc[3][cj + 2][lx];
-                    if (0.0 < p.x) zb -= he;
+                    zb += sh.ax[3][cj][lx];
+                    if (wet) zb -= he;
                 }
             }
         }
@@ -247,17 +264,17 @@ __device__ __forceinline__ void htile_body(HAcc& acc, const State& s, const Geo&
                 map1(gy[u], g.ny, g.bl1, g.bh1, src, edge, flip);
                 if (src != j && edge != j) continue;
                 const int ly = gy[u] - (j0 - 2);
-                const PC p = pcell(s, g, i, gy[u]);
-                const double he = acc[0][ly][ci + 2];
+                const double he = sh.ay[0][ly][ci];
+                const bool wet = 0.0 < sh.cell[0][ly + 1][ci + 3];
                 if (src == j) {
-                    l1 += acc[1][ly][ci + 2];
-                    const double hv = acc[2][ly][ci + 2];
+                    l1 += sh.ay[1][ly][ci];
+                    const double hv = sh.ay[2][ly][ci];
                     l2 += flip ? -hv : hv;
-                    if (0.0 < p.x) eb += he;
+                    if (wet) eb += he;
                 }
                 if (edge == j) {
-                    zb += acc[3][ly][ci + 2];
-                    if (0.0 < p.x) zb -= he;
+                    zb += sh.ay[3][ly][ci];
+                    if (wet) zb -= he;
                 }
             }
         }
@@ -458,7 +475,7 @@ __device__ __forceinline__ Geo geo_of(const AdjScen& d)
 // start: tile prefix
 __global__ __launch_bounds__(kThreads) void htile_kernel(Batch b)
 {
-    __shared__ HAcc acc;
+    __shared__ HShared acc;
     const int blk = blockIdx.x;
     const AdjScen& d = b.sc[find_scen(b, blk)];
     const int lt = blk - d.tile0;
@@ -718,8 +735,9 @@ extern "C" int hx_adjoint_batch_create(hx_ctx* c, int solver, int nscen, const h
         d.rows = sa.rows; d.nsx = sa.nsx; d.nstrips = sa.nstrips;
         d.strip0 = strip[s];
         strip[s + 1] = strip[s] + d.nstrips;
-        d.ntx = (gm.nx + kTX - 1) / kTX;
-        d.ntiles = d.ntx * ((gm.ny + kTY - 1) / kTY);
+        const int tx = hyd ? kHX : kTX, ty = hyd ? kHY : kTY;
+        d.ntx = (gm.nx + tx - 1) / tx;
+        d.ntiles = d.ntx * ((gm.ny + ty - 1) / ty);
         d.tile0 = tile[s];
         tile[s + 1] = tile[s] + d.ntiles;
         cellb[s + 1] = cellb[s] + cell_blocks(N);

@@ -494,9 +494,11 @@ struct HFace {
     bool zfR, nodata;
 };
 
+// c: {Lm, L, R, Rp} x {h, u, v, z, 1/h} (h rebuilt; u, v and 1/h zero where
+// h <= 1e-12, as cell_prims makes them).
 __device__ __forceinline__ void hface_recon(const double* c, HFace& f)
 {
-    const double *Lm = c, *L = c + 4, *R = c + 8, *Rp = c + 12;
+    const double *Lm = c, *L = c + 5, *R = c + 10, *Rp = c + 15;
     const double zL = L[3], zR = R[3];
     f.nodata = fabs0(zL + 9999.0) < kT10 || fabs0(zR + 9999.0) < kT10;
     f.zfR = zL < zR;
@@ -507,11 +509,10 @@ __device__ __forceinline__ void hface_recon(const double* c, HFace& f)
     const double etaLf = eL + 0.5 * f.se_l.s;
     const double etaRf = eR - 0.5 * f.se_r.s;
     for (int q = 0; q < 4; ++q) {
-        const double h = c[4 * q];
-        f.wv[q] = h > kT12;
-        f.ih[q] = f.wv[q] ? rcp(h) : 0.0;
-        f.u[q] = c[4 * q + 1] * f.ih[q];
-        f.v[q] = c[4 * q + 2] * f.ih[q];
+        f.ih[q] = c[5 * q + 4];
+        f.wv[q] = f.ih[q] != 0.0;
+        f.u[q] = c[5 * q + 1];
+        f.v[q] = c[5 * q + 2];
     }
     f.su_l = vl_eval(f.u[0], f.u[1], f.u[2]);
     f.su_r = vl_eval(f.u[1], f.u[2], f.u[3]);
@@ -531,10 +532,20 @@ __device__ __forceinline__ void hface_recon(const double* c, HFace& f)
     f.hRf = (0.0 < f.xRf) ? f.xRf : 0.0;
 }
 
+// A cell's primitives as the hydrostatic face uses them: 1/h, u and v where
+// h > 1e-12 (HydrostaticRoe.cpp:273-341), zero elsewhere.
+__device__ __forceinline__ void cell_prims(double h, double hu, double hv, double& ih, double& u,
+                                           double& v)
+{
+    ih = (h > kT12) ? rcp(h) : 0.0;
+    u = hu * ih;
+    v = hv * ih;
+}
+
 // ---------------------------------------------------------------------------
 // Hydrostatic-reconstruction face (HydrostaticRoe.cpp:218-396).
-// c: {Lm, L, R, Rp} x {h, hu, hv, z} (h rebuilt).  FLb, FRb: adjoints of
-// the two face fluxes.  cb (16): adjoints of c.
+// c: {Lm, L, R, Rp} x {h, u, v, z, 1/h} (hface_recon).  FLb, FRb: adjoints
+// of the two face fluxes.  cb (16): adjoints of {h, hu, hv, z} per cell.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void hydro_face_vjp(const double* c, double nx, double ny,
                                                const double* FLb, const double* FRb, double* cb)
