@@ -3,8 +3,9 @@ list (src/io/NativePlotFile.cpp:19-97), the plotfile / VisMF / FAB layout of
 amrex::WriteMultiLevelPlotfile, and a bit-for-bit data round trip over
 single- and multi-box, single- and two-level files.
 
-Parity unpinned beyond the layout: no AMReX or yt reader is available in
-this image, and the reference ships no plotfile to compare against."""
+These tests read the files back with the writer's own parser; an
+independent reader written from the AMReX format checks the same files in
+test_plotfile_format.py."""
 import os
 from types import SimpleNamespace
 
