@@ -500,11 +500,16 @@ __device__ __forceinline__ bool roe_core_adv(const RoeCell<R>& A, const RoeCell<
     if (wave_all(ok)) return true;
     RoeCore<R> g;
     roe_core<R, DIR>(A, B, ds, g);
-    o.F0 = ok ? o.F0 : g.F0;
-    o.F1 = ok ? o.F1 : g.F1;
-    o.F2 = ok ? o.F2 : g.F2;
-    o.b0 = ok ? R(0) : g.b0;
-    o.b2 = ok ? R(0) : g.b2;
+    if (wave_any(ok)) {
+        o.F0 = ok ? o.F0 : g.F0;
+        o.F1 = ok ? o.F1 : g.F1;
+        o.F2 = ok ? o.F2 : g.F2;
+        o.b0 = ok ? R(0) : g.b0;
+        o.b2 = ok ? R(0) : g.b2;
+    } else {   // every lane general (a rough bed): no merge
+        o.F0 = g.F0; o.F1 = g.F1; o.F2 = g.F2;
+        o.b0 = g.b0; o.b2 = g.b2;
+    }
     o.l0 = g.l0;
     o.l2 = g.l2;
     o.vt = g.vt;
@@ -530,6 +535,31 @@ __device__ __forceinline__ void src_split(const RoeCore<R>& f, R& Sm0, R& Sm1, R
     Sp1 = (n0 ? k0b : R(0)) + (n2 ? k2b : R(0));
     Sp2 = (n0 ? k0c : R(0)) + (n2 ? k2c : R(0));
 }
+
+#if HX_FAST
+// Effective fluxes of a face that carries a source share (fast build), in
+// the face-normal frame: D+ = F + S+ for the cell on the low side and
+// D- = F - S- for the cell on the high side.  Where every lane of the
+// wavefront has l0 < 0 < l2 (subcritical faces), S- is the third wave's
+// share and S+ the first's, so the selects of src_split drop out; the
+// products are rounded on their own, as in src_split, and the two forms
+// agree up to the sign of a zero whatever the grouping into wavefronts.
+template <typename R>
+__device__ __forceinline__ void face_dpm(const RoeCore<R>& f, R& Dp0, R& Dp1, R& Dp2, R& Dm0,
+                                         R& Dm1, R& Dm2)
+{
+    if (wave_all((f.l0 < R(0)) && (f.l2 > R(0)))) {
+        const R k0b = f.b0 * f.l0, k0c = f.b0 * f.vt, k2b = f.b2 * f.l2, k2c = f.b2 * f.vt;
+        Dp0 = f.F0 + f.b0; Dp1 = f.F1 + k0b; Dp2 = f.F2 + k0c;
+        Dm0 = f.F0 - f.b2; Dm1 = f.F1 - k2b; Dm2 = f.F2 - k2c;
+    } else {
+        R Sm0, Sm1, Sm2, Sp0, Sp1, Sp2;
+        src_split(f, Sm0, Sm1, Sm2, Sp0, Sp1, Sp2);
+        Dp0 = f.F0 + Sp0; Dp1 = f.F1 + Sp1; Dp2 = f.F2 + Sp2;
+        Dm0 = f.F0 - Sm0; Dm1 = f.F1 - Sm1; Dm2 = f.F2 - Sm2;
+    }
+}
+#endif
 
 // The face with its split sources, rotated back to the x-y frame.
 template <typename R, int DIR>
@@ -615,7 +645,9 @@ __device__ __forceinline__ void hydro_prims(R hr, R qx, R qy, R z, R& eta, R& u,
     v = ok ? ih(qy) : R(0);
 }
 
-// Homogeneous Roe flux on face states (kernels/HydrostaticRoe.H:43-159).
+#if !HX_FAST
+// Homogeneous Roe flux on face states (kernels/HydrostaticRoe.H:43-159),
+// in the reference's operation order (exact build).
 template <typename R, int DIR>
 __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& F0, R& F1, R& F2)
 {
@@ -623,20 +655,6 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
     hi = (hi >= k::toldry) ? hi : R(0);
     hj = (hj >= k::toldry) ? hj : R(0);
     const bool wl = hi > R(0), wr = hj > R(0);
-#if HX_FAST
-    // a wet side has h >= TOLDRY, so max(h, TOLDRY) is h there and a
-    // positive finite argument on a dry side (whose values are not used)
-    R sl, rl, sr, rr;
-    sqrt_rsq(fmax(hi, k::toldry), sl, rl);
-    sqrt_rsq(fmax(hj, k::toldry), sr, rr);
-    const R sql = wl ? sl : R(0);
-    const R sqr = wr ? sr : R(0);
-    const R il = rl * rl, ir = rr * rr;
-    const R ul = wl ? hui * il : R(0);
-    const R vl = wl ? hvi * il : R(0);
-    const R ur = wr ? huj * ir : R(0);
-    const R vr = wr ? hvj * ir : R(0);
-#else
     const R sql = wl ? sqrt(hi) : R(0);
     const R sqr = wr ? sqrt(hj) : R(0);
     const Div<R> il(wl ? hi : R(1)), ir(wr ? hj : R(1));
@@ -644,23 +662,13 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
     const R vl = wl ? il(hvi) : R(0);
     const R ur = wr ? ir(huj) : R(0);
     const R vr = wr ? ir(hvj) : R(0);
-#endif
     const R unl = (DIR == 0) ? ul : vl;
     const R utl = (DIR == 0) ? vl : -ul;
     const R unr = (DIR == 0) ? ur : vr;
     const R utr = (DIR == 0) ? vr : -ur;
-#if HX_FAST
-    // g max((hi + hj)/2, TOLDRY): scaling by g and halving commute with the
-    // max and with rounding
-    R ct, ict;
-    sqrt_rsq(fmax((hi + hj) * (R(0.5) * k::g), k::g * k::toldry), ct, ict);
-    const R qh = R(0.5) * ict;
-    const R rden = qh * fma(-k::tol12, qh, R(1));
-#else
     const R hmean = R(0.5) * (hi + hj);
     const R ht = (hmean > k::toldry) ? hmean : k::toldry;
     const R ct = sqrt(k::g * ht);
-#endif
     const Div<R> idsq(sql + sqr + k::tol12);
     const R ut = idsq(unl * sql + unr * sqr);
     const R vt = idsq(utl * sql + utr * sqr);
@@ -668,34 +676,13 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
     const R l1 = ut;
     R l2 = ut + ct;
     {
-#if HX_FAST
-        const R cl = SqrtG<R>::v * sql, cr = SqrtG<R>::v * sqr;
-#else
         const R cl = sqrt(k::g * hi), cr = sqrt(k::g * hj);
-#endif
         R el = unl - cl, er = unr - cr;
         if (el < R(0) && er > R(0)) l0 = Div<R>(er - el)(el * (er - l0));
         el = unl + cl;
         er = unr + cr;
         if (el < R(0) && er > R(0)) l2 = Div<R>(er - el)(er * (l2 - el));
     }
-#if HX_FAST
-    // central flux minus the upwind wave sum with qk = |lk| ak and the
-    // contact term |ut| a1 ct = |ut| (dhv - vt dh) (1/ct drops out)
-    const R mL = hi * unl, mR = hj * unr;
-    const R dh = hj - hi;
-    const R dhu = mR - mL;
-    const R dhv = hj * utr - hi * utl;
-    const R a0 = (l2 * dh - dhu) * rden;
-    const R a2 = (dhu - l0 * dh) * rden;
-    const R q0 = fabs(l0) * a0, q2 = fabs(l2) * a2;
-    const R q02 = q0 + q2;
-    const R S1 = mL * unl + mR * unr + (R(0.5) * k::g) * (hi * hi + hj * hj);
-    const R S2 = mL * utl + mR * utr;
-    const R f0 = R(0.5) * ((mL + mR) - q02);
-    const R f1 = R(0.5) * (S1 - q0 * l0 - q2 * l2);
-    const R f2 = R(0.5) * (S2 - vt * q02 - fabs(l1) * (dhv - vt * dh));
-#else
     const Div<R> iden(R(2) * ct + k::tol12);
     const Div<R> ictd(ct);
     const R dh = hj - hi;
@@ -714,11 +701,78 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
     f0 -= w0; f1 -= w0 * l0; f2 -= w0 * vt;
     f2 -= w1 * ct;
     f0 -= w2; f1 -= w2 * l2; f2 -= w2 * vt;
-#endif
     F0 = f0;
     if (DIR == 0) { F1 = f1; F2 = f2; }
     else { F1 = -f2; F2 = f1; }
 }
+#endif
+
+#if HX_FAST
+// The homogeneous face from the face depths and velocities (fast build): a
+// wet side's velocity is the MUSCL face velocity itself instead of the
+// reference's (h u) / h, which equals it to within 2 ulp, so the face needs
+// neither the face momenta nor the reciprocal depths.
+template <typename R, int DIR>
+__device__ __forceinline__ void hroe_uv(R hi, R ui, R vi, R hj, R uj, R vj, R& F0, R& F1, R& F2)
+{
+    using k = Kc<R>;
+    hi = (hi >= k::toldry) ? hi : R(0);
+    hj = (hj >= k::toldry) ? hj : R(0);
+    const bool wl = hi > R(0), wr = hj > R(0);
+    // a wet side has h >= TOLDRY, so max(h, TOLDRY) is h there and a
+    // positive finite argument on a dry side (whose values are not used)
+    R sl, rl, sr, rr;
+    sqrt_rsq(fmax(hi, k::toldry), sl, rl);
+    sqrt_rsq(fmax(hj, k::toldry), sr, rr);
+    const R sql = wl ? sl : R(0);
+    const R sqr = wr ? sr : R(0);
+    const R ul = wl ? ui : R(0);
+    const R vl = wl ? vi : R(0);
+    const R ur = wr ? uj : R(0);
+    const R vr = wr ? vj : R(0);
+    const R unl = (DIR == 0) ? ul : vl;
+    const R utl = (DIR == 0) ? vl : -ul;
+    const R unr = (DIR == 0) ? ur : vr;
+    const R utr = (DIR == 0) ? vr : -ur;
+    // g max((hi + hj)/2, TOLDRY): scaling by g and halving commute with the
+    // max and with rounding
+    R ct, ict;
+    sqrt_rsq(fmax((hi + hj) * (R(0.5) * k::g), k::g * k::toldry), ct, ict);
+    const R qh = R(0.5) * ict;
+    const R rden = qh * fma(-k::tol12, qh, R(1));
+    const Div<R> idsq(sql + sqr + k::tol12);
+    const R ut = idsq(unl * sql + unr * sqr);
+    const R vt = idsq(utl * sql + utr * sqr);
+    R l0 = ut - ct;
+    const R l1 = ut;
+    R l2 = ut + ct;
+    {
+        const R cl = SqrtG<R>::v * sql, cr = SqrtG<R>::v * sqr;
+        R el = unl - cl, er = unr - cr;
+        if (el < R(0) && er > R(0)) l0 = Div<R>(er - el)(el * (er - l0));
+        el = unl + cl;
+        er = unr + cr;
+        if (el < R(0) && er > R(0)) l2 = Div<R>(er - el)(er * (l2 - el));
+    }
+    // central flux minus the upwind wave sum with qk = |lk| ak and the
+    // contact term |ut| a1 ct = |ut| (dhv - vt dh) (1/ct drops out)
+    const R mL = hi * unl, mR = hj * unr;
+    const R dh = hj - hi;
+    const R dhu = mR - mL;
+    const R dhv = hj * utr - hi * utl;
+    const R a0 = (l2 * dh - dhu) * rden;
+    const R a2 = (dhu - l0 * dh) * rden;
+    const R q0 = fabs(l0) * a0, q2 = fabs(l2) * a2;
+    const R q02 = q0 + q2;
+    const R S1 = mL * unl + mR * unr + (R(0.5) * k::g) * (hi * hi + hj * hj);
+    const R S2 = mL * utl + mR * utr;
+    F0 = R(0.5) * ((mL + mR) - q02);
+    const R f1 = R(0.5) * (S1 - q0 * l0 - q2 * l2);
+    const R f2 = R(0.5) * (S2 - vt * q02 - fabs(l1) * (dhv - vt * dh));
+    if (DIR == 0) { F1 = f1; F2 = f2; }
+    else { F1 = -f2; F2 = f1; }
+}
+#endif
 
 // One face: NODATA wall, face bed, MUSCL face states, Audusse depths,
 // homogeneous Roe, hydrostatic pressure correction (HydrostaticRoe.cpp:234-396).
@@ -747,10 +801,10 @@ __device__ __forceinline__ void hydro_face_st(const HSide<R>& L, const HSide<R>&
                                               R& F0, R& F1, R& F2, R& pL, R& pR)
 {
     using k = Kc<R>;
-    if (fabs(L.z + R(9999)) < k::tol10 || fabs(Rc.z + R(9999)) < k::tol10) {
-        F0 = R(0); F1 = R(0); F2 = R(0); pL = R(0); pR = R(0);
-        return;
-    }
+    // a NODATA side makes the face a wall with no flux and no pressure: the
+    // face is evaluated on every lane and those lanes take zeros at the end
+    // (a wavefront without NODATA cells only votes)
+    const bool wall = fabs(L.z + R(9999)) < k::tol10 || fabs(Rc.z + R(9999)) < k::tol10;
 #if HX_FAST
     const R zf = fmax(L.z, Rc.z);   // terrain is never NaN
 #else
@@ -764,20 +818,23 @@ __device__ __forceinline__ void hydro_face_st(const HSide<R>& L, const HSide<R>&
     const R hLf = smax(R(0), etaLf - zf);
     const R hRf = smax(R(0), etaRf - zf);
 #if HX_FAST
-    // a dry face depth is an exact 0 and the face velocities are finite, so
-    // the products are zeros there (of either sign, which the homogeneous
-    // face's dry test discards)
-    const R huLf = hLf * uLf, hvLf = hLf * vLf;
-    const R huRf = hRf * uRf, hvRf = hRf * vRf;
+    hroe_uv<R, DIR>(hLf, uLf, vLf, hRf, uRf, vRf, F0, F1, F2);
 #else
     const R huLf = (hLf > R(0)) ? hLf * uLf : R(0);
     const R hvLf = (hLf > R(0)) ? hLf * vLf : R(0);
     const R huRf = (hRf > R(0)) ? hRf * uRf : R(0);
     const R hvRf = (hRf > R(0)) ? hRf * vRf : R(0);
-#endif
     hroe<R, DIR>(hLf, huLf, hvLf, hRf, huRf, hvRf, F0, F1, F2);
+#endif
     pL = R(0.5) * k::g * (hLu * hLu - hLf * hLf);
     pR = R(0.5) * k::g * (hRu * hRu - hRf * hRf);
+    if (__builtin_expect(wave_any(wall), 0)) {
+        F0 = wall ? R(0) : F0;
+        F1 = wall ? R(0) : F1;
+        F2 = wall ? R(0) : F2;
+        pL = wall ? R(0) : pL;
+        pR = wall ? R(0) : pR;
+    }
 }
 
 template <typename R, int DIR>
