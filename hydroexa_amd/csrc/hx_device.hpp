@@ -711,29 +711,13 @@ __device__ __forceinline__ void hroe(R hi, R hui, R hvi, R hj, R huj, R hvj, R& 
 // The homogeneous face from the face depths and velocities (fast build): a
 // wet side's velocity is the MUSCL face velocity itself instead of the
 // reference's (h u) / h, which equals it to within 2 ulp, so the face needs
-// neither the face momenta nor the reciprocal depths.
-template <typename R, int DIR>
-__device__ __forceinline__ void hroe_uv(R hi, R ui, R vi, R hj, R uj, R vj, R& F0, R& F1, R& F2)
+// neither the face momenta nor the reciprocal depths.  sql, sqr: sqrt of the
+// dry-cut depths; unl .. utr: the dry-cut velocities in the face frame.
+template <typename R>
+__device__ __forceinline__ void hroe_core(R hi, R hj, R sql, R sqr, R unl, R utl, R unr, R utr,
+                                          R& f0, R& f1, R& f2)
 {
     using k = Kc<R>;
-    hi = (hi >= k::toldry) ? hi : R(0);
-    hj = (hj >= k::toldry) ? hj : R(0);
-    const bool wl = hi > R(0), wr = hj > R(0);
-    // a wet side has h >= TOLDRY, so max(h, TOLDRY) is h there and a
-    // positive finite argument on a dry side (whose values are not used)
-    R sl, rl, sr, rr;
-    sqrt_rsq(fmax(hi, k::toldry), sl, rl);
-    sqrt_rsq(fmax(hj, k::toldry), sr, rr);
-    const R sql = wl ? sl : R(0);
-    const R sqr = wr ? sr : R(0);
-    const R ul = wl ? ui : R(0);
-    const R vl = wl ? vi : R(0);
-    const R ur = wr ? uj : R(0);
-    const R vr = wr ? vj : R(0);
-    const R unl = (DIR == 0) ? ul : vl;
-    const R utl = (DIR == 0) ? vl : -ul;
-    const R unr = (DIR == 0) ? ur : vr;
-    const R utr = (DIR == 0) ? vr : -ur;
     // g max((hi + hj)/2, TOLDRY): scaling by g and halving commute with the
     // max and with rounding
     R ct, ict;
@@ -766,9 +750,42 @@ __device__ __forceinline__ void hroe_uv(R hi, R ui, R vi, R hj, R uj, R vj, R& F
     const R q02 = q0 + q2;
     const R S1 = mL * unl + mR * unr + (R(0.5) * k::g) * (hi * hi + hj * hj);
     const R S2 = mL * utl + mR * utr;
-    F0 = R(0.5) * ((mL + mR) - q02);
-    const R f1 = R(0.5) * (S1 - q0 * l0 - q2 * l2);
-    const R f2 = R(0.5) * (S2 - vt * q02 - fabs(l1) * (dhv - vt * dh));
+    f0 = R(0.5) * ((mL + mR) - q02);
+    f1 = R(0.5) * (S1 - q0 * l0 - q2 * l2);
+    f2 = R(0.5) * (S2 - vt * q02 - fabs(l1) * (dhv - vt * dh));
+}
+
+template <typename R, int DIR>
+__device__ __forceinline__ void hroe_uv(R hi, R ui, R vi, R hj, R uj, R vj, R& F0, R& F1, R& F2)
+{
+    using k = Kc<R>;
+    const bool wl = hi >= k::toldry, wr = hj >= k::toldry;
+    R f0, f1, f2;
+    if (wave_all(wl && wr)) {
+        // every face of the wavefront wet on both sides: no dry cuts
+        R sl, rl, sr, rr;
+        sqrt_rsq(hi, sl, rl);
+        sqrt_rsq(hj, sr, rr);
+        hroe_core<R>(hi, hj, sl, sr, (DIR == 0) ? ui : vi, (DIR == 0) ? vi : -ui,
+                     (DIR == 0) ? uj : vj, (DIR == 0) ? vj : -uj, f0, f1, f2);
+    } else {
+        hi = wl ? hi : R(0);
+        hj = wr ? hj : R(0);
+        // max(h, TOLDRY) is h on a wet side and a positive finite argument
+        // on a dry side (whose values are not used)
+        R sl, rl, sr, rr;
+        sqrt_rsq(fmax(hi, k::toldry), sl, rl);
+        sqrt_rsq(fmax(hj, k::toldry), sr, rr);
+        const R sql = wl ? sl : R(0);
+        const R sqr = wr ? sr : R(0);
+        const R ul = wl ? ui : R(0);
+        const R vl = wl ? vi : R(0);
+        const R ur = wr ? uj : R(0);
+        const R vr = wr ? vj : R(0);
+        hroe_core<R>(hi, hj, sql, sqr, (DIR == 0) ? ul : vl, (DIR == 0) ? vl : -ul,
+                     (DIR == 0) ? ur : vr, (DIR == 0) ? vr : -ur, f0, f1, f2);
+    }
+    F0 = f0;
     if (DIR == 0) { F1 = f1; F2 = f2; }
     else { F1 = -f2; F2 = f1; }
 }
