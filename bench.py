@@ -502,10 +502,13 @@ def time_two_level(steps, arith="fast"):
     return out
 
 
-def time_amr_c3(steps, arith="fast"):
+def time_amr_c3(steps, arith="fast", bed_amp=0.05):
     """Config C3 on an AMR hierarchy (workloads.circular_dam_break): 512^2
     base on [0,1]^2, depth 2.0 inside r = 0.2 around the centre and 0.5
-    outside, bed 0.05 U[0,1) (SplitMix64), Manning 0.03, hydrostatic Roe,
+    outside, bed bed_amp U[0,1) (SplitMix64; 0.05 is the config, whose noisy
+    bed soon puts |grad h| above the threshold everywhere the water moves, so
+    the fine levels come to cover the domain; 0 gives a flat bed on which
+    they follow the front), Manning 0.03, hydrostatic Roe,
     transmissive sides, two refined levels at ratio 2, tags where
     |grad h| > 0.1, regrid every 4 steps of a level, blocking factor 16,
     subcycled, FP64.  Times `steps` coarse steps (after 4 warm-up steps) on
@@ -519,7 +522,7 @@ def time_amr_c3(steps, arith="fast"):
     from hydroexa_amd.amr import AmrHierarchy
 
     set_arith(arith)
-    w = W.circular_dam_break(512)
+    w = W.circular_dam_break(512, bed_amp=bed_amp)
     H = AmrHierarchy(w.solver, w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi,
                      max_level=2, regrid_int=4, blocking_factor=16, max_grid_size=128,
                      tag_thresh=[(0.1, 1e300, 1e300)])
@@ -542,8 +545,9 @@ def time_amr_c3(steps, arith="fast"):
         cells += per * k
         done += k
     set_arith("exact")
+    bed = f"bed {bed_amp:g} U[0,1)" if bed_amp else "flat bed"
     out = {"workload": "circular dam break (config C3): 512^2 base, max_level 2 at ratio 2, "
-                       "depth 2.0 / 0.5, bed 0.05 U[0,1), hydrostatic Roe + Manning 0.03, "
+                       f"depth 2.0 / 0.5, {bed}, hydrostatic Roe + Manning 0.03, "
                        "transmissive, tags |grad h| > 0.1, regrid_int 4, subcycled, FP64",
            "arith": arith, "coarse_steps": steps, "wall_ms_per_coarse_step": wall / steps * 1e3,
            "composite_cell_updates_per_step": cells / steps,
@@ -581,7 +585,9 @@ def main():
                                                                     arith=args.arith)}), flush=True)
         return
     if args.only == "amr":
-        print(json.dumps({"amr_c3": time_amr_c3(args.amr_steps, args.arith)}), flush=True)
+        print(json.dumps({"amr_c3": time_amr_c3(args.amr_steps, args.arith),
+                          "amr_c3_flat_bed": time_amr_c3(args.amr_steps, args.arith, 0.0)}),
+              flush=True)
         return
     if args.only == "two_level":
         print(json.dumps({"two_level": time_two_level(args.two_level_steps, args.arith)}),
@@ -668,6 +674,7 @@ def main():
         out["two_level"] = time_two_level(args.two_level_steps, args.arith)
     if args.amr_steps > 0:
         out["amr_c3"] = time_amr_c3(args.amr_steps, args.arith)
+        out["amr_c3_flat_bed"] = time_amr_c3(args.amr_steps, args.arith, 0.0)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_seconds)
     print(json.dumps(out), flush=True)
