@@ -42,7 +42,7 @@ EXPORTS = (
     "hx_adjoint_batch_create", "hx_adjoint_batch_forward", "hx_adjoint_batch_backward",
     "hx_conservation", "hx_face_fluxes", "hx_fill_fine_ghosts", "hx_reflux",
     "hx_average_down_state", "hx_two_level_run", "hx_fill_fine_region", "hx_reflux_side",
-    "hx_boxes_dt", "hx_amr_run", "hx_tag_blocks", "hx_stream_probe",
+    "hx_boxes_dt", "hx_amr_run", "hx_tag_blocks", "hx_cluster_blocks", "hx_stream_probe",
 )
 
 
@@ -130,6 +130,8 @@ def load() -> C.CDLL:
     L.hx_boxes_dt.argtypes = [vp, C.c_int, C.c_int, pp, C.c_double, dp, dp, vp]
     L.hx_tag_blocks.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.c_int, C.c_int, C.c_int,
                                 C.c_int, vp, vp]
+    L.hx_cluster_blocks.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_int, i32p, C.c_int, i32p]
     L.hx_amr_run.argtypes = [vp, C.c_int, C.c_int, C.POINTER(HxAmrLevel), C.POINTER(C.c_int),
                              C.c_int, i32p, i32p, C.c_int, C.c_int, C.c_int, HX_AMR_REGRID_FN, vp,
                              C.c_double, C.c_double, C.c_int, dp, dp, i32p, vp]

@@ -257,7 +257,7 @@ class DenseLevel:
         self.patches: List[Box] = []
         self.t_new = 0.0
         self.t_old = -1.0e200
-        self.faces: List[Tuple[torch.Tensor, torch.Tensor]] = []
+        self._faces: Optional[List[Tuple[torch.Tensor, torch.Tensor]]] = []
 
     @property
     def pitch(self) -> int:
@@ -320,7 +320,21 @@ class DenseLevel:
             for (i0, j0, w, h) in self.boxes:
                 m[j0 // g:(j0 + h) // g, i0 // g:(i0 + w) // g] = True
             self.patches = [(g * a, g * b, g * w, g * h) for (a, b, w, h) in _rects(m)]
-        self.faces = []
+        self._faces = None            # built on first use (the host-driven step)
+        nreg = sum(6 * (w // 2 + h // 2) for (_, _, w, h) in self.patches)
+        self.regbuf = torch.zeros(max(nreg, 1), dtype=self.dtype, device="cuda")
+        self.patch_arr = np.ascontiguousarray(np.asarray(self.patches, np.int32).reshape(-1, 4))
+        return True
+
+    @property
+    def faces(self) -> List[Tuple[torch.Tensor, torch.Tensor]]:
+        """Per patch, the device face lists of its flux registers: the coarse
+        faces around it (coarse indices, coarse side's view) and its own fine
+        faces (its indices, fine side's view).  hx_amr_run generates the same
+        faces on the device and does not need them."""
+        if self._faces is not None:
+            return self._faces
+        self._faces = []
         for (i0, j0, w, h) in self.patches:
             ci0, cj0, ncx, ncy = i0 // 2, j0 // 2, w // 2, h // 2
             cx = [(0, ci0, cj0 + m, 0) for m in range(ncy)] + \
@@ -332,11 +346,8 @@ class DenseLevel:
             fy = [(1, 2 * m + s, 0, 1) for m in range(ncx) for s in (0, 1)] + \
                  [(1, 2 * m + s, h, 0) for m in range(ncx) for s in (0, 1)]
             T = lambda a: torch.as_tensor(np.asarray(a, dtype=np.int32).reshape(-1, 4), device="cuda")
-            self.faces.append((T(cx + cy), T(fx + fy)))
-        nreg = sum(6 * (w // 2 + h // 2) for (_, _, w, h) in self.patches)
-        self.regbuf = torch.zeros(max(nreg, 1), dtype=self.dtype, device="cuda")
-        self.patch_arr = np.ascontiguousarray(np.asarray(self.patches, np.int32).reshape(-1, 4))
-        return True
+            self._faces.append((T(cx + cy), T(fx + fy)))
+        return self._faces
 
     def swap(self) -> None:
         self.U_new, self.U_old = self.U_old, self.U_new
@@ -492,13 +503,14 @@ class AmrHierarchy:
             cp.ng = 0
             for d in range(2):
                 cp.bc_lo[d], cp.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
+            st = _stream()
             for b in fl.patches:
                 bp = fl.patch(fl.U_new, b)
                 skip = fl.mask_addr(old_mask, b[0], b[1]) if old_mask is not None else None
                 L.check(lib.hx_fill_fine_region(
                     self.ctx, C.byref(cp), C.byref(cp), 0.0, 2, b[0] // 2, b[1] // 2, C.byref(bp),
                     1, skip, fl.pitch, fl.addr(fl.U_new, 0, b[0], b[1]),
-                    fl.addr(fl.U_new, 1, b[0], b[1]), fl.addr(fl.U_new, 2, b[0], b[1]), _stream()),
+                    fl.addr(fl.U_new, 1, b[0], b[1]), fl.addr(fl.U_new, 2, b[0], b[1]), st),
                     "hx_fill_fine_region")
 
     # -- one level's pieces ------------------------------------------------
@@ -525,13 +537,14 @@ class AmrHierarchy:
             alpha = (t - c.t_old) / (c.t_new - c.t_old)
         po, pn = self._crse_patch(c, co), self._crse_patch(c, cn)
         lib = L.load()
+        st = _stream()
         for b in f.patches:
             bp = f.patch(f.U_new, b)
             L.check(lib.hx_fill_fine_region(
                 self.ctx, C.byref(po), C.byref(pn), float(alpha), 2, b[0] // 2, b[1] // 2,
                 C.byref(bp), 0, f.mask_addr(f.mask, b[0], b[1]), f.pitch,
                 f.addr(f.U_new, 0, b[0], b[1]), f.addr(f.U_new, 1, b[0], b[1]),
-                f.addr(f.U_new, 2, b[0], b[1]), _stream()), "hx_fill_fine_region")
+                f.addr(f.U_new, 2, b[0], b[1]), st), "hx_fill_fine_region")
 
     def _flux(self, patch, faces, pair, sx, sy, accumulate, reg) -> None:
         L.check(L.load().hx_face_fluxes(self.ctx, self.solver, C.byref(patch), faces.shape[0],

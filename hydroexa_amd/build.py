@@ -24,6 +24,7 @@ LIB = os.path.join(LIBDIR, "libhydroexa.so")
 SOURCES = [(os.path.join(CSRC, "hx_kernels.hip"), "off"),
            (os.path.join(CSRC, "hx_fast.hip"), "on"),
            (os.path.join(CSRC, "hx_adjoint.hip"), "fast")]
+HOST_SOURCES = [os.path.join(CSRC, "hx_regrid.cpp")]
 ARCH = os.environ.get("HYDROEXA_ARCH", "gfx950")
 
 
@@ -100,9 +101,20 @@ def build(verbose: bool = False) -> str:
         log.close()
     if failed:
         raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
+    # host-only bookkeeping (grid generation) in plain C++
+    host_objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(LIBDIR, os.path.basename(src) + ".o")
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+               "-c", src, "-o", obj]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"g++ failed on {os.path.basename(src)}")
+        host_objs.append(obj)
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC"] + [o for _, o, _, _ in jobs] + \
-          ["-o", tmp]
+          host_objs + ["-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -43,6 +43,11 @@ struct hx_ctx {
     double* hist;        // 2 * hist_cap doubles (dt, vel), allocated with the context
     int hist_cap;
     hx::TwoLevelWork* tlw;   // hx_two_level_run's slots, dt pair and error flag
+    hx::CflSlot* amr_slots;  // hx_amr_run's per-level CFL slots (HX_AMR_MAX_LEVELS)
+    hx::AmrPatch* amr_tab;   // hx_amr_run's patch tables (amr_cap entries, grown on regrid)
+    hx::AmrPatch* amr_host;  // pinned staging of the tables; holds what amr_tab holds
+    int amr_cap;
+    int amr_n;               // entries amr_tab holds (-1: none)
     hx::RunMark mark;
     void* scratch;           // this device's static scratch (hx_run_level_noalloc)
     uint32_t* pflags;    // prune flags

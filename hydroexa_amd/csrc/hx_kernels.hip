@@ -35,13 +35,13 @@ constexpr int kCflRows = 16;
 constexpr int kCflUnroll = 4;
 
 template <typename R>
-__global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy, R rdx, R rdy,
-                                                     int mul, CflSlot* slot)
+__device__ __forceinline__ void cfl_block(const PatchView<R>& p, R dx, R dy, R rdx, R rdy, int mul,
+                                          CflSlot* slot, int bx, int by)
 {
     CflAcc<R> acc;
     acc.init();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const int j0 = blockIdx.y * kCflRows;
+    const int i = bx * kBlock + threadIdx.x;
+    const int j0 = by * kCflRows;
     const int j1 = min(j0 + kCflRows, p.ny);
     if (i < p.nx) {
         int j = j0;
@@ -62,6 +62,13 @@ __global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy,
         }
     }
     cfl_commit(acc, slot);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy, R rdx, R rdy,
+                                                     int mul, CflSlot* slot)
+{
+    cfl_block(p, dx, dy, rdx, rdy, mul, slot, blockIdx.x, blockIdx.y);
 }
 
 __global__ void clear_slots_kernel(CflSlot* s, int n, int* err)
@@ -397,18 +404,15 @@ __device__ __forceinline__ R mc_slope(R m, R c, R p)
 // conservative linear interpolation of eta, hu, hv from the time-blended
 // coarse parent, then h = max(0, eta - z) on the fine ghost's own bed.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, PatchView<R> cn,
-                                                          R alpha, int ci0, int cj0, int nxf,
-                                                          int nyf, int64_t fpitch, int whole,
-                                                          const R* __restrict__ zf, R* hf, R* huf,
-                                                          R* hvf, const int* err,
-                                                          const uint8_t* __restrict__ skip,
-                                                          int64_t spitch)
+__device__ __forceinline__ void fillfine_cell(const PatchView<R>& co, const PatchView<R>& cn,
+                                              R alpha, int ci0, int cj0, int nxf, int nyf,
+                                              int64_t fpitch, int whole, const R* __restrict__ zf,
+                                              R* hf, R* huf, R* hvf,
+                                              const uint8_t* __restrict__ skip, int64_t spitch,
+                                              int t)
 {
-    if (err && *err) return;
     const int W = nxf + 4, H = nyf + 4;
     const int n = whole ? W * H : W * H - nxf * nyf;
-    const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
     // whole: every cell of the box and its ghosts.  Otherwise the ghost ring:
     // rows -2, -1 and ny, ny+1 in full, then the side columns
@@ -439,6 +443,20 @@ __global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, Patch
     hvf[o] = v[2];
 }
 
+template <typename R>
+__global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, PatchView<R> cn,
+                                                          R alpha, int ci0, int cj0, int nxf,
+                                                          int nyf, int64_t fpitch, int whole,
+                                                          const R* __restrict__ zf, R* hf, R* huf,
+                                                          R* hvf, const int* err,
+                                                          const uint8_t* __restrict__ skip,
+                                                          int64_t spitch)
+{
+    if (err && *err) return;
+    fillfine_cell(co, cn, alpha, ci0, cj0, nxf, nyf, fpitch, whole, zf, hf, huf, hvf, skip, spitch,
+                  (int)(blockIdx.x * kBlock + threadIdx.x));
+}
+
 // Reflux: register q of coarse face q holds -dt dA F_coarse + sum over the
 // fine substeps and fine faces of dt_f dA_f F_fine.  The coarse cell outside
 // the box beside the face gains (-reg)/vol on the box's low sides and
@@ -448,14 +466,10 @@ __global__ __launch_bounds__(kBlock) void fillfine_kernel(PatchView<R> co, Patch
 // side by side in AMReX's order; cells outside the level (a box side on the
 // domain boundary) are left alone.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void reflux_kernel(int ci0, int cj0, int ncx, int ncy,
-                                                        int64_t pitch, R vol,
-                                                        const R* __restrict__ reg, R* h, R* hu,
-                                                        R* hv, const int* err, int orient,
-                                                        int cnx, int cny)
+__device__ __forceinline__ void reflux_face(int ci0, int cj0, int ncx, int ncy, int64_t pitch, R vol,
+                                            const R* __restrict__ reg, R* h, R* hu, R* hv,
+                                            int orient, int cnx, int cny, int q)
 {
-    if (err && *err) return;
-    const int q = blockIdx.x * kBlock + threadIdx.x;
     const int nq = 2 * ncy + 2 * ncx;
     if (q >= nq) return;
     int i, j, side;
@@ -472,17 +486,25 @@ __global__ __launch_bounds__(kBlock) void reflux_kernel(int ci0, int cj0, int nc
     hv[o] += (sgn * r[2]) / vol;
 }
 
+template <typename R>
+__global__ __launch_bounds__(kBlock) void reflux_kernel(int ci0, int cj0, int ncx, int ncy,
+                                                        int64_t pitch, R vol,
+                                                        const R* __restrict__ reg, R* h, R* hu,
+                                                        R* hv, const int* err, int orient,
+                                                        int cnx, int cny)
+{
+    if (err && *err) return;
+    reflux_face(ci0, cj0, ncx, ncy, pitch, vol, reg, h, hu, hv, orient, cnx, cny,
+                (int)(blockIdx.x * kBlock + threadIdx.x));
+}
+
 // Covered coarse cells from the fine cells: c = sum of the 2 x 2 children
 // (jref outer, iref inner) times 1/4, AMReX average_down for cell data.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void avgdown_state_kernel(int ci0, int cj0, int ncx, int ncy,
-                                                               int64_t fpitch, const R* fh,
-                                                               const R* fhu, const R* fhv,
-                                                               int64_t cpitch, R* h, R* hu, R* hv,
-                                                               const int* err)
+__device__ __forceinline__ void avgdown_cell(int ci0, int cj0, int ncx, int ncy, int64_t fpitch,
+                                             const R* fh, const R* fhu, const R* fhv,
+                                             int64_t cpitch, R* h, R* hu, R* hv, int q)
 {
-    if (err && *err) return;
-    const int q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= ncx * ncy) return;
     const int ic = q % ncx, jc = q / ncx;
     const R* src[3] = {fh, fhu, fhv};
@@ -493,6 +515,96 @@ __global__ __launch_bounds__(kBlock) void avgdown_state_kernel(int ci0, int cj0,
             for (int ir = 0; ir < 2; ++ir) c += src[k][(int64_t)(2 * jc + jr) * fpitch + 2 * ic + ir];
         dst[k][(int64_t)(cj0 + jc) * cpitch + ci0 + ic] = R(0.25) * c;
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void avgdown_state_kernel(int ci0, int cj0, int ncx, int ncy,
+                                                               int64_t fpitch, const R* fh,
+                                                               const R* fhu, const R* fhv,
+                                                               int64_t cpitch, R* h, R* hu, R* hv,
+                                                               const int* err)
+{
+    if (err && *err) return;
+    avgdown_cell(ci0, cj0, ncx, ncy, fpitch, fh, fhu, fhv, cpitch, h, hu, hv,
+                 (int)(blockIdx.x * kBlock + threadIdx.x));
+}
+
+// ---------------------------------------------------------------------------
+// AMR patch-table launches (AmrPatch): one launch per operation and level
+// over every box of the level.  A workgroup finds its box by binary search
+// over the box's first block and does what that box's own launch does, so
+// the results are the per-box launches' bit for bit.
+// ---------------------------------------------------------------------------
+
+// CFL reduction of every box of a level (its valid cells) into one slot.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void amr_cfl_kernel(AmrLevelArgs<R> L,
+                                                         const AmrPatch* __restrict__ tab,
+                                                         int npatch, int mul, CflSlot* slot)
+{
+    const int b = blockIdx.x;
+    const int k = amr_find(tab, npatch, b, [](const AmrPatch& t) { return t.cfl0; });
+    const AmrPatch P = tab[k];
+    const int lb = b - P.cfl0;
+    cfl_block(amr_view(L, L.u_in, P.i0, P.j0, P.w, P.h), L.dx, L.dy, L.rdx, L.rdy, mul, slot,
+              lb % P.cflx, lb / P.cflx);
+}
+
+// Ghost rings of every box of fine level F from coarse level C (states co
+// and cn blended by alpha); cells the fine level owns keep their values.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void amr_fill_kernel(AmrLevelArgs<R> C, const R* co,
+                                                          const R* cn, R alpha, AmrLevelArgs<R> F,
+                                                          const AmrPatch* __restrict__ tab,
+                                                          int npatch)
+{
+    const int b = blockIdx.x;
+    const int k = amr_find(tab, npatch, b, [](const AmrPatch& t) { return t.fill0; });
+    const AmrPatch P = tab[k];
+    const PatchView<R> po = amr_view(C, co, 0, 0, C.nx, C.ny);
+    const PatchView<R> pn = amr_view(C, cn, 0, 0, C.nx, C.ny);
+    const int64_t off = (int64_t)(P.j0 + 2) * F.pitch + (P.i0 + 2);
+    R* u = F.u_out;
+    fillfine_cell(po, pn, alpha, P.i0 / 2, P.j0 / 2, P.w, P.h, F.pitch, 0, F.terrain + off, u + off,
+                  u + F.plane + off, u + 2 * F.plane + off, F.mask + off, F.pitch,
+                  (b - P.fill0) * kBlock + (int)threadIdx.x);
+}
+
+// Reflux of one side (orient) of every box of fine table tab into coarse
+// level C's current state (C.u_out); regs as amr_faces_kernel's.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void amr_reflux_kernel(AmrLevelArgs<R> C,
+                                                            const AmrPatch* __restrict__ tab,
+                                                            int npatch, const R* __restrict__ regs,
+                                                            int orient, R vol)
+{
+    const int b = blockIdx.x;
+    const int k = amr_find(tab, npatch, b, [](const AmrPatch& t) { return t.face0; });
+    const AmrPatch P = tab[k];
+    const int64_t off = 2 * C.pitch + 2;
+    R* u = C.u_out;
+    reflux_face(P.i0 / 2, P.j0 / 2, P.w / 2, P.h / 2, C.pitch, vol, regs + P.reg, u + off,
+                u + C.plane + off, u + 2 * C.plane + off, orient, C.nx, C.ny,
+                (b - P.face0) * kBlock + (int)threadIdx.x);
+}
+
+// average_down of every box of fine level F (state F.u_in) into coarse
+// level C's current state (C.u_out).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void amr_avgdown_kernel(AmrLevelArgs<R> F, AmrLevelArgs<R> C,
+                                                             const AmrPatch* __restrict__ tab,
+                                                             int npatch)
+{
+    const int b = blockIdx.x;
+    const int k = amr_find(tab, npatch, b, [](const AmrPatch& t) { return t.avg0; });
+    const AmrPatch P = tab[k];
+    const int64_t fo = (int64_t)(P.j0 + 2) * F.pitch + (P.i0 + 2);
+    const int64_t co = 2 * C.pitch + 2;
+    const R* f = F.u_in;
+    R* c = C.u_out;
+    avgdown_cell(P.i0 / 2, P.j0 / 2, P.w / 2, P.h / 2, F.pitch, f + fo, f + F.plane + fo,
+                 f + 2 * F.plane + fo, C.pitch, c + co, c + C.plane + co, c + 2 * C.plane + co,
+                 (b - P.avg0) * kBlock + (int)threadIdx.x);
 }
 
 }  // namespace hx
@@ -566,15 +678,18 @@ extern "C" int hx_ctx_create(hx_ctx** out, int device)
     if (hipMalloc(&c->slots, 4 * sizeof(CflSlot)) != hipSuccess ||
         hipMalloc(&c->err, sizeof(int)) != hipSuccess ||
         hipMalloc(&c->hist, sizeof(double) * 2 * kHistCap) != hipSuccess ||
-        hipMalloc(&c->tlw, sizeof(TwoLevelWork)) != hipSuccess) {
+        hipMalloc(&c->tlw, sizeof(TwoLevelWork)) != hipSuccess ||
+        hipMalloc(&c->amr_slots, sizeof(CflSlot) * HX_AMR_MAX_LEVELS) != hipSuccess) {
         hipFree(c->slots);
         hipFree(c->err);
         hipFree(c->hist);
         hipFree(c->tlw);
+        hipFree(c->amr_slots);
         delete c;
         return HX_E_NOMEM;
     }
     c->mark.valid = 0;
+    c->amr_n = -1;
     if (device >= kMaxDevices) {
         hx_ctx_destroy(c);
         return HX_E_BADARG;
@@ -613,6 +728,9 @@ extern "C" int hx_ctx_destroy(hx_ctx* c)
     hipFree(c->err);
     hipFree(c->hist);
     hipFree(c->tlw);
+    hipFree(c->amr_slots);
+    if (c->amr_tab) hipFree(c->amr_tab);
+    if (c->amr_host) hipHostFree(c->amr_host);
     if (c->pflags) hipFree(c->pflags);
     if (c->pboxes) hipFree(c->pboxes);
     if (c->cons) hipFree(c->cons);
@@ -1798,44 +1916,114 @@ struct AmrRun {
     hx_amr_regrid_fn regrid;
     void* user;
     double dt[HX_AMR_MAX_LEVELS];
-    CflSlot* slots;     // device, one per level
+    CflSlot* slots;     // device, one per level (the context's)
     hipStream_t st;
+    // patch tables of every level, concatenated (host copy and the
+    // context's device buffer), and each level's launch sizes
+    std::vector<AmrPatch> tab;
+    AmrPatch* dtab;
+    int toff[HX_AMR_MAX_LEVELS], tn[HX_AMR_MAX_LEVELS];
+    int strips[HX_AMR_MAX_LEVELS], cfl_blocks[HX_AMR_MAX_LEVELS];
+    int fill_blocks[HX_AMR_MAX_LEVELS], face_blocks[HX_AMR_MAX_LEVELS];
+    int avg_blocks[HX_AMR_MAX_LEVELS];
+    bool upload_pending;   // a table upload from the pinned staging may be in flight
 };
 
-size_t amr_es(int dtype) { return dtype == HX_F64 ? 8 : 4; }
-
-// hx_patch of a rectangle of a level (i0, j0, w, h; w = nx, h = ny and i0 = j0
-// = 0 for the whole level) on buffer b: sides on the domain boundary keep the
-// physical rule, the others read stored ghosts.
-hx_patch amr_patch(const AmrRun& r, int l, int b, int i0, int j0, int w, int h)
+// A level's arrays for the patch-table kernels: u_in and u_out the state
+// buffers bi and bo.
+template <typename R>
+AmrLevelArgs<R> amr_args(const AmrRun& r, int l, int bi, int bo)
 {
     const hx_amr_level& L = r.lv[l];
-    const size_t es = amr_es(r.dtype);
-    const int64_t pitch = L.nx + 4;
-    const size_t plane = (size_t)(L.ny + 4) * pitch * es;
-    const size_t off = ((size_t)(j0 + 2) * pitch + (i0 + 2)) * es;
-    hx_patch p;
-    std::memset(&p, 0, sizeof(p));
-    p.nx = w; p.ny = h; p.ng = 2; p.dtype = r.dtype; p.pitch = pitch;
-    char* u = (char*)L.u[b];
-    p.h = u + off; p.hu = u + plane + off; p.hv = u + 2 * plane + off;
-    p.eta = nullptr;
-    p.z = (const char*)L.terrain + off;
-    p.n = (const char*)L.terrain + plane + off;
-    p.dx = L.dx; p.dy = L.dy;
-    p.bc_lo[0] = i0 == 0 ? r.bclo[0] : HX_BC_GHOST;
-    p.bc_lo[1] = j0 == 0 ? r.bclo[1] : HX_BC_GHOST;
-    p.bc_hi[0] = i0 + w == L.nx ? r.bchi[0] : HX_BC_GHOST;
-    p.bc_hi[1] = j0 + h == L.ny ? r.bchi[1] : HX_BC_GHOST;
-    return p;
+    AmrLevelArgs<R> a;
+    a.pitch = L.nx + 4;
+    a.plane = (int64_t)(L.ny + 4) * a.pitch;
+    a.u_in = (const R*)L.u[bi];
+    a.u_out = (R*)L.u[bo];
+    a.terrain = (const R*)L.terrain;
+    a.mask = L.mask;
+    a.nx = L.nx; a.ny = L.ny;
+    a.bclo0 = r.bclo[0]; a.bclo1 = r.bclo[1]; a.bchi0 = r.bchi[0]; a.bchi1 = r.bchi[1];
+    a.dx = (R)L.dx; a.dy = (R)L.dy;
+    a.rdx = (R)(1.0 / L.dx); a.rdy = (R)(1.0 / L.dy);
+    a.dx_pow2 = is_pow2(L.dx) && is_pow2(L.dy) ? 1 : 0;
+    return a;
 }
 
-hx_patch amr_level_patch(const AmrRun& r, int l, int b)
+int amr_reg_offset(const hx_amr_level& F, int k);
+
+// Patch tables of every level from the hosts' box lists (at the start of a
+// run and after each regrid), uploaded to the context's table buffer, which
+// grows when a hierarchy has more boxes than it holds.
+template <typename R>
+int amr_tables(AmrRun& r)
 {
-    hx_patch p = amr_patch(r, l, b, 0, 0, r.lv[l].nx, r.lv[l].ny);
-    p.ng = 0;
-    for (int d = 0; d < 2; ++d) { p.bc_lo[d] = r.bclo[d]; p.bc_hi[d] = r.bchi[d]; }
-    return p;
+    const int cols = (r.solver == HX_SOLVER_ROE) ? kRoeCols : kHydroCols;
+    r.tab.clear();
+    for (int l = 0; l < *r.nlev; ++l) {
+        const hx_amr_level& L = r.lv[l];
+        const int64_t pitch = L.nx + 4;
+        r.toff[l] = (int)r.tab.size();
+        r.tn[l] = L.npatch;
+        int strips = 0, cfl = 0, fill = 0, face = 0, avg = 0;
+        for (int k = 0; k < L.npatch; ++k) {
+            const int32_t* b = L.patches + 4 * k;
+            AmrPatch P;
+            P.i0 = b[0]; P.j0 = b[1]; P.w = b[2]; P.h = b[3];
+            P.rows = clamp_strip_rows<R>(eff_rows(r.c, r.solver, r.dtype, P.w, P.h), pitch, pitch);
+            P.nsx = (P.w + cols - 1) / cols;
+            P.strip0 = strips;
+            strips += P.nsx * ((P.h + P.rows - 1) / P.rows);
+            P.cflx = (P.w + kBlock - 1) / kBlock;
+            P.cfl0 = cfl;
+            cfl += P.cflx * ((P.h + kCflRows - 1) / kCflRows);
+            const int ring = (P.w + 4) * (P.h + 4) - P.w * P.h;
+            P.fill0 = fill;
+            fill += (ring + kBlock - 1) / kBlock;
+            P.face0 = face;
+            face += (2 * (P.w / 2 + P.h / 2) + kBlock - 1) / kBlock;
+            P.avg0 = avg;
+            avg += ((P.w / 2) * (P.h / 2) + kBlock - 1) / kBlock;
+            P.reg = amr_reg_offset(L, k);
+            r.tab.push_back(P);
+        }
+        r.strips[l] = strips; r.cfl_blocks[l] = cfl; r.fill_blocks[l] = fill;
+        r.face_blocks[l] = face; r.avg_blocks[l] = avg;
+    }
+    hx_ctx* c = r.c;
+    const int n = (int)r.tab.size();
+    const size_t bytes = sizeof(AmrPatch) * n;
+    r.dtab = c->amr_tab;
+    // the device already holds this table (a regrid that kept the grids)
+    if (n == c->amr_n && (n == 0 || std::memcmp(c->amr_host, r.tab.data(), bytes) == 0))
+        return HX_OK;
+    if (n > c->amr_cap) {
+        const int cap = std::max(n, 2 * c->amr_cap + 256);
+        // kernels queued earlier may still read the old table
+        HX_TRY(hipStreamSynchronize(r.st));
+        if (c->amr_tab) HX_TRY(hipFree(c->amr_tab));
+        if (c->amr_host) HX_TRY(hipHostFree(c->amr_host));
+        c->amr_tab = nullptr;
+        c->amr_host = nullptr;
+        c->amr_cap = 0;
+        c->amr_n = -1;
+        if (hipMalloc((void**)&c->amr_tab, sizeof(AmrPatch) * cap) != hipSuccess ||
+            hipHostMalloc((void**)&c->amr_host, sizeof(AmrPatch) * cap) != hipSuccess)
+            return HX_E_NOMEM;
+        c->amr_cap = cap;
+        r.dtab = c->amr_tab;
+    } else if (r.upload_pending) {
+        // the pinned staging copy of an earlier upload may still be in flight
+        HX_TRY(hipStreamSynchronize(r.st));
+    }
+    std::memcpy(c->amr_host, r.tab.data(), bytes);
+    c->amr_n = n;
+    if (n) {
+        // stream-ordered: launches queued before read the old table, those after the new
+        HX_TRY(hipMemcpyAsync(r.dtab, c->amr_host, bytes, hipMemcpyHostToDevice, r.st));
+        r.upload_pending = true;
+    }
+    return HX_OK;
 }
 
 bool almost_equal(double x, double y)
@@ -1853,16 +2041,13 @@ int amr_fill(AmrRun& r, int l, double t)
     if (almost_equal(t, C.t_new)) { bo = bn = C.cur; }
     else if (almost_equal(t, C.t_old)) { bo = bn = C.cur ^ 1; }
     else { bo = C.cur ^ 1; bn = C.cur; alpha = (t - C.t_old) / (C.t_new - C.t_old); }
-    const hx_patch po = amr_level_patch(r, l - 1, bo), pn = amr_level_patch(r, l - 1, bn);
-    for (int k = 0; k < F.npatch; ++k) {
-        const int32_t* b = F.patches + 4 * k;
-        const hx_patch fp = amr_patch(r, l, F.cur, b[0], b[1], b[2], b[3]);
-        const uint8_t* skip = F.mask + (size_t)(b[1] + 2) * (F.nx + 4) + (b[0] + 2);
-        const int rc = fillfine_t<R>(&po, &pn, alpha, b[0] / 2, b[1] / 2, &fp, 0, (void*)fp.h,
-                                     (void*)fp.hu, (void*)fp.hv, r.st, nullptr, skip, F.nx + 4);
-        if (rc) return rc;
-    }
-    return HX_OK;
+    if (r.fill_blocks[l] == 0) return HX_OK;
+    const AmrLevelArgs<R> ca = amr_args<R>(r, l - 1, C.cur, C.cur);
+    const AmrLevelArgs<R> fa = amr_args<R>(r, l, F.cur, F.cur);
+    hipLaunchKernelGGL(amr_fill_kernel<R>, dim3(r.fill_blocks[l]), dim3(kBlock), 0, r.st, ca,
+                       (const R*)C.u[bo], (const R*)C.u[bn], (R)alpha, fa, r.dtab + r.toff[l],
+                       r.tn[l]);
+    return hip_status(hipGetLastError());
 }
 
 int amr_reg_offset(const hx_amr_level& F, int k)
@@ -1876,72 +2061,59 @@ template <typename R>
 int amr_advance(AmrRun& r, int l, double t, double dt)
 {
     hx_amr_level& L = r.lv[l];
+    const bool fast = (r.c->arith == HX_ARITH_FAST);
     int rc;
     if (l > 0 && (rc = amr_fill<R>(r, l, t))) return rc;
     if (r.reflux && l < *r.nlev - 1) {            // CrseInit for every patch of level l + 1
         hx_amr_level& F = r.lv[l + 1];
-        const hx_patch cp = amr_level_patch(r, l, L.cur);
-        for (int k = 0; k < F.npatch; ++k) {
-            const int32_t* b = F.patches + 4 * k;
-            FaceReq q{};
-            q.pair = 1; q.accumulate = 0; q.box_fine = 0;
-            q.ci0 = b[0] / 2; q.cj0 = b[1] / 2; q.ncx = b[2] / 2; q.ncy = b[3] / 2;
-            q.nfaces = 2 * (q.ncx + q.ncy);
-            q.sx = -dt * L.dy; q.sy = -dt * L.dx;
-            R* reg = (R*)F.regs + amr_reg_offset(F, k);
-            if ((rc = launch_faces(r.c, r.solver, &cp, q, reg, r.st))) return rc;
-        }
+        const AmrLevelArgs<R> la = amr_args<R>(r, l, L.cur, L.cur);
+        rc = fast ? fast::launch_amr_faces<R>(r.solver, la, r.dtab + r.toff[l + 1], r.tn[l + 1],
+                                              r.face_blocks[l + 1], 0, -dt * L.dy, -dt * L.dx,
+                                              (R*)F.regs, r.st)
+                  : exact::launch_amr_faces<R>(r.solver, la, r.dtab + r.toff[l + 1], r.tn[l + 1],
+                                               r.face_blocks[l + 1], 0, -dt * L.dy, -dt * L.dx,
+                                               (R*)F.regs, r.st);
+        if (rc) return HX_E_HIP;
     }
     if (r.reflux && l > 0) {                      // FineAdd of this level's patches
-        for (int k = 0; k < L.npatch; ++k) {
-            const int32_t* b = L.patches + 4 * k;
-            const hx_patch fp = amr_patch(r, l, L.cur, b[0], b[1], b[2], b[3]);
-            FaceReq q{};
-            q.pair = 2; q.accumulate = 1; q.box_fine = 1;
-            q.ci0 = b[0] / 2; q.cj0 = b[1] / 2; q.ncx = b[2] / 2; q.ncy = b[3] / 2;
-            q.nfaces = 4 * (q.ncx + q.ncy);
-            q.sx = dt * L.dy; q.sy = dt * L.dx;
-            R* reg = (R*)L.regs + amr_reg_offset(L, k);
-            if ((rc = launch_faces(r.c, r.solver, &fp, q, reg, r.st))) return rc;
-        }
+        const AmrLevelArgs<R> la = amr_args<R>(r, l, L.cur, L.cur);
+        rc = fast ? fast::launch_amr_faces<R>(r.solver, la, r.dtab + r.toff[l], r.tn[l],
+                                              r.face_blocks[l], 1, dt * L.dy, dt * L.dx,
+                                              (R*)L.regs, r.st)
+                  : exact::launch_amr_faces<R>(r.solver, la, r.dtab + r.toff[l], r.tn[l],
+                                               r.face_blocks[l], 1, dt * L.dy, dt * L.dx,
+                                               (R*)L.regs, r.st);
+        if (rc) return HX_E_HIP;
     }
     L.cur ^= 1;
-    for (int k = 0; k < L.npatch; ++k) {
-        const int32_t* b = L.patches + 4 * k;
-        const hx_patch in = amr_patch(r, l, L.cur ^ 1, b[0], b[1], b[2], b[3]);
-        const hx_patch out = amr_patch(r, l, L.cur, b[0], b[1], b[2], b[3]);
-        StepArgs<R> a{};
-        a.p = view_of<R>(&in);
-        a.ho = (R*)out.h; a.huo = (R*)out.hu; a.hvo = (R*)out.hv;
-        a.opitch = out.pitch;
-        a.dt_host = dt;
-        fill_geometry(a, r.solver, eff_rows(r.c, r.solver, r.dtype, in.nx, in.ny), in.dx, in.dy);
-        if ((rc = launch(r.c, r.solver, r.dtype, a, r.st))) return rc;
-    }
-    return HX_OK;
+    const AmrLevelArgs<R> la = amr_args<R>(r, l, L.cur ^ 1, L.cur);
+    const int occ = eff_occ(r.c, r.solver, r.dtype);
+    rc = fast ? fast::launch_amr_advance<R>(r.solver, occ, la, r.dtab + r.toff[l], r.tn[l],
+                                            r.strips[l], (R)dt, r.st)
+              : exact::launch_amr_advance<R>(r.solver, occ, la, r.dtab + r.toff[l], r.tn[l],
+                                             r.strips[l], (R)dt, r.st);
+    return rc ? HX_E_HIP : HX_OK;
 }
 
 template <typename R>
 int amr_reflux_avgdown(AmrRun& r, int l)
 {
     hx_amr_level &L = r.lv[l], &F = r.lv[l + 1];
-    const hx_patch cp = amr_level_patch(r, l, L.cur);
-    int rc;
-    if (r.reflux)
-        for (int orient = 0; orient < 4; ++orient)
-            for (int k = 0; k < F.npatch; ++k) {
-                const int32_t* b = F.patches + 4 * k;
-                const R* reg = (const R*)F.regs + amr_reg_offset(F, k);
-                if ((rc = reflux_t<R>(&cp, (void*)cp.h, (void*)cp.hu, (void*)cp.hv, b[0] / 2, b[1] / 2,
-                                      b[2] / 2, b[3] / 2, reg, r.st, nullptr, orient)))
-                    return rc;
-            }
-    for (int k = 0; k < F.npatch; ++k) {
-        const int32_t* b = F.patches + 4 * k;
-        const hx_patch fp = amr_patch(r, l + 1, F.cur, b[0], b[1], b[2], b[3]);
-        if ((rc = avgdown_t<R>(&fp, b[0] / 2, b[1] / 2, &cp, (void*)cp.h, (void*)cp.hu,
-                               (void*)cp.hv, r.st)))
-            return rc;
+    const AmrLevelArgs<R> ca = amr_args<R>(r, l, L.cur, L.cur);
+    const AmrPatch* ft = r.dtab + r.toff[l + 1];
+    const int fn = r.tn[l + 1];
+    if (r.reflux && r.face_blocks[l + 1] > 0)
+        for (int orient = 0; orient < 4; ++orient) {   // AMReX's side order, all boxes per side
+            hipLaunchKernelGGL(amr_reflux_kernel<R>, dim3(r.face_blocks[l + 1]), dim3(kBlock), 0,
+                               r.st, ca, ft, fn, (const R*)F.regs, orient, (R)(L.dx * L.dy));
+            const int rc = hip_status(hipGetLastError());
+            if (rc) return rc;
+        }
+    if (r.avg_blocks[l + 1] > 0) {
+        const AmrLevelArgs<R> fa = amr_args<R>(r, l + 1, F.cur, F.cur);
+        hipLaunchKernelGGL(amr_avgdown_kernel<R>, dim3(r.avg_blocks[l + 1]), dim3(kBlock), 0, r.st,
+                           fa, ca, ft, fn);
+        return hip_status(hipGetLastError());
     }
     return HX_OK;
 }
@@ -1957,6 +2129,7 @@ int amr_time_step(AmrRun& r, int l, double time)
         if ((rc = r.regrid(r.user, l, time))) return rc;
         for (int k = l; k < *r.nlev; ++k) r.lv[k].last_regrid = r.lv[k].istep;
         for (int k = old_n; k < *r.nlev; ++k) r.dt[k] = r.dt[k - 1] / (r.subcycle ? 2.0 : 1.0);
+        if ((rc = amr_tables<R>(r))) return rc;
         L = &r.lv[l];
     }
     L->t_old = L->t_new;
@@ -1981,16 +2154,17 @@ int amr_compute_dt(AmrRun& r, double cfl, int* collapsed)
     hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, r.st, r.slots, n, (int*)nullptr);
     for (int l = 0; l < n; ++l) {
         const hx_amr_level& L = r.lv[l];
-        for (int k = 0; k < L.npatch; ++k) {
-            const int32_t* b = L.patches + 4 * k;
-            const hx_patch p = amr_patch(r, l, L.cur, b[0], b[1], b[2], b[3]);
-            const int rc = launch_cfl<R>(&p, r.slots + l, r.st);
-            if (rc) return rc;
-        }
+        if (r.cfl_blocks[l] == 0) continue;
+        const AmrLevelArgs<R> la = amr_args<R>(r, l, L.cur, L.cur);
+        hipLaunchKernelGGL(amr_cfl_kernel<R>, dim3(r.cfl_blocks[l]), dim3(kBlock), 0, r.st, la,
+                           r.dtab + r.toff[l], r.tn[l], la.dx_pow2, r.slots + l);
+        const int rc = hip_status(hipGetLastError());
+        if (rc) return rc;
     }
     CflSlot s[HX_AMR_MAX_LEVELS];
     HX_TRY(hipMemcpyAsync(s, r.slots, sizeof(CflSlot) * n, hipMemcpyDeviceToHost, r.st));
     HX_TRY(hipStreamSynchronize(r.st));
+    r.upload_pending = false;
     double tmp[HX_AMR_MAX_LEVELS];
     *collapsed = 0;
     for (int l = 0; l < n; ++l) {
@@ -2044,8 +2218,10 @@ extern "C" int hx_amr_run(hx_ctx* c, int solver, int dtype, hx_amr_level* levels
     r.regrid_int = regrid_int; r.regrid = regrid; r.user = user;
     r.bclo[0] = bc_lo[0]; r.bclo[1] = bc_lo[1]; r.bchi[0] = bc_hi[0]; r.bchi[1] = bc_hi[1];
     r.st = (hipStream_t)stream;
-    HX_TRY(hipMalloc((void**)&r.slots, sizeof(CflSlot) * HX_AMR_MAX_LEVELS));
-    int rc = HX_OK, k = 0;
+    r.slots = c->amr_slots;
+    int rc = (dtype == HX_F64) ? amr_tables<double>(r) : amr_tables<float>(r);
+    if (rc) return rc;
+    int k = 0;
     while (*time < t_end && k < max_steps) {
         int collapsed = 0;
         rc = (dtype == HX_F64) ? amr_compute_dt<double>(r, cfl, &collapsed)
@@ -2060,7 +2236,6 @@ extern "C" int hx_amr_run(hx_ctx* c, int solver, int dtype, hx_amr_level* levels
         ++k;
     }
     if (hipStreamSynchronize(r.st) != hipSuccess && !rc) rc = HX_E_HIP;
-    (void)hipFree(r.slots);
     if (steps_done) *steps_done = k;
     return rc;
 }

@@ -106,6 +106,76 @@ struct FaceReq {
     const int* err;     // non-null and set: nothing is done (a stopped run)
 };
 
+// One box of an AMR level in the level's patch table: every per-box
+// operation of a substep (the advance, the CFL reduction, the ghost fill,
+// the flux-register faces, reflux and average_down) runs as one launch over
+// the table, each box owning a run of strips or blocks from its first index
+// below.
+struct AmrPatch {
+    int i0, j0, w, h;     // box in the level's cells
+    int rows, nsx;        // advance strips of the box
+    int strip0;           // first strip of the box in the launch
+    int cfl0, cflx;       // CFL reduction: first block, blocks across x
+    int fill0;            // ghost fill: first block
+    int face0;            // flux-register rows (and reflux faces): first block
+    int avg0;             // average_down: first block
+    int reg;              // offset of the box's flux register (elements)
+};
+
+// A level's dense arrays as the batched kernels see them: each buffer holds
+// the h, hu, hv planes plane elements apart, the terrain its z and roughness
+// planes; a cell (i, j) lies at (j + 2) pitch + (i + 2) (two padding cells),
+// pitch = nx + 4.  Box sides on the domain boundary take the physical BCs.
+template <typename R>
+struct AmrLevelArgs {
+    const R* u_in;        // the state read
+    R* u_out;             // the state written (the advance)
+    const R* terrain;
+    const uint8_t* mask;  // 1 on the cells the level owns (pitch as the arrays)
+    int64_t pitch, plane;
+    int nx, ny;
+    int bclo0, bclo1, bchi0, bchi1;
+    R dx, dy, rdx, rdy;
+    int dx_pow2;
+};
+
+// View of box (i0, j0, w, h) of a level on state u.
+template <typename R>
+__host__ __device__ inline PatchView<R> amr_view(const AmrLevelArgs<R>& L, const R* u, int i0,
+                                                 int j0, int w, int h)
+{
+    const int64_t off = (int64_t)(j0 + 2) * L.pitch + (i0 + 2);
+    PatchView<R> p;
+    p.h = u + off;
+    p.hu = u + L.plane + off;
+    p.hv = u + 2 * L.plane + off;
+    p.eta = nullptr;
+    p.z = L.terrain + off;
+    p.n = L.terrain + L.plane + off;
+    p.pitch = L.pitch;
+    p.nx = w;
+    p.ny = h;
+    p.bl0 = (i0 == 0) ? L.bclo0 : -1;
+    p.bl1 = (j0 == 0) ? L.bclo1 : -1;
+    p.bh0 = (i0 + w == L.nx) ? L.bchi0 : -1;
+    p.bh1 = (j0 + h == L.ny) ? L.bchi1 : -1;
+    return p;
+}
+
+// The box of a patch table that owns index key: the last k with
+// first(t[k]) <= key (first indices ascend).  key is uniform across the
+// caller's wavefront or workgroup, so the search is scalar.
+template <typename F>
+__device__ inline int amr_find(const AmrPatch* t, int n, int key, F first)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first(t[mid]) <= key) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Scenario table of a batched adjoint rollout (FP64): one entry per
 // scenario, with the buffers its forward and its segment recomputations
 // step through.  State k of the forward lives in ckpt[k / every] when k is a
@@ -225,11 +295,23 @@ namespace exact {
 template <typename R>
 int launch_ensemble(int solver, int occ, const EnsArgs<R>& e, int grid, hipStream_t st);
 int launch_adj_batch(int solver, int occ, const AdjBatch& b, hipStream_t st);
+template <typename R>
+int launch_amr_advance(int solver, int occ, const AmrLevelArgs<R>& L, const AmrPatch* tab,
+                       int npatch, int strips, R dt, hipStream_t st);
+template <typename R>
+int launch_amr_faces(int solver, const AmrLevelArgs<R>& L, const AmrPatch* tab, int npatch,
+                     int blocks, int fine_box, double sx, double sy, R* regs, hipStream_t st);
 }
 namespace fast {
 template <typename R>
 int launch_ensemble(int solver, int occ, const EnsArgs<R>& e, int grid, hipStream_t st);
 int launch_adj_batch(int solver, int occ, const AdjBatch& b, hipStream_t st);
+template <typename R>
+int launch_amr_advance(int solver, int occ, const AmrLevelArgs<R>& L, const AmrPatch* tab,
+                       int npatch, int strips, R dt, hipStream_t st);
+template <typename R>
+int launch_amr_faces(int solver, const AmrLevelArgs<R>& L, const AmrPatch* tab, int npatch,
+                     int blocks, int fine_box, double sx, double sy, R* regs, hipStream_t st);
 }
 
 }  // namespace hx

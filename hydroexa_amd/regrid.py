@@ -156,10 +156,19 @@ def cluster(tags: np.ndarray, grid_eff: float) -> List[Box]:
             for c in work]
 
 
-def _row_runs(row: np.ndarray) -> List[Tuple[int, int]]:
-    d = np.diff(np.concatenate(([0], row.astype(np.int8), [0])))
-    starts, ends = np.flatnonzero(d == 1), np.flatnonzero(d == -1)
-    return list(zip(starts.tolist(), ends.tolist()))
+def _row_runs_all(mask: np.ndarray) -> dict:
+    """Row j -> its runs of true cells [(start, end)), ascending, for every
+    row of mask at once."""
+    ny, nx = mask.shape
+    m = np.zeros((ny, nx + 2), np.int8)
+    m[:, 1:-1] = mask
+    d = np.diff(m, axis=1)
+    sj, si = np.nonzero(d == 1)
+    _, ei = np.nonzero(d == -1)
+    rows = {}
+    for j, a, b in zip(sj.tolist(), si.tolist(), ei.tolist()):
+        rows.setdefault(j, []).append((a, b))
+    return rows
 
 
 def _rects(mask: np.ndarray) -> List[Box]:
@@ -168,8 +177,9 @@ def _rects(mask: np.ndarray) -> List[Box]:
     ny, nx = mask.shape
     out = []
     open_runs = {}
+    all_runs = _row_runs_all(mask) if ny and nx else {}
     for j in range(ny + 1):
-        runs = set(_row_runs(mask[j])) if j < ny else set()
+        runs = set(all_runs.get(j, ())) if j < ny else set()
         nxt = {}
         for r, j0 in open_runs.items():
             if r in runs:
@@ -261,14 +271,45 @@ def allowed_blocks(boxes: Sequence[Box], ny: int, nx: int, b: int, nest: int) ->
     return erode(vb, 1)
 
 
-def grids_from_blocks(tb: np.ndarray, allowed: np.ndarray, b: int, ratio: int, grid_eff: float,
-                      max_grid_size: int) -> List[Box]:
-    """Steps 3-5 of the module docstring from tags already grown and blocked."""
+def _order(boxes: List[Box]) -> List[Box]:
+    return sorted(boxes, key=lambda x: (x[1], x[0], x[2], x[3]))
+
+
+def grids_from_blocks_reference(tb: np.ndarray, allowed: np.ndarray, b: int, ratio: int,
+                                grid_eff: float, max_grid_size: int) -> List[Box]:
+    """Steps 3-5 of the module docstring from tags already grown and blocked,
+    in Python (the restatement the native hx_cluster_blocks is checked
+    against); boxes sorted by (j0, i0, w, h)."""
     tb = tb & allowed
     boxes = intersect_nesting(cluster(tb, grid_eff), allowed, tb)
     fine = [((i0 * b) * ratio, (j0 * b) * ratio, (w * b) * ratio, (h * b) * ratio)
             for (i0, j0, w, h) in boxes]
-    return max_size(fine, max_grid_size)
+    return _order(max_size(fine, max_grid_size))
+
+
+def grids_from_blocks(tb: np.ndarray, allowed: np.ndarray, b: int, ratio: int, grid_eff: float,
+                      max_grid_size: int) -> List[Box]:
+    """Steps 3-5 of the module docstring from tags already grown and blocked
+    (nby x nbx booleans), by the library's hx_cluster_blocks; boxes sorted by
+    (j0, i0, w, h), as grids_from_blocks_reference gives them."""
+    import ctypes as C
+    from . import _lib as L
+    t = np.ascontiguousarray(tb, np.uint8)
+    a = np.ascontiguousarray(allowed, np.uint8)
+    if t.shape != a.shape or t.ndim != 2:
+        raise ValueError("tags and allowed must be equal 2-D block arrays")
+    n = C.c_int32()
+    cap = 256
+    while True:
+        out = np.zeros(4 * cap, np.int32)
+        rc = L.load().hx_cluster_blocks(t.ctypes.data, a.ctypes.data, t.shape[0], t.shape[1], int(b),
+                                        int(ratio), float(grid_eff), int(max_grid_size),
+                                        out.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(n))
+        if rc == L.HX_E_BADARG and n.value > cap:
+            cap = n.value
+            continue
+        L.check(rc, "hx_cluster_blocks")
+        return [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(n.value)]
 
 
 def covers(boxes: Sequence[Box], ny: int, nx: int) -> np.ndarray:

@@ -375,6 +375,23 @@ int hx_tag_blocks(hx_ctx *ctx, const uint8_t *tags, int64_t tag_pitch, const uin
                   void *stream);
 
 /*
+ * New grids of level k + 1 from level k's tag blocks (host in and out; no
+ * device work): Berger-Rigoutsos clustering of the blocks that are tagged
+ * and allowed (the proper-nesting domain) at efficiency grid_eff, clusters
+ * reaching outside the allowed blocks replaced by the allowed rectangles
+ * inside them that hold tags, refined by b * ratio and chopped to
+ * max_grid_size.  tags, allowed: nby x nbx bytes, row-major.  out receives
+ * *nout boxes {i0, j0, w, h} in level k + 1's cells, sorted by (j0, i0, w,
+ * h); HX_E_BADARG with *nout set when cap is too small.  No counterpart in
+ * the reference, which calls AMReX's AmrMesh::MakeNewGrids
+ * (src/state/AmrMeshState_Physics.cpp:50-105); this is that grid
+ * generation's clustering and chopping, done natively.
+ */
+int hx_cluster_blocks(const uint8_t *tags, const uint8_t *allowed, int nby, int nbx, int b,
+                      int ratio, double grid_eff, int max_grid_size, int32_t *out, int cap,
+                      int32_t *nout);
+
+/*
  * hx_level_dt over the valid cells of nbox boxes of one level (host array
  * of patches), one reduction and one synchronisation.
  */

@@ -88,3 +88,25 @@ def test_new_grids_invariants(seed):
 def test_new_grids_empty_without_tags():
     v = np.ones((32, 32), bool)
     assert R.new_grids(np.zeros((32, 32), bool), v, 2, 8, 64, 1, 0.7, 2) == []
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_native_clustering_equals_the_python_restatement(seed):
+    """hx_cluster_blocks (the library's grid generation) gives the boxes of
+    the Python restatement, in the same order, on random tag and nesting
+    patterns: clustering at three efficiencies, nesting cut-outs, two block
+    sizes and three max_grid_size chops."""
+    rng = np.random.default_rng(100 + seed)
+    for t in range(150):
+        ny, nx = rng.integers(1, 70, 2)
+        tb = np.zeros((ny, nx), bool)
+        for _ in range(rng.integers(0, 6)):
+            j, i = rng.integers(0, ny), rng.integers(0, nx)
+            tb[j:j + rng.integers(1, 12), i:i + rng.integers(1, 12)] = True
+        tb |= rng.random((ny, nx)) < rng.random() * 0.05
+        allowed = R.erode(rng.random((ny, nx)) < 0.9, 1) if t % 2 else np.ones((ny, nx), bool)
+        b = int(rng.choice([4, 8]))
+        eff = float(rng.choice([0.5, 0.7, 0.9]))
+        mgs = int(rng.choice([32, 64, 128]))
+        assert R.grids_from_blocks(tb, allowed, b, 2, eff, mgs) == \
+            R.grids_from_blocks_reference(tb, allowed, b, 2, eff, mgs), (seed, t)
