@@ -304,9 +304,19 @@ class DenseLevel:
         if new == self.boxes:
             return False
         self.boxes = new
-        self.mask.zero_()
-        for (i0, j0, w, h) in self.boxes:
-            self.mask[NG + j0:NG + j0 + h, NG + i0:NG + i0 + w] = 1
+        # the mask from the box list in four device operations whatever the
+        # number of boxes: +1 / -1 at each box's corners of a difference
+        # array, two prefix sums, then > 0 (boxes are disjoint)
+        H, W = self.mask.shape
+        b = torch.as_tensor(np.asarray(self.boxes, np.int64).reshape(-1, 4), device=self.mask.device)
+        j0, i0 = b[:, 1] + NG, b[:, 0] + NG
+        j1, i1 = j0 + b[:, 3], i0 + b[:, 2]
+        one = torch.ones(len(self.boxes), dtype=torch.int32, device=self.mask.device)
+        d = torch.zeros((H + 1) * (W + 1), dtype=torch.int32, device=self.mask.device)
+        for jj, ii, sg in ((j0, i0, 1), (j0, i1, -1), (j1, i0, -1), (j1, i1, 1)):
+            d.index_add_(0, jj * (W + 1) + ii, one * sg)
+        cov = d.view(H + 1, W + 1).cumsum(0).cumsum(1)[:H, :W]
+        self.mask.copy_(cov > 0)
         if len(self.boxes) == 1:
             self.patches = list(self.boxes)
         else:
@@ -454,7 +464,7 @@ class AmrHierarchy:
                     continue
             else:
                 if k > 0:
-                    self._fill_ghosts(k, time)
+                    self._fill_ghosts(k, time, whole_level=True)
                 th = self.tag_thresh[min(k, len(self.tag_thresh) - 1)]
                 tags = torch.empty((lv.ny, lv.nx), dtype=torch.uint8, device="cuda")
                 thr = (C.c_double * 3)(*th)
@@ -503,15 +513,16 @@ class AmrHierarchy:
             cp.ng = 0
             for d in range(2):
                 cp.bc_lo[d], cp.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
-            st = _stream()
-            for b in fl.patches:
-                bp = fl.patch(fl.U_new, b)
-                skip = fl.mask_addr(old_mask, b[0], b[1]) if old_mask is not None else None
-                L.check(lib.hx_fill_fine_region(
-                    self.ctx, C.byref(cp), C.byref(cp), 0.0, 2, b[0] // 2, b[1] // 2, C.byref(bp),
-                    1, skip, fl.pitch, fl.addr(fl.U_new, 0, b[0], b[1]),
-                    fl.addr(fl.U_new, 1, b[0], b[1]), fl.addr(fl.U_new, 2, b[0], b[1]), st),
-                    "hx_fill_fine_region")
+            # one fill over the whole level: every cell the level did not hold
+            # takes the interpolation from the coarser level (each box's own
+            # fill gives its cells the same values; cells outside the new
+            # boxes are refilled as ghosts before any use)
+            bp = fl.patch(fl.U_new)
+            skip = fl.mask_addr(old_mask, 0, 0) if old_mask is not None else None
+            L.check(lib.hx_fill_fine_region(
+                self.ctx, C.byref(cp), C.byref(cp), 0.0, 2, 0, 0, C.byref(bp), 1, skip, fl.pitch,
+                fl.addr(fl.U_new, 0, 0, 0), fl.addr(fl.U_new, 1, 0, 0), fl.addr(fl.U_new, 2, 0, 0),
+                _stream()), "hx_fill_fine_region")
 
     # -- one level's pieces ------------------------------------------------
     def _crse_patch(self, lv: DenseLevel, U: torch.Tensor) -> L.HxPatch:
@@ -521,10 +532,11 @@ class AmrHierarchy:
             p.bc_lo[d], p.bc_hi[d] = self.bc_lo[d], self.bc_hi[d]
         return p
 
-    def _fill_ghosts(self, lev: int, t: float) -> None:
+    def _fill_ghosts(self, lev: int, t: float, whole_level: bool = False) -> None:
         """Ring cells of every box the level does not own, from the coarser
         level at time t (GetData's time selection: the new state, the old
-        state, or their blend)."""
+        state, or their blend).  whole_level: every cell the level does not
+        own, in one launch (the rings take the same values)."""
         c, f = self.levels[lev - 1], self.levels[lev]
         if _almost_equal(t, c.t_new):
             co = cn = c.U_new
@@ -538,6 +550,13 @@ class AmrHierarchy:
         po, pn = self._crse_patch(c, co), self._crse_patch(c, cn)
         lib = L.load()
         st = _stream()
+        if whole_level:
+            bp = f.patch(f.U_new)
+            L.check(lib.hx_fill_fine_region(
+                self.ctx, C.byref(po), C.byref(pn), float(alpha), 2, 0, 0, C.byref(bp), 1,
+                f.mask_addr(f.mask, 0, 0), f.pitch, f.addr(f.U_new, 0, 0, 0),
+                f.addr(f.U_new, 1, 0, 0), f.addr(f.U_new, 2, 0, 0), st), "hx_fill_fine_region")
+            return
         for b in f.patches:
             bp = f.patch(f.U_new, b)
             L.check(lib.hx_fill_fine_region(
