@@ -304,32 +304,26 @@ class DenseLevel:
         if new == self.boxes:
             return False
         self.boxes = new
-        # the mask from the box list in four device operations whatever the
-        # number of boxes: +1 / -1 at each box's corners of a difference
-        # array, two prefix sums, then > 0 (boxes are disjoint)
-        H, W = self.mask.shape
-        b = torch.as_tensor(np.asarray(self.boxes, np.int64).reshape(-1, 4), device=self.mask.device)
-        j0, i0 = b[:, 1] + NG, b[:, 0] + NG
-        j1, i1 = j0 + b[:, 3], i0 + b[:, 2]
-        one = torch.ones(len(self.boxes), dtype=torch.int32, device=self.mask.device)
-        d = torch.zeros((H + 1) * (W + 1), dtype=torch.int32, device=self.mask.device)
-        for jj, ii, sg in ((j0, i0, 1), (j0, i1, -1), (j1, i0, -1), (j1, i1, 1)):
-            d.index_add_(0, jj * (W + 1) + ii, one * sg)
-        cov = d.view(H + 1, W + 1).cumsum(0).cumsum(1)[:H, :W]
-        self.mask.copy_(cov > 0)
+        # the union painted on the host at the boxes' common alignment g (a
+        # small array): expanded by g on the device into the mask in one copy,
+        # and split into the patches
+        g = 0
+        for bx in self.boxes:
+            for v in bx:
+                g = gcd(g, v)
+        g = max(g, 1)
+        nyb, nxb = -(-self.ny // g), -(-self.nx // g)
+        mb = np.zeros((nyb, nxb), np.uint8)
+        for (i0, j0, w, h) in self.boxes:
+            mb[j0 // g:(j0 + h) // g, i0 // g:(i0 + w) // g] = 1
+        t = torch.from_numpy(mb).to(self.mask.device)
+        full = t[:, None, :, None].expand(nyb, g, nxb, g).reshape(nyb * g, nxb * g)
+        self.mask.zero_()
+        self.mask[NG:NG + self.ny, NG:NG + self.nx].copy_(full[:self.ny, :self.nx])
         if len(self.boxes) == 1:
             self.patches = list(self.boxes)
         else:
-            # rasterise the union at the boxes' common alignment (at least 2)
-            g = 0
-            for b in self.boxes:
-                for v in b:
-                    g = gcd(g, v)
-            g = max(g, 1)
-            m = np.zeros((-(-self.ny // g), -(-self.nx // g)), bool)
-            for (i0, j0, w, h) in self.boxes:
-                m[j0 // g:(j0 + h) // g, i0 // g:(i0 + w) // g] = True
-            self.patches = [(g * a, g * b, g * w, g * h) for (a, b, w, h) in _rects(m)]
+            self.patches = [(g * a, g * b, g * w, g * h) for (a, b, w, h) in _rects(mb.astype(bool))]
         self._faces = None            # built on first use (the host-driven step)
         nreg = sum(6 * (w // 2 + h // 2) for (_, _, w, h) in self.patches)
         self.regbuf = torch.zeros(max(nreg, 1), dtype=self.dtype, device="cuda")
