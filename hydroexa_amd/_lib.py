@@ -41,7 +41,7 @@ EXPORTS = (
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
     "hx_adjoint_batch_create", "hx_adjoint_batch_forward", "hx_adjoint_batch_backward",
     "hx_conservation", "hx_face_fluxes", "hx_fill_fine_ghosts", "hx_reflux",
-    "hx_average_down_state", "hx_two_level_run", "hx_fill_fine_region", "hx_reflux_side",
+    "hx_average_down_state", "hx_two_level_run", "hx_two_level_enqueue", "hx_fill_fine_region", "hx_reflux_side",
     "hx_boxes_dt", "hx_amr_run", "hx_tag_blocks", "hx_cluster_blocks", "hx_stream_probe",
 )
 
@@ -138,6 +138,8 @@ def load() -> C.CDLL:
     L.hx_two_level_run.argtypes = [vp, C.c_int, pp, vp, vp, vp, pp, vp, vp, vp, C.c_int, C.c_int,
                                    C.c_int, C.c_double, C.c_int, vp, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int32), vp]
+    L.hx_two_level_enqueue.argtypes = [vp, C.c_int, pp, vp, vp, vp, pp, vp, vp, vp, C.c_int,
+                                       C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, vp]
     L.hx_fill_fine_ghosts.argtypes = [vp, pp, pp, C.c_double, C.c_int, C.c_int, C.c_int, pp,
                                       C.c_int, vp, vp, vp, vp]
     L.hx_reflux.argtypes = [vp, pp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]

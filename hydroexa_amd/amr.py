@@ -228,6 +228,35 @@ class TwoLevel:
         L.check(rc, "hx_two_level_run")
         return hist[:nsteps]
 
+    def enqueue(self, nsteps: int, cfl: float, dt_hist: torch.Tensor,
+                status: torch.Tensor) -> None:
+        """nsteps coupled steps as stream work only (hx_two_level_enqueue):
+        no host copy or synchronisation, so the call may be captured in a
+        HIP graph (torch.cuda.graph) and replayed.  nsteps must be even, so
+        that every replay starts from and ends in the same buffers.
+        dt_hist: float64 device tensor of >= 2 nsteps (coarse dt, fine dt);
+        status: int32 device tensor, set to 0 or to k + 1 when step k's
+        estimate collapsed.  time and steps are left to the caller, who
+        reads them from dt_hist."""
+        if nsteps < 2 or nsteps % 2:
+            raise ValueError("enqueue: nsteps must be even and positive")
+        if (dt_hist.dtype != torch.float64 or not dt_hist.is_cuda or
+                dt_hist.numel() < 2 * nsteps or not dt_hist.is_contiguous()):
+            raise ValueError("enqueue: dt_hist must be a contiguous float64 device tensor "
+                             "of at least 2 * nsteps values")
+        if status.dtype != torch.int32 or not status.is_cuda or status.numel() < 1:
+            raise ValueError("enqueue: status must be an int32 device tensor")
+        c, f = self.crse, self.fine
+        ca, fa = c.U_old, f.U_old
+        plane = c.nx * c.ny * ca.element_size()
+        L.check(L.load().hx_two_level_enqueue(
+            context(), self.solver, C.byref(c.patch(c.U_new)), ca.data_ptr(),
+            ca.data_ptr() + plane, ca.data_ptr() + 2 * plane, C.byref(f.patch(f.U_new)),
+            f._ptr(fa, 0), f._ptr(fa, 1), f._ptr(fa, 2), self.box[0], self.box[1],
+            int(self.subcycle), float(cfl), int(nsteps), C.c_void_p(self.reg.data_ptr()),
+            C.c_void_p(dt_hist.data_ptr()), C.c_void_p(status.data_ptr()), _stream()),
+            "hx_two_level_enqueue")
+
     def composite_mass(self) -> float:
         """Sum of h dA over the coarse level (covered cells hold the average
         of their fine children after each step)."""

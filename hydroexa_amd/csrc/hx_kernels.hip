@@ -1762,10 +1762,10 @@ static int two_level_t(hx_ctx* c, int solver, const hx_patch* crse, void* const*
     return HX_OK;
 }
 
-extern "C" int hx_two_level_run(hx_ctx* c, int solver, const hx_patch* crse, void* ch, void* chu,
-                                void* chv, const hx_patch* fine, void* fh, void* fhu, void* fhv,
-                                int ci0, int cj0, int subcycle, double cfl, int nsteps, void* reg,
-                                double* dt_hist, int32_t* steps_done, void* stream)
+// Arguments of a coupled two-level run (hx_two_level_run, hx_two_level_enqueue).
+static int two_level_check(const hx_ctx* c, int solver, const hx_patch* crse, void* ch, void* chu,
+                           void* chv, const hx_patch* fine, void* fh, void* fhu, void* fhv,
+                           int ci0, int cj0, double cfl, int nsteps, void* reg)
 {
     if (!c) return HX_E_STATE;
     if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
@@ -1784,6 +1784,17 @@ extern "C" int hx_two_level_run(hx_ctx* c, int solver, const hx_patch* crse, voi
     const int ncx = fine->nx / 2, ncy = fine->ny / 2;
     if (ci0 < 1 || cj0 < 1 || ci0 + ncx > crse->nx - 1 || cj0 + ncy > crse->ny - 1)
         return HX_E_BADARG;
+    return HX_OK;
+}
+
+extern "C" int hx_two_level_run(hx_ctx* c, int solver, const hx_patch* crse, void* ch, void* chu,
+                                void* chv, const hx_patch* fine, void* fh, void* fhu, void* fhv,
+                                int ci0, int cj0, int subcycle, double cfl, int nsteps, void* reg,
+                                double* dt_hist, int32_t* steps_done, void* stream)
+{
+    int rc = two_level_check(c, solver, crse, ch, chu, chv, fine, fh, fhu, fhv, ci0, cj0, cfl,
+                             nsteps, reg);
+    if (rc) return rc;
     if (steps_done) *steps_done = 0;
     if (nsteps == 0) return HX_OK;
     HX_TRY(hipSetDevice(c->device));
@@ -1811,6 +1822,40 @@ extern "C" int hx_two_level_run(hx_ctx* c, int solver, const hx_patch* crse, voi
         }
     }
     if (steps_done) *steps_done = nsteps;
+    return HX_OK;
+}
+
+__global__ void tl_status_kernel(const int* err, int32_t* out)
+{
+    if (threadIdx.x == 0) *out = *err;
+}
+
+extern "C" int hx_two_level_enqueue(hx_ctx* c, int solver, const hx_patch* crse, void* ch,
+                                    void* chu, void* chv, const hx_patch* fine, void* fh,
+                                    void* fhu, void* fhv, int ci0, int cj0, int subcycle,
+                                    double cfl, int nsteps, void* reg, double* dt_hist_dev,
+                                    int32_t* status_dev, void* stream)
+{
+    int rc = two_level_check(c, solver, crse, ch, chu, chv, fine, fh, fhu, fhv, ci0, cj0, cfl,
+                             nsteps, reg);
+    if (rc) return rc;
+    if (nsteps < 1 || nsteps > c->hist_cap) return HX_E_BADARG;
+    HX_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    void* calt[3] = {ch, chu, chv};
+    void* falt[3] = {fh, fhu, fhv};
+    int fcur = 0;
+    double* hist = dt_hist_dev ? dt_hist_dev : c->hist;
+    rc = (crse->dtype == HX_F64)
+             ? two_level_t<double>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0, cfl,
+                                   0, nsteps, fcur, reg, c->tlw, hist, st)
+             : two_level_t<float>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0, cfl,
+                                  0, nsteps, fcur, reg, c->tlw, hist, st);
+    if (rc) return rc;
+    if (status_dev) {
+        hipLaunchKernelGGL(tl_status_kernel, dim3(1), dim3(64), 0, st, &c->tlw->err, status_dev);
+        HX_TRY(hipGetLastError());
+    }
     return HX_OK;
 }
 

@@ -476,6 +476,26 @@ int hx_two_level_run(hx_ctx *ctx, int solver, const hx_patch *crse, void *ch, vo
                      int32_t *steps_done, void *stream);
 
 /*
+ * hx_two_level_run's nsteps coupled steps (1 <= nsteps <= 4096) as stream
+ * work only: kernel launches on stream, no host copy, no allocation and no
+ * synchronisation, so that a caller may capture the call in a HIP graph and
+ * replay it.  No reference counterpart: it is for callers that capture the
+ * coupled step (graph replay of a fixed-size coarse step loop).  Arguments
+ * as for hx_two_level_run; each call starts from the CFL of its inputs.
+ * dt_hist_dev (device, optional): receives 2 x nsteps (coarse dt, fine dt);
+ * when null they stay in the context.  status_dev (device, optional): one
+ * int32 set at the end of the work to 0, or to k + 1 when the estimate of
+ * step k fell below 1e-12 (the steps from k on then leave the state as it
+ * was).  The final state lies where hx_two_level_run leaves it after
+ * nsteps steps: with nsteps even, in crse's and fine's own buffers.
+ * Calls on one context share its work area: run them in stream order.
+ */
+int hx_two_level_enqueue(hx_ctx *ctx, int solver, const hx_patch *crse, void *ch, void *chu,
+                         void *chv, const hx_patch *fine, void *fh, void *fhu, void *fhv,
+                         int ci0, int cj0, int subcycle, double cfl, int nsteps, void *reg,
+                         double *dt_hist_dev, int32_t *status_dev, void *stream);
+
+/*
  * Adjoint of a single-level rollout (FP64).  No reference counterpart:
  * HydroEXA computes no derivatives.  The rollout is nsteps of the step
  * behind hx_advance_level (plain Roe or hydrostatic Roe with Manning

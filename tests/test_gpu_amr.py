@@ -264,6 +264,75 @@ def test_two_level_run_longer_than_the_context_history(gpu):
     assert torch.equal(A_.fine.valid(), B_.fine.valid())
 
 
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+@pytest.mark.parametrize("subcycle", [True, False])
+def test_two_level_graph_replay_equals_eager(gpu, model, subcycle):
+    """hx_two_level_enqueue captured in a HIP graph (4 coupled steps) and
+    replayed three times, with eager torch work and synchronisations between
+    the replays, gives after every replay bit for bit what the same call gives
+    eagerly on a twin hierarchy; the eager call equals hx_two_level_run."""
+    rng = np.random.default_rng(47)
+    st = wet_state(rng, 96, 120)
+    box = (21, 17, 40, 32)
+    lv = [Level(*st, 2.0, 2.0, (0, 1), (1, 0)) for _ in range(3)]
+    G_, E_, R_ = (TwoLevel(model, v, box, subcycle=subcycle) for v in lv)
+    dev = lv[0].U_new.device
+    hg = torch.zeros(8, dtype=torch.float64, device=dev)
+    he = torch.zeros(8, dtype=torch.float64, device=dev)
+    sg = torch.full((1,), -1, dtype=torch.int32, device=dev)
+    se = torch.full((1,), -1, dtype=torch.int32, device=dev)
+    print(f"graph replay: {model} subcycle {subcycle}", flush=True)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):             # warm-up outside the capture
+        G_.enqueue(4, 0.45, hg, sg)
+    torch.cuda.current_stream().wait_stream(side)
+    E_.enqueue(4, 0.45, he, se)
+    hist = R_.run(4, 0.45)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(he.cpu().numpy().reshape(4, 2), hist)
+    assert torch.equal(lv[1].U_new, lv[2].U_new)
+    assert torch.equal(E_.fine.valid(), R_.fine.valid())
+    assert torch.equal(lv[0].U_new, lv[1].U_new) and int(se.item()) == 0
+
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        G_.enqueue(4, 0.45, hg, sg)
+    torch.cuda.synchronize()
+    assert torch.equal(lv[0].U_new, lv[1].U_new)    # capture runs nothing
+    for r in range(3):
+        sg.fill_(-1)
+        g.replay()
+        x = torch.randn(256, 256, device=dev, dtype=torch.float64)
+        y = (x @ x).sum()                            # eager work between replays
+        E_.enqueue(4, 0.45, he, se)
+        torch.cuda.synchronize()
+        assert torch.isfinite(y)
+        assert int(sg.item()) == 0 and int(se.item()) == 0, f"replay {r}"
+        assert torch.equal(hg, he), f"replay {r}: dt history"
+        assert torch.equal(lv[0].U_new, lv[1].U_new), f"replay {r}: coarse state"
+        assert torch.equal(G_.fine.valid(), E_.fine.valid()), f"replay {r}: fine state"
+        assert torch.equal(G_.fine.U_new, E_.fine.U_new), f"replay {r}: fine storage"
+    del g
+
+
+def test_two_level_enqueue_rejects_odd_steps_and_reports_collapse(gpu):
+    rng = np.random.default_rng(42)
+    h, hu, hv, z, n = wet_state(rng, 32, 32)
+    hu[3, 3] = 1e13
+    lv = Level(h, hu, hv, z, n, 1.0, 1.0)
+    T = TwoLevel("Roe", lv, (10, 10, 8, 8))
+    dh = torch.zeros(8, dtype=torch.float64, device=lv.U_new.device)
+    s = torch.full((1,), -1, dtype=torch.int32, device=lv.U_new.device)
+    with pytest.raises(ValueError):
+        T.enqueue(3, 0.45, dh, s)
+    before = lv.U_new.clone()
+    T.enqueue(4, 0.45, dh, s)
+    torch.cuda.synchronize()
+    assert int(s.item()) == 1                  # the first step's estimate collapsed
+    assert torch.equal(lv.U_new, before)
+
+
 def test_two_level_run_reports_dt_collapse(gpu):
     rng = np.random.default_rng(42)
     h, hu, hv, z, n = wet_state(rng, 32, 32)
