@@ -127,7 +127,7 @@ constexpr int kTX = 64, kTY = 16;
 // columns (a reflective high side folds ghost n+1 into cell n-2, which may
 // be the last cell of the tile before the one holding n-1); x and y faces
 // share one flat item list, since they write different images.
-constexpr int kHX = 32, kHY = 16;
+constexpr int kHX = 16, kHY = 16, kHThreads = 256;
 constexpr int kSX = kHX + 6, kSY = kHY + 6;
 constexpr int kAXX = kHX + 5, kAYY = kHY + 5;
 struct HShared {
@@ -146,7 +146,7 @@ __device__ __forceinline__ void htile_body(HShared& sh, const State& s, const Ge
     const int t = threadIdx.x;
     // stage the cells (indices -2 .. n+1 on each axis: the faces read no
     // further) and clear the images
-    for (int k = t; k < kSY * kSX; k += kThreads) {
+    for (int k = t; k < kSY * kSX; k += kHThreads) {
         const int ly = k / kSX, lx = k - ly * kSX;
         const int i = i0 - 3 + lx, j = j0 - 3 + ly;
         double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -158,25 +158,27 @@ __device__ __forceinline__ void htile_body(HShared& sh, const State& s, const Ge
         }
         for (int m = 0; m < 5; ++m) sh.cell[m][ly][lx] = v[m];
     }
-    for (int k = t; k < 4 * kHY * kAXX; k += kThreads) (&sh.ax[0][0][0])[k] = 0.0;
-    for (int k = t; k < 4 * kAYY * kHX; k += kThreads) (&sh.ay[0][0][0])[k] = 0.0;
+    for (int k = t; k < 4 * kHY * kAXX; k += kHThreads) (&sh.ax[0][0][0])[k] = 0.0;
+    for (int k = t; k < 4 * kAYY * kHX; k += kHThreads) (&sh.ay[0][0][0])[k] = 0.0;
     __syncthreads();
     // x faces f in [i0 - 1, i0 + kHX + 2) of the tile rows, then y faces f in
-    // [j0 - 1, j0 + kHY + 2) of the tile columns
+    // [j0 - 1, j0 + kHY + 2) of the tile columns; the y items start at a
+    // multiple of the wavefront size, so each wavefront holds faces of one
+    // direction and runs one instance of the VJP
     constexpr int nX = (kHX + 3) * kHY, nY = (kHY + 3) * kHX;
+    constexpr int nXw = (nX + 63) / 64 * 64;
     const double rx = dt / dx, ry = dt / dy;
-    for (int base = 0; base < nX + nY; base += kThreads) {
+    for (int base = 0; base < nXw + nY; base += kHThreads) {
         const int q = base + t;
-        // the face's cells and output adjoints, gathered per direction; one
-        // VJP call for both, so a wavefront holding x and y faces runs it once
-        double c[20], mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0}, cb[16];
+        // the face's cells and output adjoints, gathered per direction
+        double mL[3] = {0.0, 0.0, 0.0}, mR[3] = {0.0, 0.0, 0.0}, cb[16];
+        const double* cs = &sh.cell[0][0][0];   // the face's first stencil cell
         int dir = -1, a0 = 0, a1 = 0;     // image coordinates of slot 0
         if (q < nX) {
             const int row = q / (kHX + 3), fp = q - row * (kHX + 3);
             const int j = j0 + row, f = i0 - 1 + fp;
             if (j < g.ny && f >= 0 && f <= g.nx) {
-                for (int k = 0; k < 4; ++k)
-                    for (int m = 0; m < 5; ++m) c[5 * k + m] = sh.cell[m][row + 3][fp + k];
+                cs = &sh.cell[0][row + 3][fp];
                 const int64_t cl = (int64_t)j * g.nx + f - 1;
                 if (f >= 1)
                     for (int k = 0; k < 3; ++k) mL[k] = -rx * lun[k * N + cl];
@@ -186,13 +188,12 @@ __device__ __forceinline__ void htile_body(HShared& sh, const State& s, const Ge
                 a0 = row;                // slot k lands on x index f - 2 + k
                 a1 = fp - 1;
             }
-        } else if (q < nX + nY) {
-            const int qq = q - nX;
+        } else if (q >= nXw && q < nXw + nY) {
+            const int qq = q - nXw;
             const int gp = qq / kHX, cp = qq - gp * kHX;
             const int i = i0 + cp, f = j0 - 1 + gp;
             if (i < g.nx && f >= 0 && f <= g.ny) {
-                for (int k = 0; k < 4; ++k)
-                    for (int m = 0; m < 5; ++m) c[5 * k + m] = sh.cell[m][gp + k][cp + 3];
+                cs = &sh.cell[0][gp][cp + 3];
                 if (f >= 1)
                     for (int k = 0; k < 3; ++k) mL[k] = -ry * lun[k * N + (int64_t)(f - 1) * g.nx + i];
                 if (f < g.ny)
@@ -202,7 +203,11 @@ __device__ __forceinline__ void htile_body(HShared& sh, const State& s, const Ge
                 a1 = cp;
             }
         }
-        if (dir >= 0) hydro_face_vjp(c, dir == 0 ? 1.0 : 0.0, dir == 0 ? 0.0 : 1.0, mL, mR, cb);
+        if (q < nXw) {
+            if (dir == 0) hydro_face_vjp<0>(cs, 1, kSY * kSX, mL, mR, cb);
+        } else if (dir == 1) {
+            hydro_face_vjp<1>(cs, kSX, kSY * kSX, mL, mR, cb);
+        }
         for (int k = 0; k < 4; ++k) {
             if (dir == 0) {
                 const int lx = a1 + k;
@@ -220,7 +225,7 @@ __device__ __forceinline__ void htile_body(HShared& sh, const State& s, const Ge
     // positions filled from it (x ghosts in the x image, y ghosts in the y
     // image), with the ghost-fill and rebuild chain rules; a position's depth
     // was rebuilt from eta where its staged depth is positive
-    for (int k = t; k < kHX * kHY; k += kThreads) {
+    for (int k = t; k < kHX * kHY; k += kHThreads) {
         const int ci = k % kHX, cj = k / kHX;
         const int i = i0 + ci, j = j0 + cj;
         if (i >= g.nx || j >= g.ny) continue;
@@ -349,8 +354,8 @@ __device__ __forceinline__ void rtile_body(RAcc& acc, const State& s, const Geo&
                         Spb[k] = -r * lL[k];
                         Smb[k] = -r * lR[k];
                     }
-                    roe_face_vjp(L, R, ds, dir == 0 ? 1.0 : 0.0, dir == 0 ? 0.0 : 1.0, Fb, Smb,
-                                 Spb, Lb, Rb);
+                    if (dir == 0) roe_face_vjp<0>(L, R, ds, Fb, Smb, Spb, Lb, Rb);
+                    else roe_face_vjp<1>(L, R, ds, Fb, Smb, Spb, Lb, Rb);
                     have = true;
                 }
             }
@@ -473,7 +478,7 @@ __device__ __forceinline__ Geo geo_of(const AdjScen& d)
 }
 
 // start: tile prefix
-__global__ __launch_bounds__(kThreads) void htile_kernel(Batch b)
+__global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void htile_kernel(Batch b)
 {
     __shared__ HShared acc;
     const int blk = blockIdx.x;
@@ -1025,7 +1030,7 @@ extern "C" int hx_adjoint_batch_backward(hx_adjoint* a, const double* const* g_s
             tb.t = t;
             tb.s0 = s0;
             tb.lam_in = lam_in;
-            if (hyd) hipLaunchKernelGGL(htile_kernel, dim3(a->total_tiles), dim3(kThreads), 0, hs, tb);
+            if (hyd) hipLaunchKernelGGL(htile_kernel, dim3(a->total_tiles), dim3(kHThreads), 0, hs, tb);
             else hipLaunchKernelGGL(rtile_kernel, dim3(a->total_tiles), dim3(kThreads), 0, hs, tb);
             lam_in ^= 1;
         }

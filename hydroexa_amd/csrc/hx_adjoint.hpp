@@ -82,9 +82,29 @@ struct RoeTape {
     bool cutl, cutr, htact, fix0, fix2;
 };
 
+// Rotation to the frame of a face normal to direction DIR (0: x, 1: y) and
+// its adjoint, with the products by the zero components of the normal left
+// out: un = u nx + v ny, ut = -u ny + v nx.
+template <int DIR>
+struct Nrm {
+    static __device__ __forceinline__ double un(double u, double v) { return DIR == 0 ? u : v; }
+    static __device__ __forceinline__ double ut(double u, double v) { return DIR == 0 ? v : -u; }
+    static __device__ __forceinline__ double ub(double unb, double utb)
+    {
+        return DIR == 0 ? unb : -utb;
+    }
+    static __device__ __forceinline__ double vb(double unb, double utb)
+    {
+        return DIR == 0 ? utb : unb;
+    }
+};
+
+// UV = false: the inputs are the two states' depths and momenta; UV = true:
+// depths and velocities (the hydrostatic face reconstructs velocities, and
+// u = (h u) / h is u up to rounding, so the core takes them directly).
+template <int DIR, bool UV>
 __device__ __forceinline__ void roe_core_fwd(double hl, double hul, double hvl, double hr,
-                                             double hur, double hvr, double nx, double ny,
-                                             RoeTape& t)
+                                             double hur, double hvr, RoeTape& t)
 {
     t.cutl = hl >= kDry;
     t.cutr = hr >= kDry;
@@ -97,15 +117,22 @@ __device__ __forceinline__ void roe_core_fwd(double hl, double hul, double hvl, 
     sqrt_rsq(hr > 0.0 ? hr : 1.0, s, r);
     t.sqr = hr > 0.0 ? s : 0.0;
     t.rsr = hr > 0.0 ? r : 0.0;
-    const double ihl = t.rsl * t.rsl, ihr = t.rsr * t.rsr;
-    t.ul = hul * ihl;
-    t.vl = hvl * ihl;
-    t.ur = hur * ihr;
-    t.vr = hvr * ihr;
-    t.unl = t.ul * nx + t.vl * ny;
-    t.utl = -t.ul * ny + t.vl * nx;
-    t.unr = t.ur * nx + t.vr * ny;
-    t.utr = -t.ur * ny + t.vr * nx;
+    if (UV) {
+        t.ul = hl > 0.0 ? hul : 0.0;
+        t.vl = hl > 0.0 ? hvl : 0.0;
+        t.ur = hr > 0.0 ? hur : 0.0;
+        t.vr = hr > 0.0 ? hvr : 0.0;
+    } else {
+        const double ihl = t.rsl * t.rsl, ihr = t.rsr * t.rsr;
+        t.ul = hul * ihl;
+        t.vl = hvl * ihl;
+        t.ur = hur * ihr;
+        t.vr = hvr * ihr;
+    }
+    t.unl = Nrm<DIR>::un(t.ul, t.vl);
+    t.utl = Nrm<DIR>::ut(t.ul, t.vl);
+    t.unr = Nrm<DIR>::un(t.ur, t.vr);
+    t.utr = Nrm<DIR>::ut(t.ur, t.vr);
     const double hmean = 0.5 * (hl + hr);
     t.htact = hmean > kDry;
     t.ht = t.htact ? hmean : kDry;
@@ -164,10 +191,11 @@ struct RoeBar {
 
 // Reverse sweep of roe_core_fwd.  Fb: adjoint of the unrotated flux; A:
 // extra adjoints of intermediates.  Out: adjoints of the six inputs.
-__device__ __forceinline__ void roe_core_rev(const RoeTape& t, double nx, double ny, double Fb0,
-                                             double Fb1, double Fb2, RoeBar A, double* hlb_o,
-                                             double* hulb, double* hvlb, double* hrb_o,
-                                             double* hurb, double* hvrb)
+template <int DIR, bool UV>
+__device__ __forceinline__ void roe_core_rev(const RoeTape& t, double Fb0, double Fb1,
+                                             double Fb2, RoeBar A, double* hlb_o, double* hulb,
+                                             double* hvlb, double* hrb_o, double* hurb,
+                                             double* hvrb)
 {
     double hlb = A.hl, hrb = A.hr;
     double unlb = 0.0, utlb = 0.0, unrb = 0.0, utrb = 0.0;
@@ -282,21 +310,31 @@ __device__ __forceinline__ void roe_core_rev(const RoeTape& t, double nx, double
     sqlb += kSqrtG * cglb;
     sqrb += kSqrtG * cgrb;
     // rotation to the face frame
-    const double ulb = unlb * nx - utlb * ny, vlb = unlb * ny + utlb * nx;
-    const double urb = unrb * nx - utrb * ny, vrb = unrb * ny + utrb * nx;
+    const double ulb = Nrm<DIR>::ub(unlb, utlb), vlb = Nrm<DIR>::vb(unlb, utlb);
+    const double urb = Nrm<DIR>::ub(unrb, utrb), vrb = Nrm<DIR>::vb(unrb, utrb);
     double hulb_ = 0.0, hvlb_ = 0.0, hurb_ = 0.0, hvrb_ = 0.0;
     if (t.hl > 0.0) {
-        const double ih = t.rsl * t.rsl;
-        hulb_ = ulb * ih;
-        hvlb_ = vlb * ih;
-        hlb -= (ulb * t.ul + vlb * t.vl) * ih;
+        if (UV) {
+            hulb_ = ulb;
+            hvlb_ = vlb;
+        } else {
+            const double ih = t.rsl * t.rsl;
+            hulb_ = ulb * ih;
+            hvlb_ = vlb * ih;
+            hlb -= (ulb * t.ul + vlb * t.vl) * ih;
+        }
         hlb += sqlb * 0.5 * t.rsl;
     }
     if (t.hr > 0.0) {
-        const double ih = t.rsr * t.rsr;
-        hurb_ = urb * ih;
-        hvrb_ = vrb * ih;
-        hrb -= (urb * t.ur + vrb * t.vr) * ih;
+        if (UV) {
+            hurb_ = urb;
+            hvrb_ = vrb;
+        } else {
+            const double ih = t.rsr * t.rsr;
+            hurb_ = urb * ih;
+            hvrb_ = vrb * ih;
+            hrb -= (urb * t.ur + vrb * t.vr) * ih;
+        }
         hrb += sqrb * 0.5 * t.rsr;
     }
     *hlb_o = t.cutl ? hlb : 0.0;
@@ -312,13 +350,13 @@ __device__ __forceinline__ void roe_core_rev(const RoeTape& t, double nx, double
 // In: L, R = {h, hu, hv, z, n} (h rebuilt); adjoints of F, Sm, Sp (rotated,
 // as the update uses them).  Out: Lb, Rb (5 each).
 // ---------------------------------------------------------------------------
+template <int DIR>
 __device__ __forceinline__ void roe_face_vjp(const double* L, const double* R, double ds,
-                                             double nx, double ny, const double* Fb,
-                                             const double* Smb, const double* Spb, double* Lb,
-                                             double* Rb)
+                                             const double* Fb, const double* Smb,
+                                             const double* Spb, double* Lb, double* Rb)
 {
     RoeTape t;
-    roe_core_fwd(L[0], L[1], L[2], R[0], R[1], R[2], nx, ny, t);
+    roe_core_fwd<DIR, false>(L[0], L[1], L[2], R[0], R[1], R[2], t);
     const double hl = t.hl, hr = t.hr;
     // bed thrust and friction (Roe.H:151-189)
     const double dz = R[3] - L[3];
@@ -381,8 +419,8 @@ __device__ __forceinline__ void roe_face_vjp(const double* L, const double* R, d
         }
     }
     // ---- reverse ----
-    auto unrot1 = [&](const double* X) { return X[1] * nx + X[2] * ny; };
-    auto unrot2 = [&](const double* X) { return -X[1] * ny + X[2] * nx; };
+    auto unrot1 = [&](const double* X) { return Nrm<DIR>::un(X[1], X[2]); };
+    auto unrot2 = [&](const double* X) { return Nrm<DIR>::ut(X[1], X[2]); };
     RoeBar A{};
     // sources: wave k goes to Sm when lam_k > 0, to Sp when lam_k < 0
     double betb0 = 0.0, betb2 = 0.0;
@@ -441,95 +479,49 @@ __device__ __forceinline__ void roe_face_vjp(const double* L, const double* R, d
     Lb[3] = -dzb;
     Lb[4] = 0.5 * ntb;
     Rb[4] = 0.5 * ntb;
-    roe_core_rev(t, nx, ny, Fb[0], unrot1(Fb), unrot2(Fb), A, &Lb[0], &Lb[1], &Lb[2], &Rb[0],
-                 &Rb[1], &Rb[2]);
+    roe_core_rev<DIR, false>(t, Fb[0], unrot1(Fb), unrot2(Fb), A, &Lb[0], &Lb[1], &Lb[2],
+                             &Rb[0], &Rb[1], &Rb[2]);
 }
 
 // ---------------------------------------------------------------------------
-// van Leer slope (HydrostaticRoe.cpp:173-195) with the quantities its
-// reverse sweep reuses.
+// van Leer slope (HydrostaticRoe.cpp:173-195): phi(r) dp with r = dm / dp,
+// phi(r) = (r + |r|) / (1 + |r|), and 0 where |dp| <= 1e-12.  It equals the
+// harmonic mean 2 dm dp / (dm + dp) where r >= 0 and 0 where r < 0, which
+// needs one reciprocal; its partials there, 2 (dp / (dm + dp))^2 and
+// 2 (dm / (dm + dp))^2, are at r = 0 the one-sided limits that the
+// convention |x|' = +1 at 0 gives.  With dm, dp of one sign (or dm = 0),
+// |dm + dp| >= |dp| > 1e-12.
 // ---------------------------------------------------------------------------
 struct VL {
-    double dp, idp, r, iD, phi, s;
-    bool flat;
+    double dm, dp, iS;
+    bool act;
 };
 
 __device__ __forceinline__ VL vl_eval(double qm, double q, double qp)
 {
     VL v;
-    const double dm = q - qm;
+    v.dm = q - qm;
     v.dp = qp - q;
-    v.flat = fabs0(v.dp) <= kT12;
-    v.idp = rcp(v.flat ? 1.0 : v.dp);
-    v.r = dm * v.idp;
-    const double ar = fabs0(v.r);
-    v.iD = rcp(1.0 + ar);
-    v.phi = (v.r + ar) * v.iD;
-    v.s = v.flat ? 0.0 : v.phi * v.dp;
+    v.act = (fabs0(v.dp) > kT12) && ((v.dp > 0.0) ? (v.dm >= 0.0) : (v.dm <= 0.0));
+    v.iS = rcp(v.act ? v.dm + v.dp : 1.0);
     return v;
+}
+
+__device__ __forceinline__ double vl_s(const VL& v)
+{
+    return v.act ? 2.0 * v.dm * v.dp * v.iS : 0.0;
 }
 
 __device__ __forceinline__ void vl_rev(const VL& v, double sb, double& qmb, double& qb,
                                        double& qpb)
 {
-    if (v.flat) return;
-    const double sg = sgn0(v.r);
-    const double phib = sb * v.dp;
-    double dpb = sb * v.phi;
-    const double Nb = phib * v.iD;                      // phi = N / D, N = r + |r|
-    const double rb = Nb * (1.0 + sg) - Nb * v.phi * sg; // D = 1 + |r|
-    const double dmb = rb * v.idp;
-    dpb -= rb * v.r * v.idp;
+    if (!v.act) return;
+    const double a = v.dp * v.iS, b = v.dm * v.iS;
+    const double dmb = 2.0 * sb * a * a;
+    const double dpb = 2.0 * sb * b * b;
     qb += dmb - dpb;
     qmb -= dmb;
     qpb += dpb;
-}
-
-// Reconstructed face states of one hydrostatic face.
-struct HFace {
-    VL se_l, se_r, su_l, su_r, sv_l, sv_r;
-    double u[4], v[4], ih[4];
-    bool wv[4];
-    double zf, uLf, uRf, vLf, vRf, xLu, xRu, xLf, xRf, hLu, hRu, hLf, hRf;
-    bool zfR, nodata;
-};
-
-// c: {Lm, L, R, Rp} x {h, u, v, z, 1/h} (h rebuilt; u, v and 1/h zero where
-// h <= 1e-12, as cell_prims makes them).
-__device__ __forceinline__ void hface_recon(const double* c, HFace& f)
-{
-    const double *Lm = c, *L = c + 5, *R = c + 10, *Rp = c + 15;
-    const double zL = L[3], zR = R[3];
-    f.nodata = fabs0(zL + 9999.0) < kT10 || fabs0(zR + 9999.0) < kT10;
-    f.zfR = zL < zR;
-    f.zf = f.zfR ? zR : zL;
-    const double eLm = Lm[0] + Lm[3], eL = L[0] + zL, eR = R[0] + zR, eRp = Rp[0] + Rp[3];
-    f.se_l = vl_eval(eLm, eL, eR);
-    f.se_r = vl_eval(eL, eR, eRp);
-    const double etaLf = eL + 0.5 * f.se_l.s;
-    const double etaRf = eR - 0.5 * f.se_r.s;
-    for (int q = 0; q < 4; ++q) {
-        f.ih[q] = c[5 * q + 4];
-        f.wv[q] = f.ih[q] != 0.0;
-        f.u[q] = c[5 * q + 1];
-        f.v[q] = c[5 * q + 2];
-    }
-    f.su_l = vl_eval(f.u[0], f.u[1], f.u[2]);
-    f.su_r = vl_eval(f.u[1], f.u[2], f.u[3]);
-    f.sv_l = vl_eval(f.v[0], f.v[1], f.v[2]);
-    f.sv_r = vl_eval(f.v[1], f.v[2], f.v[3]);
-    f.uLf = f.u[1] + 0.5 * f.su_l.s;
-    f.uRf = f.u[2] - 0.5 * f.su_r.s;
-    f.vLf = f.v[1] + 0.5 * f.sv_l.s;
-    f.vRf = f.v[2] - 0.5 * f.sv_r.s;
-    f.xLu = etaLf - zL;
-    f.xRu = etaRf - zR;
-    f.xLf = etaLf - f.zf;
-    f.xRf = etaRf - f.zf;
-    f.hLu = (0.0 < f.xLu) ? f.xLu : 0.0;
-    f.hRu = (0.0 < f.xRu) ? f.xRu : 0.0;
-    f.hLf = (0.0 < f.xLf) ? f.xLf : 0.0;
-    f.hRf = (0.0 < f.xRf) ? f.xRf : 0.0;
 }
 
 // A cell's primitives as the hydrostatic face uses them: 1/h, u and v where
@@ -543,76 +535,97 @@ __device__ __forceinline__ void cell_prims(double h, double hu, double hv, doubl
 }
 
 // ---------------------------------------------------------------------------
-// Hydrostatic-reconstruction face (HydrostaticRoe.cpp:218-396).
-// c: {Lm, L, R, Rp} x {h, u, v, z, 1/h} (hface_recon).  FLb, FRb: adjoints
-// of the two face fluxes.  cb (16): adjoints of {h, hu, hv, z} per cell.
+// Hydrostatic-reconstruction face (HydrostaticRoe.cpp:218-396), normal to
+// direction DIR.  Cell k of the stencil {Lm, L, R, Rp}, quantity m of
+// {h, u, v, z, 1/h} is cs[k sk + m sm] (h rebuilt; u, v and 1/h zero where
+// h <= 1e-12, as cell_prims makes them).  FLb, FRb: adjoints of the two face
+// fluxes.  cb (16): adjoints of {h, hu, hv, z} per cell.
+// The forward keeps only the face values; the reverse reads the cells again
+// and re-evaluates the slopes it differentiates, so no slope intermediate is
+// live across the Roe core.  The homogeneous Roe flux takes the face depths
+// and velocities: (h u) / h in the forward is u up to rounding, so the
+// product and the division cancel in the derivative.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void hydro_face_vjp(const double* c, double nx, double ny,
+template <int DIR>
+__device__ __forceinline__ void hydro_face_vjp(const double* cs, int sk, int sm,
                                                const double* FLb, const double* FRb, double* cb)
 {
+    auto c = [&](int k, int m) { return cs[k * sk + m * sm]; };
     for (int k = 0; k < 16; ++k) cb[k] = 0.0;
-    HFace f;
-    hface_recon(c, f);
-    if (f.nodata) return;
+    const double zL = c(1, 3), zR = c(2, 3);
+    if (fabs0(zL + 9999.0) < kT10 || fabs0(zR + 9999.0) < kT10) return;   // NODATA wall
+    const bool zfR = zL < zR;
+    const double zf = zfR ? zR : zL;
+    double etaLf, etaRf, uLf, uRf, vLf, vRf;
+    {
+        const double e0 = c(0, 0) + c(0, 3), e1 = c(1, 0) + zL, e2 = c(2, 0) + zR,
+                     e3 = c(3, 0) + c(3, 3);
+        etaLf = e1 + 0.5 * vl_s(vl_eval(e0, e1, e2));
+        etaRf = e2 - 0.5 * vl_s(vl_eval(e1, e2, e3));
+        const double u0 = c(0, 1), u1 = c(1, 1), u2 = c(2, 1), u3 = c(3, 1);
+        uLf = u1 + 0.5 * vl_s(vl_eval(u0, u1, u2));
+        uRf = u2 - 0.5 * vl_s(vl_eval(u1, u2, u3));
+        const double v0 = c(0, 2), v1 = c(1, 2), v2 = c(2, 2), v3 = c(3, 2);
+        vLf = v1 + 0.5 * vl_s(vl_eval(v0, v1, v2));
+        vRf = v2 - 0.5 * vl_s(vl_eval(v1, v2, v3));
+    }
+    const double xLu = etaLf - zL, xRu = etaRf - zR, xLf = etaLf - zf, xRf = etaRf - zf;
+    const double hLu = (0.0 < xLu) ? xLu : 0.0, hRu = (0.0 < xRu) ? xRu : 0.0;
+    const double hLf = (0.0 < xLf) ? xLf : 0.0, hRf = (0.0 < xRf) ? xRf : 0.0;
     RoeTape t;
-    roe_core_fwd(f.hLf, (f.hLf > 0.0) ? f.hLf * f.uLf : 0.0, (f.hLf > 0.0) ? f.hLf * f.vLf : 0.0,
-                 f.hRf, (f.hRf > 0.0) ? f.hRf * f.uRf : 0.0, (f.hRf > 0.0) ? f.hRf * f.vRf : 0.0,
-                 nx, ny, t);
+    roe_core_fwd<DIR, true>(hLf, uLf, vLf, hRf, uRf, vRf, t);
     // ---- reverse ----
     // FL = rot(f) + (0, pL nx, pL ny), FR likewise
     const double fb[3] = {FLb[0] + FRb[0], FLb[1] + FRb[1], FLb[2] + FRb[2]};
-    const double pLb = FLb[1] * nx + FLb[2] * ny;
-    const double pRb = FRb[1] * nx + FRb[2] * ny;
-    double hLub = pLb * kG * f.hLu, hRub = pRb * kG * f.hRu;
-    double hLfb = -pLb * kG * f.hLf, hRfb = -pRb * kG * f.hRf;
-    double huLfb, hvLfb, huRfb, hvRfb, hLfb2, hRfb2;
+    const double pLb = Nrm<DIR>::un(FLb[1], FLb[2]);
+    const double pRb = Nrm<DIR>::un(FRb[1], FRb[2]);
+    double hLub = pLb * kG * hLu, hRub = pRb * kG * hRu;
+    double hLfb = -pLb * kG * hLf, hRfb = -pRb * kG * hRf;
+    double uLfb, vLfb, uRfb, vRfb, hLfb2, hRfb2;
     RoeBar A{};
-    roe_core_rev(t, nx, ny, fb[0], fb[1] * nx + fb[2] * ny, -fb[1] * ny + fb[2] * nx, A, &hLfb2,
-                 &huLfb, &hvLfb, &hRfb2, &huRfb, &hvRfb);
+    roe_core_rev<DIR, true>(t, fb[0], Nrm<DIR>::un(fb[1], fb[2]), Nrm<DIR>::ut(fb[1], fb[2]), A,
+                            &hLfb2, &uLfb, &vLfb, &hRfb2, &uRfb, &vRfb);
     hLfb += hLfb2;
     hRfb += hRfb2;
-    double uLfb = 0.0, vLfb = 0.0, uRfb = 0.0, vRfb = 0.0;
-    if (f.hLf > 0.0) {
-        hLfb += huLfb * f.uLf + hvLfb * f.vLf;
-        uLfb = huLfb * f.hLf;
-        vLfb = hvLfb * f.hLf;
-    }
-    if (f.hRf > 0.0) {
-        hRfb += huRfb * f.uRf + hvRfb * f.vRf;
-        uRfb = huRfb * f.hRf;
-        vRfb = hvRfb * f.hRf;
-    }
     double etaLfb = 0.0, etaRfb = 0.0, zfb = 0.0, zLb = 0.0, zRb = 0.0;
-    if (0.0 < f.xLf) { etaLfb += hLfb; zfb -= hLfb; }
-    if (0.0 < f.xRf) { etaRfb += hRfb; zfb -= hRfb; }
-    if (0.0 < f.xLu) { etaLfb += hLub; zLb -= hLub; }
-    if (0.0 < f.xRu) { etaRfb += hRub; zRb -= hRub; }
+    if (0.0 < xLf) { etaLfb += hLfb; zfb -= hLfb; }
+    if (0.0 < xRf) { etaRfb += hRfb; zfb -= hRfb; }
+    if (0.0 < xLu) { etaLfb += hLub; zLb -= hLub; }
+    if (0.0 < xRu) { etaRfb += hRub; zRb -= hRub; }
+    // the cells are read again below rather than kept in registers
+    asm volatile("" ::: "memory");
     // velocity reconstruction
-    double ub[4] = {0.0, 0.0, 0.0, 0.0}, vb[4] = {0.0, 0.0, 0.0, 0.0};
-    ub[1] += uLfb;
-    vl_rev(f.su_l, 0.5 * uLfb, ub[0], ub[1], ub[2]);
-    ub[2] += uRfb;
-    vl_rev(f.su_r, -0.5 * uRfb, ub[1], ub[2], ub[3]);
-    vb[1] += vLfb;
-    vl_rev(f.sv_l, 0.5 * vLfb, vb[0], vb[1], vb[2]);
-    vb[2] += vRfb;
-    vl_rev(f.sv_r, -0.5 * vRfb, vb[1], vb[2], vb[3]);
-    for (int q = 0; q < 4; ++q) {
-        if (f.wv[q]) {
-            cb[4 * q + 1] += ub[q] * f.ih[q];
-            cb[4 * q + 2] += vb[q] * f.ih[q];
-            cb[4 * q + 0] -= (ub[q] * f.u[q] + vb[q] * f.v[q]) * f.ih[q];
+    {
+        const double u0 = c(0, 1), u1 = c(1, 1), u2 = c(2, 1), u3 = c(3, 1);
+        const double v0 = c(0, 2), v1 = c(1, 2), v2 = c(2, 2), v3 = c(3, 2);
+        double ub[4] = {0.0, uLfb, uRfb, 0.0}, vb[4] = {0.0, vLfb, vRfb, 0.0};
+        vl_rev(vl_eval(u0, u1, u2), 0.5 * uLfb, ub[0], ub[1], ub[2]);
+        vl_rev(vl_eval(u1, u2, u3), -0.5 * uRfb, ub[1], ub[2], ub[3]);
+        vl_rev(vl_eval(v0, v1, v2), 0.5 * vLfb, vb[0], vb[1], vb[2]);
+        vl_rev(vl_eval(v1, v2, v3), -0.5 * vRfb, vb[1], vb[2], vb[3]);
+        const double uq[4] = {u0, u1, u2, u3}, vq[4] = {v0, v1, v2, v3};
+        for (int q = 0; q < 4; ++q) {
+            const double ih = c(q, 4);
+            if (ih != 0.0) {
+                cb[4 * q + 1] += ub[q] * ih;
+                cb[4 * q + 2] += vb[q] * ih;
+                cb[4 * q + 0] -= (ub[q] * uq[q] + vb[q] * vq[q]) * ih;
+            }
         }
     }
     // free-surface reconstruction
-    double eb[4] = {0.0, etaLfb, etaRfb, 0.0};
-    vl_rev(f.se_l, 0.5 * etaLfb, eb[0], eb[1], eb[2]);
-    vl_rev(f.se_r, -0.5 * etaRfb, eb[1], eb[2], eb[3]);
-    for (int q = 0; q < 4; ++q) {
-        cb[4 * q + 0] += eb[q];
-        cb[4 * q + 3] += eb[q];
+    {
+        const double e0 = c(0, 0) + c(0, 3), e1 = c(1, 0) + zL, e2 = c(2, 0) + zR,
+                     e3 = c(3, 0) + c(3, 3);
+        double eb[4] = {0.0, etaLfb, etaRfb, 0.0};
+        vl_rev(vl_eval(e0, e1, e2), 0.5 * etaLfb, eb[0], eb[1], eb[2]);
+        vl_rev(vl_eval(e1, e2, e3), -0.5 * etaRfb, eb[1], eb[2], eb[3]);
+        for (int q = 0; q < 4; ++q) {
+            cb[4 * q + 0] += eb[q];
+            cb[4 * q + 3] += eb[q];
+        }
     }
-    if (f.zfR) zRb += zfb; else zLb += zfb;
+    if (zfR) zRb += zfb; else zLb += zfb;
     cb[4 + 3] += zLb;
     cb[8 + 3] += zRb;
 }
