@@ -198,6 +198,46 @@ inline void fill_geometry(StepArgs<R>& a, int solver, int rows, double dx, doubl
     a.nstrips = a.nsx * nsy;
 }
 
+#ifndef HX_TAIL_ROWS_ROE
+#define HX_TAIL_ROWS_ROE 4
+#endif
+#ifndef HX_TAIL_ROWS_HYDRO
+#define HX_TAIL_ROWS_HYDRO 8
+#endif
+#ifndef HX_TAIL_PCT
+#define HX_TAIL_PCT 100
+#endif
+
+// Banded tiling of a level's strips for advance_kernel.  A launch ends when
+// its last wavefronts do; with equal strips the SIMDs go idle one by one
+// over the last wavefront's lifetime.  The kernel gives XCD x the rows
+// [x ny / 8, (x + 1) ny / 8) and marches them bottom to top in dispatch
+// order, so the strips that start last are the ones of the band's top
+// `tail_band` rows, cut to tail_rows rows: about one round of the XCD's
+// wavefront slots (CUs / 8 x 4 SIMDs x waves per SIMD).  Levels that fill
+// the device fewer than two times keep the uniform tiling.
+template <typename R>
+inline void plan_bands(StepArgs<R>& a, int solver, int cus, int simd_waves)
+{
+    a.band_wgs = 0;
+    const int ht = (solver == HX_SOLVER_ROE) ? HX_TAIL_ROWS_ROE : HX_TAIL_ROWS_HYDRO;
+    if (ht <= 0 || a.rows <= ht || cus < 8 || simd_waves <= 0) return;
+    const int64_t slots_xcd = (int64_t)(cus / 8) * 4 * simd_waves;
+    if ((int64_t)a.nstrips < 2 * 8 * slots_xcd) return;
+    int64_t tail = (HX_TAIL_PCT * slots_xcd * ht / 100 + a.nsx - 1) / a.nsx;
+    tail = std::max<int64_t>(ht, ((tail + ht - 1) / ht) * ht);
+    const int bmin = a.p.ny / 8, bmax = (a.p.ny + 7) / 8;
+    if (tail + a.rows > bmin) return;
+    auto strips = [&](int b) {
+        const int64_t nm = (b - tail) / a.rows;
+        const int64_t rest = b - nm * a.rows;
+        return (int64_t)a.nsx * (nm + (rest + ht - 1) / ht);
+    };
+    a.tail_rows = ht;
+    a.tail_band = (int)tail;
+    a.band_wgs = (int)((std::max(strips(bmin), strips(bmax)) + kWaves - 1) / kWaves);
+}
+
 // Defined in hx_kernels.hip.
 int check_patch(const hx_patch* in, int need_ghost);
 int launch_step_f64(const hx_ctx* c, int solver, const StepArgs<double>& a, hipStream_t st);

@@ -786,9 +786,11 @@ int hx::host::advance_simd_waves(const hx_ctx* c, int solver, int dtype)
 }
 
 template <typename R>
-static int launch(const hx_ctx* c, int solver, int dtype, const StepArgs<R>& a, hipStream_t st)
+static int launch(const hx_ctx* c, int solver, int dtype, const StepArgs<R>& a0, hipStream_t st)
 {
     const int occ = eff_occ(c, solver, dtype);
+    StepArgs<R> a = a0;
+    plan_bands(a, solver, c->cus, advance_simd_waves(c, solver, dtype));
     const int rc = (c->arith == HX_ARITH_FAST) ? fast::launch_advance<R>(solver, occ, a, st)
                                                : exact::launch_advance<R>(solver, occ, a, st);
     return rc ? HX_E_HIP : HX_OK;

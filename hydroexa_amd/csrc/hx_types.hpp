@@ -88,7 +88,37 @@ struct StepArgs {
     // output cells [sk_i0, sk_i1) x [sk_j0, sk_j1) are left out of the CFL
     // reduction (a later pass reduces them after they change)
     int sk_i0 = 0, sk_j0 = 0, sk_i1 = 0, sk_j1 = 0;
+    // banded tiling (advance_kernel): when band_wgs > 0 the grid is
+    // 8 * band_wgs workgroups and XCD x marches rows [x ny / 8, (x + 1) ny / 8)
+    // in strips of `rows` rows, the last tail_band of them in strips of
+    // tail_rows; 0: uniform strips of `rows` rows
+    int tail_rows = 0, tail_band = 0, band_wgs = 0;
 };
+
+// The strip of the banded tiling that item i (workgroup index within its
+// XCD times kWaves plus the wavefront) of XCD `xcd` marches: its column
+// index sxi and rows [y0, y1); none when y0 >= y1.  XCD x takes the rows
+// [x ny / 8, (x + 1) ny / 8) bottom to top: strips of a.rows rows, then, over
+// at least the band's last a.tail_band rows, strips of a.tail_rows rows.
+template <typename R>
+__host__ __device__ inline void band_strip(const StepArgs<R>& a, unsigned xcd, int i, int& sxi,
+                                           int& y0, int& y1)
+{
+    const int ny = a.p.ny;
+    const int yb0 = (int)(((int64_t)xcd * ny) >> 3);
+    const int yb1 = (int)(((int64_t)(xcd + 1) * ny) >> 3);
+    const int nm0 = (yb1 - yb0 - a.tail_band) / a.rows;
+    const int nm = nm0 > 0 ? nm0 : 0;
+    const int sr = i / a.nsx;
+    sxi = i - sr * a.nsx;
+    if (sr < nm) {
+        y0 = yb0 + sr * a.rows;
+        y1 = y0 + a.rows;
+    } else {
+        y0 = yb0 + nm * a.rows + (sr - nm) * a.tail_rows;
+        y1 = (y0 + a.tail_rows < yb1) ? y0 + a.tail_rows : yb1;
+    }
+}
 
 // Faces whose effective fluxes build a flux register: a device list of
 // {dir, i, j, side}, or (faces == null) the faces around the fine box of

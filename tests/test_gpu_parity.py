@@ -542,6 +542,33 @@ def test_strip_height_leaves_results_unchanged(gpu, arith, model):
         set_arith("exact")
 
 
+@pytest.mark.parametrize("arith", ["exact", "fast"])
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_banded_tiling_equals_uniform_strips(gpu, arith, model):
+    """A level that fills the device at least twice is marched in XCD bands
+    whose top rows go to short strips (plan_bands): 16-row strips on a
+    1000 x 8192 rough, partly dry level (17 strips across, 4352 or more
+    wavefronts, banded on 256 CUs) give the same states and dt, bit for bit,
+    as strips no taller than the tail strips (uniform tiling) after 8 steps."""
+    from hydroexa_amd import set_arith, set_tuning
+    st = random_state(np.random.default_rng(17), 8192, 1000, dry_frac=0.1, friction=True)
+    set_arith(arith)
+    try:
+        outs = []
+        for rows in (16, 4 if model == "Roe" else 8):
+            set_tuning(model, torch.float64, rows, 0)
+            lv = Level(*st, 1.0 / 1000, 1.0 / 8192, (0, 1), (1, 0))
+            dth, _ = SolverVariant(model).run(lv, 0.45, 8)
+            outs.append(([lv.U_new[k].cpu().numpy() for k in range(3)], np.asarray(dth)))
+            del lv
+        for k in range(3):
+            assert np.array_equal(outs[1][0][k], outs[0][0][k]), k
+        assert np.array_equal(outs[1][1], outs[0][1])
+    finally:
+        set_tuning(model, torch.float64, 0, 0)
+        set_arith("exact")
+
+
 @pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
 def test_widest_pitch_shortens_strips(gpu, model):
     """The advance addresses a strip as base + 32-bit byte offsets, so at the

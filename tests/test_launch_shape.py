@@ -153,3 +153,94 @@ def test_strip_clamp_leaves_ordinary_levels_alone(offsets):
     assert offsets(True, HYD, 4100, 4104, 4100, 2)[0] == 16
     # at a pitch of 2^26 FP64 elements three rows fit
     assert offsets(True, HYD, WIDEST, WIDEST, WIDEST, 0)[0] == 3
+
+
+BANDS_SRC = r"""
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#include "hx_host.hpp"
+// Plans the banded tiling of an nx x ny level (plan_bands) and enumerates
+// every item of the grid through band_strip, as advance_kernel does: prints
+// "uniform", or "banded <workgroups> <tail rows> <tail band> <max height>"
+// followed by "ok" when every cell of every strip column is marched exactly
+// once and no strip is taller than the planned height.
+int main(int argc, char** argv)
+{
+    (void)argc;
+    const int solver = atoi(argv[1]), nx = atoi(argv[2]), ny = atoi(argv[3]);
+    const int rows = atoi(argv[4]), cus = atoi(argv[5]), waves = atoi(argv[6]);
+    hx::StepArgs<double> a{};
+    a.p.nx = nx;
+    a.p.ny = ny;
+    a.p.pitch = nx;
+    a.opitch = nx;
+    hx::host::fill_geometry(a, solver, rows, 1.0, 1.0);
+    hx::host::plan_bands(a, solver, cus, waves);
+    if (!a.band_wgs) {
+        printf("uniform\n");
+        return 0;
+    }
+    std::vector<int> hits((size_t)a.nsx * ny, 0);
+    int tallest = 0;
+    bool ok = true;
+    for (long b = 0; b < 8L * a.band_wgs; ++b)
+        for (int w = 0; w < hx::kWaves; ++w) {
+            int sxi, y0, y1;
+            hx::band_strip(a, (unsigned)(b & 7), (int)((b >> 3) * hx::kWaves + w), sxi, y0, y1);
+            if (y0 >= y1) continue;
+            if (sxi < 0 || sxi >= a.nsx || y0 < 0 || y1 > ny) { ok = false; continue; }
+            tallest = y1 - y0 > tallest ? y1 - y0 : tallest;
+            for (int y = y0; y < y1; ++y) hits[(size_t)y * a.nsx + sxi]++;
+        }
+    for (int v : hits) ok = ok && (v == 1);
+    printf("banded %d %d %d %d %s\n", a.band_wgs, a.tail_rows, a.tail_band, tallest,
+           ok && tallest <= a.rows ? "ok" : "bad");
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def bands(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not found")
+    d = tmp_path_factory.mktemp("bands")
+    src, exe = d / "bands.cpp", d / "bands"
+    src.write_text(BANDS_SRC)
+    subprocess.run([HIPCC, "-std=c++17", f"-I{ROOT}/include", f"-I{ROOT}/hydroexa_amd/csrc",
+                    str(src), "-o", str(exe)], check=True, capture_output=True)
+
+    def run(solver, nx, ny, rows, cus=256, waves=3):
+        return subprocess.run([str(exe), *map(str, (solver, nx, ny, rows, cus, waves))],
+                              check=True, capture_output=True, text=True).stdout.split()
+    return run
+
+
+@pytest.mark.parametrize("solver,nx,ny,rows", [
+    (ROE, 4096, 4096, 16), (ROE, 4096, 4103, 16), (ROE, 1000, 8192, 16), (ROE, 4096, 4096, 32),
+    (HYD, 4096, 4096, 31), (HYD, 4100, 4097, 16), (HYD, 1000, 8192, 16), (ROE, 8192, 8192, 16),
+])
+def test_banded_tiling_marches_every_cell_once(bands, solver, nx, ny, rows):
+    out = bands(solver, nx, ny, rows)
+    assert out[0] == "banded", out
+    assert out[-1] == "ok", out
+    wgs, tail_rows, tail_band, tallest = map(int, out[1:5])
+    assert tail_rows == (4 if solver == ROE else 8) and tail_band >= tail_rows
+    assert tallest == rows
+
+
+def test_banded_tiling_at_the_headline_size(bands):
+    # 4096^2 plain Roe, 16-row strips, 3 wavefronts per SIMD on 256 CUs: one
+    # round of an XCD's 384 slots in 4-row strips is 23 rows of 67 strips,
+    # rounded to 24; the 512-row bands keep 30 strips of 16 rows and 8 of 4
+    out = bands(ROE, 4096, 4096, 16)
+    assert out == ["banded", str((67 * 38 + 3) // 4), "4", "24", "16", "ok"]
+
+
+def test_levels_that_fill_the_device_less_than_twice_stay_uniform(bands):
+    assert bands(ROE, 2048, 2048, 16) == ["uniform"]        # 4352 strips < 2 x 3072
+    assert bands(HYD, 2048, 2048, 24) == ["uniform"]
+    assert bands(ROE, 4096, 4096, 4) == ["uniform"]         # no shorter tail strip
+    assert bands(HYD, 4096, 4096, 8) == ["uniform"]
+    assert bands(ROE, 4096, 4096, 16, 256, 0) == ["uniform"]  # occupancy unknown
