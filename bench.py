@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+CONDITION_MS = 60.0
 
 
 def parse():
@@ -43,6 +44,8 @@ def parse():
     ap.add_argument("--rows", type=int, default=0, help="strip rows per wavefront (0 = default)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget of the five timed CPU-baseline samples, per thread count")
+    ap.add_argument("--condition-ms", type=float, default=CONDITION_MS,
+                    help="HBM streaming right before each timed region's warm-up (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the other advance workloads (bumpy bed, FP32, lakes at rest)")
@@ -139,7 +142,44 @@ def cpu_baseline(n, budget_s):
             "cpu_model": _cpu_model()}
 
 
-def time_gpu(w, dtype, steps, warmup, rows, arith="fast", keep=False):
+def condition_device(ms=CONDITION_MS, cells=1 << 22):
+    """Keep the device streaming HBM for about `ms` ms (hx_stream_probe on
+    its own buffers, no state of the workload touched).  After an idle
+    period -- the host-to-device upload of a level -- the device runs an
+    HBM-bound kernel at a clock state that differs from its sustained one for
+    about 30 steps of the 4096^2 advance (the same on a lake at rest with no
+    flow at all), so a short timed region that starts right after an upload
+    measures that transient.  Running this just before the warm-up steps
+    puts the timed steps at the sustained state a long simulation runs at.
+    Returns the milliseconds spent."""
+    import ctypes as C
+    import torch
+    from hydroexa_amd import _lib as L
+
+    if ms <= 0:
+        return 0.0
+    lib = L.load()
+    ctx = C.c_void_p()
+    L.check(lib.hx_ctx_create(C.byref(ctx), torch.cuda.current_device()), "hx_ctx_create")
+    try:
+        planes = [torch.full((cells,), 0.5 + k, dtype=torch.float64, device="cuda") for k in range(8)]
+        ins = (C.c_void_p * 5)(*[t.data_ptr() for t in planes[:5]])
+        outs = (C.c_void_p * 3)(*[t.data_ptr() for t in planes[5:]])
+        sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while (time.perf_counter() - t0) * 1e3 < ms:
+            for _ in range(50):
+                L.check(lib.hx_stream_probe(ctx, cells, ins, outs, sp), "hx_stream_probe")
+            torch.cuda.synchronize()
+        spent = (time.perf_counter() - t0) * 1e3
+        del planes
+    finally:
+        lib.hx_ctx_destroy(ctx)
+    return spent
+
+
+def time_gpu(w, dtype, steps, warmup, rows, arith="fast", keep=False, condition_ms=CONDITION_MS):
     import torch
     from hydroexa_amd import Level, SolverVariant, set_arith, set_tuning
 
@@ -148,6 +188,7 @@ def time_gpu(w, dtype, steps, warmup, rows, arith="fast", keep=False):
     set_arith(arith)
     lv = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi, dtype=tdt)
     sv = SolverVariant(w.solver)
+    condition_device(condition_ms)
     sv.run(lv, w.cfl, warmup)
     torch.cuda.synchronize()
     st = torch.cuda.current_stream()
@@ -599,7 +640,7 @@ def main():
         w = W.lake_at_rest(args.n, args.steps)
     es = 8 if args.dtype == "f64" else 4
     ev_ms, wall, dth, fin = time_gpu(w, args.dtype, args.steps, args.warmup, args.rows,
-                                     args.arith, keep=True)
+                                     args.arith, keep=True, condition_ms=args.condition_ms)
     ms_per_step = ev_ms / args.steps
     cups = w.cells * args.steps / (ev_ms / 1e3)
     bytes_per_launch = 8 * es * w.cells          # read h,hu,hv,z,n; write h,hu,hv
@@ -607,7 +648,8 @@ def main():
     # the other arithmetic build on the same workload, and the distance
     # between the two final states
     other = "exact" if args.arith == "fast" else "fast"
-    ev2, _, _, fin2 = time_gpu(w, args.dtype, args.steps, args.warmup, args.rows, other, keep=True)
+    ev2, _, _, fin2 = time_gpu(w, args.dtype, args.steps, args.warmup, args.rows, other, keep=True,
+                               condition_ms=args.condition_ms)
     linf = [float((fin[k] - fin2[k]).abs().max() / fin2[k].abs().max().clamp_min(1e-300))
             for k in range(3)]
     kname = f"advance_{w.solver.lower()}_{args.dtype}_{args.arith}_{args.n}"
@@ -636,6 +678,11 @@ def main():
         "arith_check": {"other": other, "other_ms_per_step": ev2 / args.steps,
                         "other_Mcups": w.cells * args.steps / (ev2 / 1e3) / 1e6,
                         "rel_linf_h_hu_hv": linf},
+        "timing": {"warmup_steps": args.warmup, "timed_steps": args.steps,
+                   "device_conditioning_ms": args.condition_ms,
+                   "conditioning": "hx_stream_probe on its own buffers right before the warm-up "
+                                   "steps, after the level's upload (the device's clock state "
+                                   "after idle differs from its sustained one for ~30 steps)"},
         "wall_s": wall,
         "final_dt": float(dth[-1]) if len(dth) else None,
     }
@@ -659,7 +706,7 @@ def main():
                  ("lake_at_rest_2048_f64_fast", W.lake_at_rest(2048, 1), "f64", "fast", 0),
                  ("lake_at_rest_1024_f32_fast", W.lake_at_rest(1024, 1), "f32", "fast", 0))
         for name, wk, dt_, ar, rows in cases:
-            ms, _, _, _ = time_gpu(wk, dt_, 200, 10, rows, ar)
+            ms, _, _, _ = time_gpu(wk, dt_, 200, 10, rows, ar, condition_ms=args.condition_ms)
             e = 8 if dt_ == "f64" else 4
             extra[name] = {"Mcups": wk.cells * 200 / (ms / 1e3) / 1e6, "ms_per_step": ms / 200,
                            "hbm_frac": 8 * e * wk.cells / (ms / 200 / 1e3) / 1e9 / HBM_PEAK_GBS}
