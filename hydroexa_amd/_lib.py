@@ -35,7 +35,8 @@ HX_ARITH_FAST = 1
 # Every symbol include/hydroexa.h declares.
 EXPORTS = (
     "hx_ctx_create", "hx_ctx_destroy", "hx_abi_version", "hx_status_string",
-    "hx_advance_level", "hx_level_dt", "hx_tag_level", "hx_masked_average_down",
+    "hx_advance_level", "hx_advance_level_fluxes", "hx_level_dt", "hx_tag_level",
+    "hx_masked_average_down",
     "hx_prune_boxes", "hx_run_level", "hx_run_level_resume", "hx_run_level_noalloc", "hx_set_tuning", "hx_set_arith",
     "hx_ensemble_create", "hx_ensemble_destroy", "hx_ensemble_set_tuning", "hx_ensemble_run",
     "hx_adjoint_create", "hx_adjoint_destroy", "hx_adjoint_forward", "hx_adjoint_backward",
@@ -101,6 +102,7 @@ def load() -> C.CDLL:
     L.hx_status_string.argtypes = [C.c_int]
     L.hx_status_string.restype = C.c_char_p
     L.hx_advance_level.argtypes = [vp, C.c_int, pp, C.c_double, vp, vp, vp, C.c_int64, vp]
+    L.hx_advance_level_fluxes.argtypes = [vp, C.c_int, pp, C.c_double, vp, vp, vp, C.c_int64, vp, vp]
     L.hx_level_dt.argtypes = [vp, C.c_int, pp, C.c_double, dp, dp, vp]
     L.hx_tag_level.argtypes = [vp, C.c_int, pp, C.c_double, dp, vp, vp]
     L.hx_masked_average_down.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int,

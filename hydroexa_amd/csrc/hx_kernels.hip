@@ -798,7 +798,7 @@ static int launch(const hx_ctx* c, int solver, int dtype, const StepArgs<R>& a0,
 
 template <typename R>
 static int advance_t(hx_ctx* c, int solver, const hx_patch* in, double dt, void* ho, void* huo,
-                     void* hvo, int64_t opitch, hipStream_t st)
+                     void* hvo, int64_t opitch, void* ef, hipStream_t st)
 {
     StepArgs<R> a{};
     a.p = view_of<R>(in);
@@ -807,12 +807,13 @@ static int advance_t(hx_ctx* c, int solver, const hx_patch* in, double dt, void*
     a.hvo = (R*)hvo;
     a.opitch = opitch;
     a.dt_host = dt;
+    a.ef = (R*)ef;
     fill_geometry(a, solver, eff_rows(c, solver, in->dtype, in->nx, in->ny), in->dx, in->dy);
     return launch(c, solver, in->dtype, a, st);
 }
 
-extern "C" int hx_advance_level(hx_ctx* c, int solver, const hx_patch* in, double dt, void* ho,
-                                void* huo, void* hvo, int64_t opitch, void* stream)
+static int advance_any(hx_ctx* c, int solver, const hx_patch* in, double dt, void* ho, void* huo,
+                       void* hvo, int64_t opitch, void* ef, void* stream)
 {
     if (!c) return HX_E_STATE;
     if (solver != HX_SOLVER_ROE && solver != HX_SOLVER_HYDROSTATIC_ROE) return HX_E_BADARG;
@@ -822,8 +823,22 @@ extern "C" int hx_advance_level(hx_ctx* c, int solver, const hx_patch* in, doubl
         return HX_E_BADARG;
     HX_TRY(hipSetDevice(c->device));
     hipStream_t st = (hipStream_t)stream;
-    return (in->dtype == HX_F64) ? advance_t<double>(c, solver, in, dt, ho, huo, hvo, opitch, st)
-                                 : advance_t<float>(c, solver, in, dt, ho, huo, hvo, opitch, st);
+    return (in->dtype == HX_F64) ? advance_t<double>(c, solver, in, dt, ho, huo, hvo, opitch, ef, st)
+                                 : advance_t<float>(c, solver, in, dt, ho, huo, hvo, opitch, ef, st);
+}
+
+extern "C" int hx_advance_level(hx_ctx* c, int solver, const hx_patch* in, double dt, void* ho,
+                                void* huo, void* hvo, int64_t opitch, void* stream)
+{
+    return advance_any(c, solver, in, dt, ho, huo, hvo, opitch, nullptr, stream);
+}
+
+extern "C" int hx_advance_level_fluxes(hx_ctx* c, int solver, const hx_patch* in, double dt,
+                                       void* ho, void* huo, void* hvo, int64_t opitch,
+                                       void* edge_flux, void* stream)
+{
+    if (!edge_flux) return HX_E_BADARG;
+    return advance_any(c, solver, in, dt, ho, huo, hvo, opitch, edge_flux, stream);
 }
 
 template <typename R>

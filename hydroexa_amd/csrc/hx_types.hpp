@@ -93,6 +93,12 @@ struct StepArgs {
     // in strips of `rows` rows, the last tail_band of them in strips of
     // tail_rows; 0: uniform strips of `rows` rows
     int tail_rows = 0, tail_band = 0, band_wgs = 0;
+    // when non-null, the effective fluxes of the patch's boundary faces as
+    // its own cells take them: (2 ny + 2 nx) x 3 values, x-low faces
+    // (i = 0) by row, x-high (i = nx), y-low (j = 0) by column, y-high
+    // (j = ny); plain Roe D- on the low faces and D+ on the high faces,
+    // hydrostatic 0.5 FL + 0.5 FR
+    R* ef = nullptr;
 };
 
 // The strip of the banded tiling that item i (workgroup index within its

@@ -121,18 +121,38 @@ class SolverVariant:
         return self.model is not None
 
     # -- SolverVariant::compute_fluxes ------------------------------------
-    def compute_fluxes(self, level: Level, dt: float, time: float = 0.0) -> None:
+    def compute_fluxes(self, level: Level, dt: float, time: float = 0.0,
+                       edge_fluxes: torch.Tensor = None) -> None:
         """Advance ``level`` by dt.  Swaps U_new/U_old, then writes U_new from
-        U_old (src/solvers/Roe.cpp:255, HydrostaticRoe.cpp:405)."""
+        U_old (src/solvers/Roe.cpp:255, HydrostaticRoe.cpp:405).
+
+        edge_fluxes: optional (2 ny + 2 nx, 3) device tensor of the level's
+        dtype that receives, from the same kernel, the effective fluxes of the
+        box's boundary faces as its own cells took them -- what the reference
+        hands to the fine side of a flux register (Roe.cpp:544-566,
+        HydrostaticRoe.cpp:612-627): x-low faces by row, x-high, y-low by
+        column, y-high (hx_advance_level_fluxes)."""
         kind = self._kind()
+        if edge_fluxes is not None:
+            _require_cuda(edge_fluxes, "edge_fluxes")
+            if (tuple(edge_fluxes.shape) != (2 * level.ny + 2 * level.nx, 3)
+                    or edge_fluxes.dtype != level.dtype or not edge_fluxes.is_contiguous()):
+                raise ValueError("edge_fluxes must be a contiguous (2 ny + 2 nx, 3) tensor of the "
+                                 "level's dtype")
         level.swap()
         p = level.patch(level.U_old)
         es = level.U_new.element_size()
         plane = level.nx * level.ny * es
         base = level.U_new.data_ptr()
-        rc = L.load().hx_advance_level(context(), kind, C.byref(p), float(dt), base, base + plane,
-                                       base + 2 * plane, level.nx, _stream())
-        L.check(rc, "hx_advance_level")
+        if edge_fluxes is None:
+            rc = L.load().hx_advance_level(context(), kind, C.byref(p), float(dt), base,
+                                           base + plane, base + 2 * plane, level.nx, _stream())
+            L.check(rc, "hx_advance_level")
+        else:
+            rc = L.load().hx_advance_level_fluxes(context(), kind, C.byref(p), float(dt), base,
+                                                  base + plane, base + 2 * plane, level.nx,
+                                                  edge_fluxes.data_ptr(), _stream())
+            L.check(rc, "hx_advance_level_fluxes")
 
     # -- compute_dt_level / BaseSolver::compute_dt -------------------------
     def compute_dt_level(self, level: Level, cfl: float) -> Tuple[float, float]:

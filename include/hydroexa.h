@@ -96,11 +96,30 @@ const char *hx_status_string(int status);
  * fused into one kernel.  Output arrays (device, out_pitch) receive the new
  * h, hu, hv of the valid cells; they must not alias the inputs.  The
  * reference swaps U_new/U_old inside the call; here the caller swaps.
- * Reflux face fluxes are not produced.
+ * Reflux face fluxes are not produced (hx_advance_level_fluxes does).
  */
 int hx_advance_level(hx_ctx *ctx, int solver, const hx_patch *in, double dt,
                      void *h_out, void *hu_out, void *hv_out, int64_t out_pitch,
                      void *stream);
+
+/*
+ * hx_advance_level, plus the effective fluxes of the box's boundary faces as
+ * the box's own cells took them in the same kernel: the fine side of a flux
+ * register (src/solvers/Roe.cpp:544-566, the fr_as_fine branch of the
+ * face-oriented selector: D- = F - S- on the low faces, whose cell lies on
+ * the high side, D+ = F + S+ on the high faces; hydrostatic 0.5 FL + 0.5 FR,
+ * src/solvers/HydrostaticRoe.cpp:612-627).  edge_flux (device, patch dtype,
+ * not aliasing anything else): (2 ny + 2 nx) rows of 3 values (h, hu, hv
+ * components, x-y frame), unscaled: rows [0, ny) the x-low faces (i = 0)
+ * from j = 0 up, [ny, 2 ny) the x-high faces (i = nx), [2 ny, 2 ny + nx) the
+ * y-low faces (j = 0) from i = 0 right, [2 ny + nx, 2 ny + 2 nx) the y-high
+ * faces (j = ny).  The reference hands these to FluxRegister::FineAdd with
+ * scale dt dA; they equal what hx_face_fluxes re-evaluates for the same
+ * faces and sides.
+ */
+int hx_advance_level_fluxes(hx_ctx *ctx, int solver, const hx_patch *in, double dt,
+                            void *h_out, void *hu_out, void *hv_out, int64_t out_pitch,
+                            void *edge_flux, void *stream);
 
 /*
  * CFL timestep of one level.  Replaces compute_dt_Impl
