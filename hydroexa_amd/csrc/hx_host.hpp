@@ -41,6 +41,7 @@ struct hx_ctx {
     hx::CflSlot* slots;  // 4 slots: 3 for the fused loop, 1 for hx_level_dt
     int* err;
     double* hist;        // 2 * hist_cap doubles (dt, vel), allocated with the context
+    double* hpin;        // pinned host staging of a run chunk's error flag and history
     int hist_cap;
     hx::TwoLevelWork* tlw;   // hx_two_level_run's slots, dt pair and error flag
     hx::CflSlot* amr_slots;  // hx_amr_run's per-level CFL slots (HX_AMR_MAX_LEVELS)

@@ -675,16 +675,22 @@ extern "C" int hx_ctx_create(hx_ctx** out, int device)
     if (!c) return HX_E_NOMEM;
     c->device = device;
     c->hist_cap = kHistCap;
+    // pinned staging for the run loop's read-back: the error flag, then up to
+    // the larger chunk's dt and vel histories (pageable copies took 20-30 us
+    // each after a run's last step)
+    const size_t pin = sizeof(double) * (1 + 2 * (size_t)std::max(kHistCap, kNoAllocSteps));
     if (hipMalloc(&c->slots, 4 * sizeof(CflSlot)) != hipSuccess ||
         hipMalloc(&c->err, sizeof(int)) != hipSuccess ||
         hipMalloc(&c->hist, sizeof(double) * 2 * kHistCap) != hipSuccess ||
         hipMalloc(&c->tlw, sizeof(TwoLevelWork)) != hipSuccess ||
-        hipMalloc(&c->amr_slots, sizeof(CflSlot) * HX_AMR_MAX_LEVELS) != hipSuccess) {
+        hipMalloc(&c->amr_slots, sizeof(CflSlot) * HX_AMR_MAX_LEVELS) != hipSuccess ||
+        hipHostMalloc(&c->hpin, pin, hipHostMallocDefault) != hipSuccess) {
         hipFree(c->slots);
         hipFree(c->err);
         hipFree(c->hist);
         hipFree(c->tlw);
         hipFree(c->amr_slots);
+        if (c->hpin) hipHostFree(c->hpin);
         delete c;
         return HX_E_NOMEM;
     }
@@ -729,6 +735,7 @@ extern "C" int hx_ctx_destroy(hx_ctx* c)
     hipFree(c->hist);
     hipFree(c->tlw);
     hipFree(c->amr_slots);
+    if (c->hpin) hipHostFree(c->hpin);
     if (c->amr_tab) hipFree(c->amr_tab);
     if (c->amr_host) hipHostFree(c->amr_host);
     if (c->pflags) hipFree(c->pflags);
@@ -1190,15 +1197,21 @@ static int run_level(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* 
                  ? run_t<double>(c, solver, in, ah, ahu, ahv, cfl, s0, s1, base, !resume, sc, st)
                  : run_t<float>(c, solver, in, ah, ahu, ahv, cfl, s0, s1, base, !resume, sc, st);
         if (rc) return rc;
-        int err = 0;
-        HX_TRY(hipMemcpyAsync(&err, sc.err, sizeof(int), hipMemcpyDeviceToHost, st));
+        // read back through the context's pinned staging: [error flag][dt][vel]
+        const size_t n = (size_t)(s1 - s0);
+        int* perr = (int*)c->hpin;
+        double* pdt = c->hpin + 1;
+        double* pvel = pdt + n;
+        HX_TRY(hipMemcpyAsync(perr, sc.err, sizeof(int), hipMemcpyDeviceToHost, st));
         if (dt_hist)
-            HX_TRY(hipMemcpyAsync(dt_hist + s0, sc.hist, sizeof(double) * (s1 - s0),
-                                  hipMemcpyDeviceToHost, st));
+            HX_TRY(hipMemcpyAsync(pdt, sc.hist, sizeof(double) * n, hipMemcpyDeviceToHost, st));
         if (vel_hist)
-            HX_TRY(hipMemcpyAsync(vel_hist + s0, sc.hist + sc.cap, sizeof(double) * (s1 - s0),
+            HX_TRY(hipMemcpyAsync(pvel, sc.hist + sc.cap, sizeof(double) * n,
                                   hipMemcpyDeviceToHost, st));
         HX_TRY(hipStreamSynchronize(st));
+        const int err = *perr;
+        if (dt_hist) std::memcpy(dt_hist + s0, pdt, sizeof(double) * n);
+        if (vel_hist) std::memcpy(vel_hist + s0, pvel, sizeof(double) * n);
         if (err) {
             if (steps_done) *steps_done = s0 + err - 1;
             return HX_E_DT_COLLAPSE;
