@@ -901,8 +901,9 @@ extern "C" int hx_level_dt(hx_ctx* c, int solver, const hx_patch* in, double cfl
                                : launch_cfl<float>(in, c->slots + 3, st);
     if (rc) return rc;
     CflSlot s;
-    HX_TRY(hipMemcpyAsync(&s, c->slots + 3, sizeof(s), hipMemcpyDeviceToHost, st));
+    HX_TRY(hipMemcpyAsync(c->hpin, c->slots + 3, sizeof(s), hipMemcpyDeviceToHost, st));
     HX_TRY(hipStreamSynchronize(st));
+    std::memcpy(&s, c->hpin, sizeof(s));
     double vmax;
     std::memcpy(&vmax, &s.vmax, sizeof(vmax));
     double est;
@@ -1840,12 +1841,16 @@ extern "C" int hx_two_level_run(hx_ctx* c, int solver, const hx_patch* crse, voi
                  : two_level_t<float>(c, solver, crse, calt, fine, falt, ci0, cj0, subcycle != 0,
                                       cfl, s0, s1, fcur, reg, c->tlw, c->hist, st);
         if (rc) return rc;
-        int err = 0;
-        HX_TRY(hipMemcpyAsync(&err, &c->tlw->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        // read back through the context's pinned staging: [error flag][dt pairs]
+        int* perr = (int*)c->hpin;
+        double* pdt = c->hpin + 1;
+        HX_TRY(hipMemcpyAsync(perr, &c->tlw->err, sizeof(int), hipMemcpyDeviceToHost, st));
         if (dt_hist)
-            HX_TRY(hipMemcpyAsync(dt_hist + 2 * (size_t)s0, c->hist, sizeof(double) * 2 * (s1 - s0),
+            HX_TRY(hipMemcpyAsync(pdt, c->hist, sizeof(double) * 2 * (s1 - s0),
                                   hipMemcpyDeviceToHost, st));
         HX_TRY(hipStreamSynchronize(st));
+        const int err = *perr;
+        if (dt_hist) std::memcpy(dt_hist + 2 * (size_t)s0, pdt, sizeof(double) * 2 * (s1 - s0));
         if (err) {
             if (steps_done) *steps_done = s0 + err - 1;
             return HX_E_DT_COLLAPSE;
@@ -1956,8 +1961,9 @@ extern "C" int hx_boxes_dt(hx_ctx* c, int solver, int nbox, const hx_patch* boxe
         if (rc) return rc;
     }
     CflSlot sl;
-    HX_TRY(hipMemcpyAsync(&sl, c->slots + 3, sizeof(sl), hipMemcpyDeviceToHost, st));
+    HX_TRY(hipMemcpyAsync(c->hpin, c->slots + 3, sizeof(sl), hipMemcpyDeviceToHost, st));
     HX_TRY(hipStreamSynchronize(st));
+    std::memcpy(&sl, c->hpin, sizeof(sl));
     double vmax;
     std::memcpy(&vmax, &sl.vmax, sizeof(vmax));
     double est;
@@ -2236,9 +2242,12 @@ int amr_compute_dt(AmrRun& r, double cfl, int* collapsed)
         const int rc = hip_status(hipGetLastError());
         if (rc) return rc;
     }
+    // read back through the context's pinned staging (HX_AMR_MAX_LEVELS slots
+    // fit in it many times over)
     CflSlot s[HX_AMR_MAX_LEVELS];
-    HX_TRY(hipMemcpyAsync(s, r.slots, sizeof(CflSlot) * n, hipMemcpyDeviceToHost, r.st));
+    HX_TRY(hipMemcpyAsync(r.c->hpin, r.slots, sizeof(CflSlot) * n, hipMemcpyDeviceToHost, r.st));
     HX_TRY(hipStreamSynchronize(r.st));
+    std::memcpy(s, r.c->hpin, sizeof(CflSlot) * n);
     r.upload_pending = false;
     double tmp[HX_AMR_MAX_LEVELS];
     *collapsed = 0;
