@@ -327,20 +327,15 @@ class DenseLevel:
         neighbours' cells as ghosts and their shared faces carry the same
         fluxes), so fewer, larger patches give the same answer with fewer
         launches."""
-        from math import gcd
         from .regrid import _rects
-        new = [tuple(int(v) for v in b) for b in boxes]
+        new = [tuple(map(int, b)) for b in boxes]
         if new == self.boxes:
             return False
         self.boxes = new
         # the union painted on the host at the boxes' common alignment g (a
         # small array): expanded by g on the device into the mask in one copy,
         # and split into the patches
-        g = 0
-        for bx in self.boxes:
-            for v in bx:
-                g = gcd(g, v)
-        g = max(g, 1)
+        g = max(int(np.gcd.reduce(np.asarray(self.boxes, np.int64).ravel())), 1) if new else 1
         nyb, nxb = -(-self.ny // g), -(-self.nx // g)
         mb = np.zeros((nyb, nxb), np.uint8)
         for (i0, j0, w, h) in self.boxes:
@@ -500,7 +495,7 @@ class AmrHierarchy:
                 from . import regrid as RG
                 b = max(self.bf // 2, 1)
                 aligned = (lv.nx % b == 0 and lv.ny % b == 0 and self.nest <= b and
-                           all(v % b == 0 for bx in lv.boxes for v in bx))
+                           not (np.asarray(lv.boxes, np.int64) % b).any())
                 if aligned:
                     # tags grown, masked and blocked on the device; only the
                     # block array comes to the host

@@ -309,7 +309,7 @@ def grids_from_blocks(tb: np.ndarray, allowed: np.ndarray, b: int, ratio: int, g
             cap = n.value
             continue
         L.check(rc, "hx_cluster_blocks")
-        return [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(n.value)]
+        return [tuple(r) for r in out[:4 * n.value].reshape(-1, 4).tolist()]
 
 
 def covers(boxes: Sequence[Box], ny: int, nx: int) -> np.ndarray:
