@@ -104,6 +104,42 @@ def test_edge_fluxes_match_reevaluation_and_oracle(gpu, arith, model, shape, bc,
         set_arith("exact")
 
 
+@pytest.mark.parametrize("arith", ["exact", "fast"])
+@pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
+def test_edge_fluxes_fp32(gpu, arith, model):
+    """The single-precision build's variant: edge fluxes equal hx_face_fluxes'
+    FP32 re-evaluation, the state equals hx_advance_level's, and the fluxes
+    lie within 2e-6 of the FP64 oracle relative to the largest component
+    (measured 2.05e-7 to 2.19e-7 in both builds and solvers)."""
+    ny, nx = 61, 190
+    lo, hi = (1, 0), (0, 1)
+    dx, dy = 0.7, 0.5
+    st = state(np.random.default_rng(34), ny, nx, True)
+    dt = 0.4 * O.level_dt(*st[:4], dx, dy)[0]
+    set_arith(arith)
+    try:
+        print(f"edge fluxes FP32: {model} {arith}", flush=True)
+        lv = Level(*st, dx, dy, lo, hi, dtype=torch.float32)
+        ref = Level(*st, dx, dy, lo, hi, dtype=torch.float32)
+        ef = torch.full((2 * ny + 2 * nx, 3), np.nan, dtype=torch.float32, device="cuda")
+        sv = SolverVariant(model)
+        sv.compute_fluxes(lv, dt, edge_fluxes=ef)
+        sv.compute_fluxes(ref, dt)
+        again = reeval(model, lv.patch(lv.U_old), edge_faces(nx, ny), torch.float32)
+        torch.cuda.synchronize()
+        assert torch.equal(lv.U_new, ref.U_new)
+        got = ef.cpu().numpy()
+        assert np.isfinite(got).all()
+        assert np.array_equal(got, again.cpu().numpy())
+        W = A.working_state(*st, lo, hi)
+        want = np.asarray([A.face_flux(MODELS[model], W, f, dx, dy) for f in edge_faces(nx, ny)])
+        err = np.abs(got.astype(np.float64) - want).max() / np.abs(want).max()
+        print(f"  rel err vs FP64 oracle {err:.2e}", flush=True)
+        assert err <= 2e-6, err
+    finally:
+        set_arith("exact")
+
+
 @pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
 def test_edge_fluxes_with_stored_ghosts(gpu, model):
     """A patch whose four sides read caller-filled ghost layers (an AMR fine
