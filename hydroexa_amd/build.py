@@ -78,6 +78,31 @@ def build_io(verbose: bool = False) -> str:
     return out
 
 
+CXX_CHECK_SRC = os.path.join(ROOT, "tests", "cxx", "binding_check.cpp")
+CXX_CHECK = os.path.join(LIBDIR, "hx_binding_check")
+
+
+def build_cxx_check(verbose: bool = False) -> str:
+    """The C++ caller of the C-ABI (tests/cxx/binding_check.cpp), compiled with
+    the host compiler against include/hydroexa.h and the HIP runtime API and
+    linked to libhydroexa.so, as a HydroEXA solver translation unit would be."""
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
+    tmp = CXX_CHECK + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(rocm, 'include')}",
+           CXX_CHECK_SRC, "-o", tmp, f"-L{LIBDIR}", "-lhydroexa",
+           f"-L{os.path.join(rocm, 'lib')}", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{os.path.join(rocm, 'lib')}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("building the C++ binding check failed")
+    os.replace(tmp, CXX_CHECK)
+    return CXX_CHECK
+
+
 def build(verbose: bool = False) -> str:
     """Compile every HIP source (the three translation units in parallel) and
     link the in-tree shared library, then the HDF5 input library.  Always
@@ -121,6 +146,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("linking libhydroexa.so failed")
     os.replace(tmp, LIB)
     build_io(verbose)
+    build_cxx_check(verbose)
     return LIB
 
 
