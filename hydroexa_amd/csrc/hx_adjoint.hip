@@ -608,12 +608,7 @@ __global__ void clear_kernel(AdjScen* sc, int nscen)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nscen) return;
-    for (int k = 0; k < 3; ++k) {
-        sc[s].slots[k].amax = 0ull;
-        sc[s].slots[k].vmax = 0ull;
-        sc[s].slots[k].flags = 0u;
-        sc[s].slots[k].pad = 0u;
-    }
+    for (int k = 0; k < 3; ++k) slot_clear(&sc[s].slots[k]);
     *sc[s].err = 0;
 }
 

@@ -904,24 +904,107 @@ __device__ __forceinline__ unsigned long long dbits(double x)
     return (unsigned long long)__double_as_longlong(x);
 }
 
-// Fold one partial (max a, max velocity, flags) into a slot.  The slot only
-// grows while partials fold into it, so it is read at device scope first and
-// the atomics that cannot change it are skipped: same-address device-scope
-// atomics serialize, and after the first partials nearly all skip.
-__device__ __forceinline__ void slot_fold(CflSlot* slot, double a, double v, unsigned f)
+// Sub-slot of the CU that runs the calling wave: (XCC_ID bits 3:0) x 8 +
+// (HW_ID CU_ID bits 11:8) mod 8.  Each shader engine
+// numbers its CUs with 8 consecutive ids, so on 256 CUs every one of 64
+// sub-slots serves 4 CUs.  Read where it is used, it holds no SGPR over the
+// march, as blockIdx.x would.
+__device__ __forceinline__ unsigned slot_sub_here()
 {
-    const unsigned long long ca = __hip_atomic_load(&slot->amax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long cv = __hip_atomic_load(&slot->vmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned cf = __hip_atomic_load(&slot->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (dbits(a) > ca) atomicMax(&slot->amax, dbits(a));
-    if (dbits(v) > cv) atomicMax(&slot->vmax, dbits(v));
-    if ((cf | f) != cf) atomicOr(&slot->flags, f);
+    constexpr int kHwIdReg = 4, kCuIdOffset = 8, kCuIdBits = 4;
+    constexpr int kXccIdReg = 20, kXccIdBits = 4;
+    static_assert(kCflSubs == 64, "sub-slot index: 8 XCDs x 8 CU ids");
+    const unsigned cu = (unsigned)__builtin_amdgcn_s_getreg(((kCuIdBits - 1) << 11) |
+                                                            (kCuIdOffset << 6) | kHwIdReg);
+    const unsigned x = (unsigned)__builtin_amdgcn_s_getreg(((kXccIdBits - 1) << 11) | kXccIdReg);
+    return ((x & 7u) << 3) | (cu & 7u);
 }
 
-// Reduce a wavefront's accumulator and fold one block into the slot.  Every
-// thread of the block must call this.
+// A slot's value read by a whole wavefront (every lane active): lane l loads
+// a and the flags of sub-slot l with one 16-byte load (and the velocity when
+// want_v, a wave-uniform condition: it only feeds the history one wavefront
+// writes); four DPP steps fold each row of 16 lanes (quad_perm [1,0,3,2]:
+// lane ^ 1, quad_perm [2,3,0,1]: lane ^ 2, row_half_mirror: i <-> 7 - i,
+// row_mirror: i <-> 15 - i) and two lane exchanges the four rows; a and the
+// flags are made wave-uniform, the velocity is left in every lane.  With
+// spread false (wave-uniform) every lane loads sub-slot 0 and nothing is
+// folded.  Scalar loads of the sub-slots would hold SGPRs the advance
+// kernels have not got.
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp64(unsigned long long x)
+{
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)x, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(x >> 32), CTRL, 0xf, 0xf, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+template <int CTRL>
+__device__ __forceinline__ void fold_dpp(unsigned long long& x)
+{
+    const unsigned long long o = dpp64<CTRL>(x);
+    x = o > x ? o : x;
+}
+template <int CTRL>
+__device__ __forceinline__ void fold_dpp(unsigned& f)
+{
+    f |= (unsigned)__builtin_amdgcn_mov_dpp((int)f, CTRL, 0xf, 0xf, false);
+}
+template <typename T>
+__device__ __forceinline__ void fold_wave(T& x)
+{
+    fold_dpp<0xB1>(x);
+    fold_dpp<0x4E>(x);
+    fold_dpp<0x141>(x);
+    fold_dpp<0x140>(x);
+#pragma unroll
+    for (int off = 16; off < 64; off <<= 1) {
+        if constexpr (sizeof(T) == 8) {
+            const T o = __shfl_xor(x, off, 64);
+            x = o > x ? o : x;
+        } else {
+            x |= __shfl_xor(x, off, 64);
+        }
+    }
+}
+__device__ __forceinline__ CflValue slot_value_wave(const CflSlot* s, bool want_v, bool spread = true)
+{
+    const CflSub& q = s->sub[spread ? (threadIdx.x & (kCflSubs - 1)) : 0u];
+    unsigned long long a = q.amax;
+    unsigned f = q.flags;
+    unsigned long long v = want_v ? q.vmax : 0ull;
+    if (spread) {
+        if (want_v) fold_wave(v);
+        fold_wave(a);
+        fold_wave(f);
+    }
+    CflValue r;
+    r.amax = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(a >> 32)) << 32) |
+             (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)a);
+    r.vmax = v;
+    r.flags = (unsigned)__builtin_amdgcn_readfirstlane((int)f);
+    return r;
+}
+
+// Fold one partial (max a, max velocity, flags) into sub-slot k.  The
+// sub-slot only grows while partials fold into it, so it is read at device
+// scope first and the atomics that cannot change it are skipped:
+// same-address device-scope atomics serialize, and after the first partials
+// nearly all skip.
+__device__ __forceinline__ void slot_fold(CflSlot* slot, unsigned k, double a, double v, unsigned f)
+{
+    CflSub& q = slot->sub[k];
+    const unsigned long long ca = __hip_atomic_load(&q.amax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long cv = __hip_atomic_load(&q.vmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned cf = __hip_atomic_load(&q.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (dbits(a) > ca) atomicMax(&q.amax, dbits(a));
+    if (dbits(v) > cv) atomicMax(&q.vmax, dbits(v));
+    if ((cf | f) != cf) atomicOr(&q.flags, f);
+}
+
+// Reduce a wavefront's accumulator and fold one block into its sub-slot of
+// the slot (sub-slot 0 when not spread).  Every thread of the block must
+// call this.
 template <typename R>
-__device__ __forceinline__ void cfl_commit(CflAcc<R> acc, CflSlot* slot)
+__device__ __forceinline__ void cfl_commit(CflAcc<R> acc, CflSlot* slot, bool spread = true)
 {
     double a = (double)acc.amax, v = (double)acc.vmax;
     unsigned f = acc.flags;
@@ -944,7 +1027,7 @@ __device__ __forceinline__ void cfl_commit(CflAcc<R> acc, CflSlot* slot)
             v = (v < sv[q]) ? sv[q] : v;
             f |= sf[q];
         }
-        if (f) slot_fold(slot, a, v, f);
+        if (f) slot_fold(slot, spread ? slot_sub_here() : 0u, a, v, f);
     }
 }
 

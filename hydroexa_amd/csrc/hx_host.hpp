@@ -68,13 +68,21 @@ namespace host {
 // hx_set_tuning, or the measured default (one MI355X):
 // * 3 waves per SIMD for FP64, 1 requested for FP32;
 // * the strip height from the level size:
-//   - the FP64 hydrostatic march is arithmetic-bound, so a launch ends with
-//     a tail in which fewer wavefronts are left than the device holds
-//     (slots = CUs x 4 SIMDs x the kernel's waves per SIMD) and SIMDs idle.
-//     Its height is the r in [8, 32] that minimizes (rounds of slots) x
-//     (r + 4 prologue rows), the larger on ties: 31 rows at 4096^2, 24 at
-//     2048^2 (2-7 % faster than 16 on the lakes at rest and the cylinder dam
-//     break), 8 on levels that do not fill the device once;
+//   - the FP64 hydrostatic march is arithmetic-bound: a launch lasts as
+//     long as its busiest CU, whose SIMDs each run one wavefront of every
+//     4-wavefront workgroup the CU holds.  Workgroups go round-robin to the
+//     8 XCDs, so the busiest CU holds k = ceil(ceil(wgs / 8) / (CUs / 8)),
+//     run in rounds of at most the kernel's waves per SIMD.  A row costs a
+//     SIMD 0.78, 1.10 and 1.60 units with 1, 2 and 3 wavefronts resident
+//     (fitted to 512^2-2048^2 lakes at rest; 0.53 per wavefront beyond 3),
+//     and a strip of r rows marches r + 4 rows.  The height is the r in
+//     [8, 32] of least (sum over rounds of the round's cost) x (r + 4), the
+//     larger on ties: 8 rows at 512^2, 10 at 768^2 and 1024^2, 14 at
+//     1280^2, 20 at 1536^2, 24 at 2048^2, 31 at 4096^2.  Against the rule
+//     that weighed every round alike (8 rows below 1536^2): 0.0239 against
+//     0.0318 ms per step at 768^2, 0.0357 against 0.0435 at 1024^2 (12 rows
+//     measured 0.0345 there), 0.0419 against 0.0501 at 1280^2; 24 rows at
+//     2048^2 are 2-7 % faster than 16;
 //   - the other marches stream HBM: 16 rows when that gives at least 3
 //     wavefronts per SIMD (14, 18 and 20 rows were 1-3 % slower on the
 //     4096^2 dam break, 24-128 rows 5-30 %, and 16 rows stay best at 2048^2
@@ -85,6 +93,12 @@ inline int default_occ(int solver, int dtype)
     (void)solver;
     return (dtype == HX_F64) ? 3 : 1;
 }
+// Cost of one row to a SIMD holding m wavefronts of the FP64 hydrostatic
+// march, in hundredths (0 for none).
+inline int64_t round_units(int64_t m)
+{
+    return m <= 0 ? 0 : m == 1 ? 78 : m == 2 ? 110 : m == 3 ? 160 : 160 * m / 3;
+}
 inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd_waves)
 {
     if (nx <= 0 || ny <= 0 || cus <= 0) return 16;
@@ -93,15 +107,29 @@ inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd
     const int64_t nsx = (nx + cols - 1) / cols;
     if (!(hydro && dtype == HX_F64)) return (nsx * ((ny + 15) / 16) >= (int64_t)cus * 12) ? 16 : 6;
     if (simd_waves <= 0) return 16;
-    const int64_t slots = (int64_t)cus * 4 * simd_waves;
+    const int64_t cus_xcd = cus >= 8 ? cus / 8 : 1;
     int best = 32;
     int64_t cost = INT64_MAX;
     for (int r = 8; r <= 32; ++r) {
-        const int64_t waves = nsx * ((ny + r - 1) / r);
-        const int64_t k = ((waves + slots - 1) / slots) * (r + 4);
-        if (k <= cost) { cost = k; best = r; }
+        const int64_t wgs = (nsx * ((ny + r - 1) / r) + 3) / 4;
+        const int64_t k = ((wgs + 7) / 8 + cus_xcd - 1) / cus_xcd;
+        const int64_t units = (k / simd_waves) * round_units(simd_waves) + round_units(k % simd_waves);
+        const int64_t c = units * (r + 4);
+        if (c <= cost) { cost = c; best = r; }
     }
     return best;
+}
+// Whether the fused loop's steps fold their CFL partials into every
+// sub-slot of a slot (1) or into sub-slot 0 (0), the next step reading the
+// same.  Spreading takes the queue of same-line atomics away from the
+// workgroups that finish together (levels up to 2048^2: 5-45 % faster per
+// step); on levels that fill the device twice the partials arrive spread
+// over the launch, and reading 64 lines in every wavefront cost more than
+// the queue (4096^2: 221.4 against 219.7 us per step FP64, 124.7 against
+// 122.8 FP32).  Below 2^23 cells: spread.
+inline int cfl_spread_for(int nx, int ny)
+{
+    return ((int64_t)nx * (int64_t)ny < ((int64_t)1 << 23)) ? 1 : 0;
 }
 inline int eff_occ(const hx_ctx* c, int solver, int dtype)
 {

@@ -36,7 +36,7 @@ constexpr int kCflUnroll = 4;
 
 template <typename R>
 __device__ __forceinline__ void cfl_block(const PatchView<R>& p, R dx, R dy, R rdx, R rdy, int mul,
-                                          CflSlot* slot, int bx, int by)
+                                          CflSlot* slot, int bx, int by, bool spread = true)
 {
     CflAcc<R> acc;
     acc.init();
@@ -61,21 +61,20 @@ __device__ __forceinline__ void cfl_block(const PatchView<R>& p, R dx, R dy, R r
             acc.add(p.h[s], p.hu[s], p.hv[s], p.z[s], dx, dy, rdx, rdy, mul != 0);
         }
     }
-    cfl_commit(acc, slot);
+    cfl_commit(acc, slot, spread);
 }
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void cfl_kernel(PatchView<R> p, R dx, R dy, R rdx, R rdy,
-                                                     int mul, CflSlot* slot)
+                                                     int mul, CflSlot* slot, int spread)
 {
-    cfl_block(p, dx, dy, rdx, rdy, mul, slot, blockIdx.x, blockIdx.y);
+    cfl_block(p, dx, dy, rdx, rdy, mul, slot, blockIdx.x, blockIdx.y, spread != 0);
 }
 
 __global__ void clear_slots_kernel(CflSlot* s, int n, int* err)
 {
-    const int t = threadIdx.x;
-    if (t < n) { s[t].amax = 0ull; s[t].vmax = 0ull; s[t].flags = 0u; s[t].pad = 0u; }
-    if (t == 0 && err) *err = 0;
+    for (int q = threadIdx.x; q < n * kCflSubs; q += blockDim.x) sub_clear(&s[q / kCflSubs], q % kCflSubs);
+    if (threadIdx.x == 0 && err) *err = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -230,7 +229,7 @@ __global__ void ens_reset_kernel(CflSlot* slots, int nslots, int* err, int nscen
                                  unsigned* queue, int* step_base)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nslots) { slots[q].amax = 0ull; slots[q].vmax = 0ull; slots[q].flags = 0u; slots[q].pad = 0u; }
+    if (q < nslots) slot_clear(slots + q);
     if (q < nscen) err[q] = 0;
     if (q < 3) queue[q] = 0u;
     if (q == 0) *step_base = 0;
@@ -627,11 +626,13 @@ struct RunScratch {
 
 // The allocation-free loop's scratch (hx_run_level_noalloc): one copy per
 // device, held until the process ends.  Layout: 3 CflSlots at byte 0, the
-// error flag at byte 256, the history of kNoAllocSteps steps at byte 512.
-// Host code never uses this name as an address: hx_ctx_create resolves the
-// device copy's address with hipGetSymbolAddress once per device.
+// error flag at byte kScratchErr, the history of kNoAllocSteps steps at byte
+// kScratchHist.  Host code never uses this name as an address: hx_ctx_create
+// resolves the device copy's address with hipGetSymbolAddress once per device.
 constexpr int kNoAllocSteps = 8192;
-constexpr size_t kScratchBytes = 512 + sizeof(double) * 2 * kNoAllocSteps;
+constexpr size_t kScratchErr = 3 * sizeof(CflSlot);
+constexpr size_t kScratchHist = kScratchErr + 256;
+constexpr size_t kScratchBytes = kScratchHist + sizeof(double) * 2 * kNoAllocSteps;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 constexpr int kMaxDevices = 64;
 static void* g_scratch[kMaxDevices];       // device address of scratch, per device
@@ -640,9 +641,8 @@ static std::mutex g_scratch_mu;
 static RunScratch scratch_run(void* base)
 {
     char* b = (char*)base;
-    return RunScratch{(CflSlot*)b, (int*)(b + 256), (double*)(b + 512), kNoAllocSteps};
+    return RunScratch{(CflSlot*)b, (int*)(b + kScratchErr), (double*)(b + kScratchHist), kNoAllocSteps};
 }
-static_assert(3 * sizeof(CflSlot) <= 256, "scratch layout");
 
 // Launch-shape defaults from the strip-height / occupancy sweep on MI355X
 // (4096^2 plain Roe and 2048^2 hydrostatic workloads): short strips keep
@@ -679,6 +679,8 @@ extern "C" int hx_ctx_create(hx_ctx** out, int device)
     // the larger chunk's dt and vel histories (pageable copies took 20-30 us
     // each after a run's last step)
     const size_t pin = sizeof(double) * (1 + 2 * (size_t)std::max(kHistCap, kNoAllocSteps));
+    static_assert(sizeof(double) * (1 + 2 * (size_t)kHistCap) >= sizeof(CflSlot) * HX_AMR_MAX_LEVELS,
+                  "the AMR runner reads its level slots back through the pinned staging");
     if (hipMalloc(&c->slots, 4 * sizeof(CflSlot)) != hipSuccess ||
         hipMalloc(&c->err, sizeof(int)) != hipSuccess ||
         hipMalloc(&c->hist, sizeof(double) * 2 * kHistCap) != hipSuccess ||
@@ -849,13 +851,14 @@ extern "C" int hx_advance_level_fluxes(hx_ctx* c, int solver, const hx_patch* in
 }
 
 template <typename R>
-static int launch_cfl(const hx_patch* in, CflSlot* slot, hipStream_t st)
+static int launch_cfl(const hx_patch* in, CflSlot* slot, hipStream_t st, int spread = 1)
 {
     const int mul = is_pow2(in->dx) && is_pow2(in->dy) ? 1 : 0;
     const dim3 grid((unsigned)((in->nx + kBlock - 1) / kBlock),
                     (unsigned)((in->ny + kCflRows - 1) / kCflRows));
     hipLaunchKernelGGL(cfl_kernel<R>, grid, dim3(kBlock), 0, st, view_of<R>(in),
-                       (R)in->dx, (R)in->dy, (R)(1.0 / in->dx), (R)(1.0 / in->dy), mul, slot);
+                       (R)in->dx, (R)in->dy, (R)(1.0 / in->dx), (R)(1.0 / in->dy), mul, slot,
+                       spread);
     return hip_status(hipGetLastError());
 }
 
@@ -900,10 +903,11 @@ extern "C" int hx_level_dt(hx_ctx* c, int solver, const hx_patch* in, double cfl
     rc = (in->dtype == HX_F64) ? launch_cfl<double>(in, c->slots + 3, st)
                                : launch_cfl<float>(in, c->slots + 3, st);
     if (rc) return rc;
-    CflSlot s;
-    HX_TRY(hipMemcpyAsync(c->hpin, c->slots + 3, sizeof(s), hipMemcpyDeviceToHost, st));
+    CflSlot sl;
+    HX_TRY(hipMemcpyAsync(c->hpin, c->slots + 3, sizeof(sl), hipMemcpyDeviceToHost, st));
     HX_TRY(hipStreamSynchronize(st));
-    std::memcpy(&s, c->hpin, sizeof(s));
+    std::memcpy(&sl, c->hpin, sizeof(sl));
+    const CflValue s = slot_value(&sl);
     double vmax;
     std::memcpy(&vmax, &s.vmax, sizeof(vmax));
     double est;
@@ -1108,9 +1112,13 @@ static int run_t(hx_ctx* c, int solver, const hx_patch* in, void* ah, void* ahu,
     a.dt_hist = sc.hist;
     a.vel_hist = sc.hist + sc.cap;
     fill_geometry(a, solver, eff_rows(c, solver, in->dtype, in->nx, in->ny), in->dx, in->dy);
+    // every step reads the slot the step before reduced into: both sides use
+    // the level's rule (the initial reduction too, and the steps of a resumed
+    // run, which take the same level)
+    a.cfl_spread = cfl_spread_for(in->nx, in->ny);
     if (s0 == 0 && fresh) {
         hipLaunchKernelGGL(clear_slots_kernel, dim3(1), dim3(64), 0, st, sc.slots, 3, sc.err);
-        const int rc = launch_cfl<R>(in, sc.slots + base, st);
+        const int rc = launch_cfl<R>(in, sc.slots + base, st, a.cfl_spread);
         if (rc) return rc;
     }
     R* bufA[3] = {(R*)in->h, (R*)in->hu, (R*)in->hv};
@@ -1667,9 +1675,16 @@ template <typename R>
 __global__ void tl_dt_kernel(const CflSlot* rc, const CflSlot* rf, CflSlot* wc, CflSlot* wf,
                              double cfl, int subcycle, int step, double* dt, double* hist, int* err)
 {
-    if (threadIdx.x != 0 || *err) return;
-    const R ec = slot_dt_est<R>(rc->amax, rc->flags);
-    const R ef = slot_dt_est<R>(rf->amax, rf->flags);
+    if (*err) return;
+    // the slots this step reduces into, one sub-slot a thread
+    for (int q = threadIdx.x; q < kCflSubs; q += blockDim.x) {
+        sub_clear(wc, q);
+        sub_clear(wf, q);
+    }
+    const CflValue vc = slot_value_wave(rc, false), vf = slot_value_wave(rf, false);   // one wavefront
+    if (threadIdx.x != 0) return;
+    const R ec = slot_dt_est<R>(vc);
+    const R ef = slot_dt_est<R>(vf);
     if ((double)ec < 1.e-12 || (double)ef < 1.e-12) {
         *err = step + 1;
         return;
@@ -1688,8 +1703,6 @@ __global__ void tl_dt_kernel(const CflSlot* rc, const CflSlot* rf, CflSlot* wc, 
     dt[1] = b;
     hist[2 * step] = a;
     hist[2 * step + 1] = b;
-    wc->amax = 0ull; wc->vmax = 0ull; wc->flags = 0u;
-    wf->amax = 0ull; wf->vmax = 0ull; wf->flags = 0u;
 }
 
 // Coupled steps [s0, s1) (history and error indices relative to s0; the
@@ -1960,18 +1973,19 @@ extern "C" int hx_boxes_dt(hx_ctx* c, int solver, int nbox, const hx_patch* boxe
                                                   : launch_cfl<float>(&boxes[b], c->slots + 3, st);
         if (rc) return rc;
     }
-    CflSlot sl;
-    HX_TRY(hipMemcpyAsync(c->hpin, c->slots + 3, sizeof(sl), hipMemcpyDeviceToHost, st));
+    CflSlot slot;
+    HX_TRY(hipMemcpyAsync(c->hpin, c->slots + 3, sizeof(slot), hipMemcpyDeviceToHost, st));
     HX_TRY(hipStreamSynchronize(st));
-    std::memcpy(&sl, c->hpin, sizeof(sl));
+    std::memcpy(&slot, c->hpin, sizeof(slot));
+    const CflValue sl = slot_value(&slot);
     double vmax;
     std::memcpy(&vmax, &sl.vmax, sizeof(vmax));
     double est;
     if (boxes[0].dtype == HX_F64) {
-        est = slot_dt_est<double>(sl.amax, sl.flags);
+        est = slot_dt_est<double>(sl);
         *dt_out = est * cfl;
     } else {
-        const float e = slot_dt_est<float>(sl.amax, sl.flags);
+        const float e = slot_dt_est<float>(sl);
         est = e;
         *dt_out = (double)(e * (float)cfl);
     }
@@ -2244,10 +2258,10 @@ int amr_compute_dt(AmrRun& r, double cfl, int* collapsed)
     }
     // read back through the context's pinned staging (HX_AMR_MAX_LEVELS slots
     // fit in it many times over)
-    CflSlot s[HX_AMR_MAX_LEVELS];
+    CflValue s[HX_AMR_MAX_LEVELS];
     HX_TRY(hipMemcpyAsync(r.c->hpin, r.slots, sizeof(CflSlot) * n, hipMemcpyDeviceToHost, r.st));
     HX_TRY(hipStreamSynchronize(r.st));
-    std::memcpy(s, r.c->hpin, sizeof(CflSlot) * n);
+    for (int l = 0; l < n; ++l) s[l] = slot_value((const CflSlot*)r.c->hpin + l);
     r.upload_pending = false;
     double tmp[HX_AMR_MAX_LEVELS];
     *collapsed = 0;

@@ -33,17 +33,70 @@ struct PatchView {
 
 enum : unsigned { CFL_WET = 1u, CFL_DRY = 2u, CFL_NODATA = 4u };
 
-struct CflSlot {
+// One CFL reduction, spread over kCflSubs partial slots, each on its own
+// 128-byte line (device-scope atomics on one line serialize, whatever their
+// addresses): a workgroup folds its partial into the sub-slot of the CU it
+// runs on (slot_sub_here).  With a single slot, the device-scope atomics of
+// workgroups that finish together all queue on one line (about 30 ns each);
+// at 384^2 to 2048^2 that queue lengthened a launch by 4-10 us.  a and the
+// flags sit in one 16-byte word, which a reader loads at once
+// (slot_value_wave, one sub-slot a lane); the host folds with slot_value.
+// A launch whose own next step reads the slot may fold into sub-slot 0 only
+// (StepArgs::cfl_spread = 0): its readers then load one line instead of 64.
+constexpr int kCflSubs = 64;
+struct alignas(128) CflSub {
     unsigned long long amax;   // bits of a non-negative double
-    unsigned long long vmax;   // bits of a non-negative double
     unsigned int flags;
     unsigned int pad;
+    unsigned long long vmax;   // bits of a non-negative double
+    unsigned char line[104];
 };
+struct CflSlot {
+    CflSub sub[kCflSubs];
+};
+static_assert(sizeof(CflSub) == 128 && sizeof(CflSlot) == 128 * kCflSubs, "CFL slot layout");
+
+// A slot's reduced value: the maximum of a and of the velocity (as the bits
+// of non-negative doubles, which order as the values do) and the OR of the
+// flags over its sub-slots.
+struct CflValue {
+    unsigned long long amax, vmax;
+    unsigned flags;
+};
+__host__ __device__ inline CflValue slot_value(const CflSlot* s)
+{
+    CflValue r{0ull, 0ull, 0u};
+    for (int k = 0; k < kCflSubs; ++k) {
+        const CflSub& q = s->sub[k];
+        r.amax = q.amax > r.amax ? q.amax : r.amax;
+        r.vmax = q.vmax > r.vmax ? q.vmax : r.vmax;
+        r.flags |= q.flags;
+    }
+    return r;
+}
+__host__ __device__ inline void sub_clear(CflSlot* s, int k)
+{
+    s->sub[k].amax = 0ull;
+    s->sub[k].flags = 0u;
+    s->sub[k].pad = 0u;
+    s->sub[k].vmax = 0ull;
+}
+__host__ __device__ inline void slot_clear(CflSlot* s)
+{
+    for (int k = 0; k < kCflSubs; ++k) sub_clear(s, k);
+}
 
 // dt estimate (before cfl) from a reduced slot: fl(1/max a) equals the
 // reference's min over cells of fl(1/a) because correctly rounded division
 // is monotone; dry cells contribute 1e6 and NODATA cells 1e30
 // (src/solvers/Roe.cpp:42-50).
+template <typename R>
+__host__ __device__ inline R slot_dt_est(unsigned long long abits, unsigned flags);
+template <typename R>
+__host__ __device__ inline R slot_dt_est(const CflValue& v)
+{
+    return slot_dt_est<R>(v.amax, v.flags);
+}
 template <typename R>
 __host__ __device__ inline R slot_dt_est(unsigned long long abits, unsigned flags)
 {
@@ -99,6 +152,9 @@ struct StepArgs {
     // (j = ny); plain Roe D- on the low faces and D+ on the high faces,
     // hydrostatic 0.5 FL + 0.5 FR
     R* ef = nullptr;
+    // 1: the CFL partials fold into the sub-slot of the CU (acc) and every
+    // sub-slot of rd is read; 0: sub-slot 0 only, both ways
+    int cfl_spread = 1;
 };
 
 // The strip of the banded tiling that item i (workgroup index within its

@@ -214,7 +214,7 @@ int hx_run_level(hx_ctx *ctx, int solver, const hx_patch *state, void *alt_h,
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc);
  * it has no counterpart in the reference.  Same arguments, kernels and
  * results as hx_run_level, but the CFL slots, error flag and dt history live
- * in the library's static per-device scratch (about 130 KiB, resolved when
+ * in the library's static per-device scratch (about 152 KiB, resolved when
  * the first context on the device is created) instead of the context's
  * buffers, and nothing is allocated or freed.  At most 8192 steps per call
  * (HX_E_BADARG beyond).  Every call on one GPU shares that scratch, which

@@ -59,17 +59,32 @@ def test_small_levels_take_6_row_strips(rule):
     assert rule(HYD, F32, 1024, 1024) == 6
 
 
-def test_hydrostatic_fp64_height_minimises_rounds_times_rows(rule):
-    # 3 resident wavefronts per SIMD, 256 CUs: 3072 slots
-    assert rule(HYD, F64, 2048, 2048, 256, 3) == 24     # 3010 wavefronts: one round
-    assert rule(HYD, F64, 4096, 4096, 256, 3) == 31     # 9177 wavefronts: three rounds
-    assert rule(HYD, F64, 1024, 1024, 256, 3) == 8      # below one round: the shortest
+def busiest_cu_cost(nx, ny, r, cus, waves):
+    """The stated cost of an r-row hydrostatic FP64 launch: workgroups of 4
+    strips dealt round-robin to 8 XCDs, the busiest CU's workgroups run in
+    rounds of `waves`, a row costs 0.78/1.10/1.60 with 1/2/3 resident."""
+    strips = -(-nx // 60) * -(-ny // r)
+    per_xcd = -(-(-(-strips // 4)) // 8)
+    k = -(-per_xcd // max(1, cus // 8))
+    unit = {0: 0, 1: 78, 2: 110, 3: 160}
+    rounds = [waves] * (k // waves) + [k % waves]
+    return sum(unit[m] if m <= 3 else 160 * m // 3 for m in rounds) * (r + 4)
+
+
+def test_hydrostatic_fp64_height_minimises_the_busiest_cu(rule):
+    # 256 CUs, 3 resident wavefronts per SIMD: the heights measured best
+    assert rule(HYD, F64, 512, 512, 256, 3) == 8        # 144 workgroups: one per CU
+    assert rule(HYD, F64, 768, 768, 256, 3) == 10       # 251 workgroups: one per CU
+    assert rule(HYD, F64, 1024, 1024, 256, 3) == 10     # two per CU beat one 19-row
+    assert rule(HYD, F64, 1280, 1280, 256, 3) == 14
+    assert rule(HYD, F64, 1536, 1536, 256, 3) == 20
+    assert rule(HYD, F64, 2048, 2048, 256, 3) == 24     # 753 workgroups: one round of 3
+    assert rule(HYD, F64, 4096, 4096, 256, 3) == 31     # three rounds of 3
     # brute force of the stated cost over [8, 32], larger height on ties
-    for (nx, ny) in ((512, 512), (1000, 3000), (4096, 1024), (2048, 2048)):
-        nsx = -(-nx // 60)
-        slots = 256 * 4 * 3
-        best = min(range(8, 33), key=lambda r: (-(-nsx * -(-ny // r) // slots) * (r + 4), -r))
-        assert rule(HYD, F64, nx, ny, 256, 3) == best
+    for (nx, ny) in ((512, 512), (1000, 3000), (4096, 1024), (2048, 2048), (3000, 700)):
+        for cus, waves in ((256, 3), (256, 1), (256, 2), (304, 4), (80, 3)):
+            best = min(range(8, 33), key=lambda r: (busiest_cu_cost(nx, ny, r, cus, waves), -r))
+            assert rule(HYD, F64, nx, ny, cus, waves) == best, (nx, ny, cus, waves)
 
 
 def test_unknown_sizes_fall_back_to_16(rule):
