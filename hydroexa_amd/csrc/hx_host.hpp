@@ -66,55 +66,80 @@ namespace host {
 
 // Launch shape of the fused advance for (solver, dtype): the value set by
 // hx_set_tuning, or the measured default (one MI355X):
-// * 3 waves per SIMD for FP64, 1 requested for FP32;
+// * 3 waves per SIMD for FP64, 1 requested for FP32 (the FP32 kernels hold
+//   6 (plain Roe) and 5 (hydrostatic) by their register counts);
 // * the strip height from the level size:
-//   - the FP64 hydrostatic march is arithmetic-bound: a launch lasts as
-//     long as its busiest CU, whose SIMDs each run one wavefront of every
-//     4-wavefront workgroup the CU holds.  Workgroups go round-robin to the
-//     8 XCDs, so the busiest CU holds k = ceil(ceil(wgs / 8) / (CUs / 8)),
-//     run in rounds of at most the kernel's waves per SIMD.  A row costs a
-//     SIMD 0.78, 1.10 and 1.60 units with 1, 2 and 3 wavefronts resident
-//     (fitted to 512^2-2048^2 lakes at rest; 0.53 per wavefront beyond 3),
-//     and a strip of r rows marches r + 4 rows.  The height is the r in
-//     [8, 32] of least (sum over rounds of the round's cost) x (r + 4), the
-//     larger on ties: 8 rows at 512^2, 10 at 768^2 and 1024^2, 14 at
-//     1280^2, 20 at 1536^2, 24 at 2048^2, 31 at 4096^2.  Against the rule
-//     that weighed every round alike (8 rows below 1536^2): 0.0239 against
-//     0.0318 ms per step at 768^2, 0.0357 against 0.0435 at 1024^2 (12 rows
-//     measured 0.0345 there), 0.0419 against 0.0501 at 1280^2; 24 rows at
-//     2048^2 are 2-7 % faster than 16;
-//   - the other marches stream HBM: 16 rows when that gives at least 3
-//     wavefronts per SIMD (14, 18 and 20 rows were 1-3 % slower on the
-//     4096^2 dam break, 24-128 rows 5-30 %, and 16 rows stay best at 2048^2
-//     in FP64 and FP32), else 6 rows, so a small level gets more, shorter
-//     strips (256^2 to 1024^2 dam breaks: 1.1-1.9x faster than 16 rows).
+//   - levels that fill the device twice at 16-row strips (the banded tiling
+//     marches them, plan_bands): plain Roe and FP32 stream HBM at 16 rows
+//     (14-20 rows within 1-3 % on the 4096^2 dam break, 24-128 rows 5-30 %
+//     slower; 2560^2 and 3072^2 FP64 best at 14-16); the FP64 hydrostatic
+//     march is arithmetic-bound and ends on a tail of idle SIMDs, so its
+//     height is the r in [8, 32] of least (sum over rounds of resident
+//     wavefronts of the round's per-row cost, round_units) x (r + 4): 31 rows
+//     at 4096^2 (5 % faster than 16);
+//   - smaller levels: a launch lasts as long as its busiest CU.  Workgroups
+//     are dealt round-robin to the 8 XCDs, so that CU holds
+//     k = ceil(ceil(workgroups / 8) / (CUs / 8)) of them; a strip of r rows
+//     marches r + 2 (plain Roe) or r + 4 (hydrostatic) rows at a per-row cost
+//     row_cost(k) fitted to heights 4-32 on 384^2 to 3072^2 levels of each
+//     march.  The height is the r in [4, 32] ([8, 32] hydrostatic FP64) of
+//     least (r + prologue) x row_cost(k), the larger on ties.  On those
+//     levels its choice measured 0.1-0.6 % above the best height on average
+//     (at most 5 %), where the rule it replaces (6 rows below a fill of
+//     3 wavefronts per SIMD, else 16) was 2-11 % above (at most 22 %).
 inline int default_occ(int solver, int dtype)
 {
     (void)solver;
     return (dtype == HX_F64) ? 3 : 1;
 }
 // Cost of one row to a SIMD holding m wavefronts of the FP64 hydrostatic
-// march, in hundredths (0 for none).
+// march, in hundredths (0 for none), on levels that fill the device twice.
 inline int64_t round_units(int64_t m)
 {
     return m <= 0 ? 0 : m == 1 ? 78 : m == 2 ? 110 : m == 3 ? 160 : 160 * m / 3;
 }
+// Per-row cost of a strip to the busiest CU when it holds k workgroups of a
+// march, in hundredths of a microsecond, k = 1..8 (least squares over the
+// measured step times, made non-decreasing); beyond 8 the cost grows by the
+// mean step of k = 4..8.
+constexpr int kRowCost[2][2][8] = {
+    {{81, 121, 169, 244, 303, 365, 427, 499},      // plain Roe, FP64
+     {73, 84, 108, 135, 166, 196, 234, 260}},      // plain Roe, FP32
+    {{164, 212, 270, 352, 418, 472, 558, 613},     // hydrostatic, FP64
+     {139, 162, 204, 249, 293, 341, 382, 422}}};   // hydrostatic, FP32
+inline int64_t row_cost(int solver, int dtype, int64_t k)
+{
+    const int* g = kRowCost[solver == HX_SOLVER_HYDROSTATIC_ROE ? 1 : 0][dtype == HX_F64 ? 0 : 1];
+    if (k <= 8) return g[k < 1 ? 0 : k - 1];
+    return g[7] + (k - 8) * (g[7] - g[3]) / 4;
+}
 inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd_waves)
 {
-    if (nx <= 0 || ny <= 0 || cus <= 0) return 16;
+    if (nx <= 0 || ny <= 0 || cus <= 0 || simd_waves <= 0) return 16;
     const bool hydro = (solver == HX_SOLVER_HYDROSTATIC_ROE);
+    const bool hydro64 = hydro && dtype == HX_F64;
     const int cols = hydro ? kHydroCols : kRoeCols;
+    const int prologue = hydro ? 4 : 2;
     const int64_t nsx = (nx + cols - 1) / cols;
-    if (!(hydro && dtype == HX_F64)) return (nsx * ((ny + 15) / 16) >= (int64_t)cus * 12) ? 16 : 6;
-    if (simd_waves <= 0) return 16;
     const int64_t cus_xcd = cus >= 8 ? cus / 8 : 1;
+    const int64_t slots_xcd = cus_xcd * 4 * simd_waves;
     int best = 32;
     int64_t cost = INT64_MAX;
-    for (int r = 8; r <= 32; ++r) {
+    if (nsx * ((ny + 15) / 16) >= 2 * 8 * slots_xcd) {
+        if (!hydro64) return 16;
+        for (int r = 8; r <= 32; ++r) {
+            const int64_t wgs = (nsx * ((ny + r - 1) / r) + 3) / 4;
+            const int64_t k = ((wgs + 7) / 8 + cus_xcd - 1) / cus_xcd;
+            const int64_t units = (k / simd_waves) * round_units(simd_waves) + round_units(k % simd_waves);
+            const int64_t c = units * (r + 4);
+            if (c <= cost) { cost = c; best = r; }
+        }
+        return best;
+    }
+    for (int r = hydro64 ? 8 : 4; r <= 32; ++r) {
         const int64_t wgs = (nsx * ((ny + r - 1) / r) + 3) / 4;
         const int64_t k = ((wgs + 7) / 8 + cus_xcd - 1) / cus_xcd;
-        const int64_t units = (k / simd_waves) * round_units(simd_waves) + round_units(k % simd_waves);
-        const int64_t c = units * (r + 4);
+        const int64_t c = (r + prologue) * row_cost(solver, dtype, k);
         if (c <= cost) { cost = c; best = r; }
     }
     return best;
@@ -144,8 +169,7 @@ inline int eff_rows(const hx_ctx* c, int solver, int dtype, int nx, int ny)
     const int r = c->rows[solver][dtype];
     if (r) return r;
     if (nx <= 0 || ny <= 0) return 16;
-    const bool query = (solver == HX_SOLVER_HYDROSTATIC_ROE && dtype == HX_F64);
-    return default_rows(solver, dtype, nx, ny, c->cus, query ? advance_simd_waves(c, solver, dtype) : 0);
+    return default_rows(solver, dtype, nx, ny, c->cus, advance_simd_waves(c, solver, dtype));
 }
 
 }  // namespace host

@@ -1,6 +1,7 @@
 """Host logic of the advance's default strip height (hx::host::default_rows in
 hydroexa_amd/csrc/hx_host.hpp), compiled for the host with hipcc and run on
-the CPU: the rule's values at the sizes its measurements cover."""
+the CPU: the rule's values at the sizes its measurements cover, and the
+stated cost restated here for other shapes, CU counts and occupancies."""
 import os
 import shutil
 import subprocess
@@ -44,52 +45,93 @@ def rule(tmp_path_factory):
     return run
 
 
-def test_streaming_marches_keep_16_rows_when_the_level_fills_the_device(rule):
-    # 16-row strips give at least 3 wavefronts per SIMD on 256 CUs
-    assert rule(ROE, F64, 4096, 4096) == 16
-    assert rule(ROE, F64, 2048, 2048) == 16
-    assert rule(ROE, F32, 2048, 2048) == 16
-    assert rule(HYD, F32, 4096, 4096) == 16
+# waves per SIMD the advance kernels hold on MI355X (their register counts)
+WAVES = {(ROE, F64): 3, (ROE, F32): 6, (HYD, F64): 3, (HYD, F32): 5}
+# per-row cost to the busiest CU holding k = 1..8 workgroups, hundredths
+ROW_COST = {(ROE, F64): (81, 121, 169, 244, 303, 365, 427, 499),
+            (ROE, F32): (73, 84, 108, 135, 166, 196, 234, 260),
+            (HYD, F64): (164, 212, 270, 352, 418, 472, 558, 613),
+            (HYD, F32): (139, 162, 204, 249, 293, 341, 382, 422)}
 
 
-def test_small_levels_take_6_row_strips(rule):
-    for n in (256, 512, 1024):
-        assert rule(ROE, F64, n, n) == 6
-        assert rule(ROE, F32, n, n) == 6
-    assert rule(HYD, F32, 1024, 1024) == 6
+def busiest_cu(nx, ny, r, cols, cus):
+    """Workgroups (4 strips each, dealt round-robin to 8 XCDs) the busiest CU
+    holds at strip height r."""
+    wgs = -(-(-(-nx // cols) * -(-ny // r)) // 4)
+    return -(-(-(-wgs // 8)) // max(1, cus // 8))
 
 
-def busiest_cu_cost(nx, ny, r, cus, waves):
-    """The stated cost of an r-row hydrostatic FP64 launch: workgroups of 4
-    strips dealt round-robin to 8 XCDs, the busiest CU's workgroups run in
-    rounds of `waves`, a row costs 0.78/1.10/1.60 with 1/2/3 resident."""
-    strips = -(-nx // 60) * -(-ny // r)
-    per_xcd = -(-(-(-strips // 4)) // 8)
-    k = -(-per_xcd // max(1, cus // 8))
+def banded_at_16(solver, dtype, nx, ny, cus, waves):
+    cols = 60 if solver == HYD else 62
+    return -(-nx // cols) * -(-ny // 16) >= 2 * 8 * max(1, cus // 8) * 4 * waves
+
+
+def small_level_cost(solver, dtype, nx, ny, r, cus):
+    """The stated cost of an r-row launch on a level below the banded
+    tiling: (r + prologue rows) x the busiest CU's per-row cost."""
+    cols, prologue = (60, 4) if solver == HYD else (62, 2)
+    g = ROW_COST[(solver, dtype)]
+    k = busiest_cu(nx, ny, r, cols, cus)
+    cost = g[k - 1] if k <= 8 else g[7] + (k - 8) * (g[7] - g[3]) // 4
+    return (r + prologue) * cost
+
+
+def rounds_cost(nx, ny, r, cus, waves):
+    """The stated cost of an r-row hydrostatic FP64 launch on a banded level:
+    the busiest CU's workgroups run in rounds of `waves`, a row costs
+    0.78/1.10/1.60 with 1/2/3 resident."""
+    k = busiest_cu(nx, ny, r, 60, cus)
     unit = {0: 0, 1: 78, 2: 110, 3: 160}
     rounds = [waves] * (k // waves) + [k % waves]
     return sum(unit[m] if m <= 3 else 160 * m // 3 for m in rounds) * (r + 4)
 
 
-def test_hydrostatic_fp64_height_minimises_the_busiest_cu(rule):
-    # 256 CUs, 3 resident wavefronts per SIMD: the heights measured best
-    assert rule(HYD, F64, 512, 512, 256, 3) == 8        # 144 workgroups: one per CU
-    assert rule(HYD, F64, 768, 768, 256, 3) == 10       # 251 workgroups: one per CU
-    assert rule(HYD, F64, 1024, 1024, 256, 3) == 10     # two per CU beat one 19-row
-    assert rule(HYD, F64, 1280, 1280, 256, 3) == 14
-    assert rule(HYD, F64, 1536, 1536, 256, 3) == 20
-    assert rule(HYD, F64, 2048, 2048, 256, 3) == 24     # 753 workgroups: one round of 3
+def test_levels_that_fill_the_device_twice_keep_16_rows(rule):
+    assert rule(ROE, F64, 4096, 4096, 256, 3) == 16     # the banded headline launch
+    assert rule(ROE, F64, 3072, 3072, 256, 3) == 16
+    assert rule(ROE, F32, 4096, 4096, 256, 6) == 16
+    assert rule(HYD, F32, 4096, 4096, 256, 5) == 16
+
+
+def test_hydrostatic_fp64_on_banded_levels_weighs_rounds_of_resident_wavefronts(rule):
     assert rule(HYD, F64, 4096, 4096, 256, 3) == 31     # three rounds of 3
-    # brute force of the stated cost over [8, 32], larger height on ties
-    for (nx, ny) in ((512, 512), (1000, 3000), (4096, 1024), (2048, 2048), (3000, 700)):
-        for cus, waves in ((256, 3), (256, 1), (256, 2), (304, 4), (80, 3)):
-            best = min(range(8, 33), key=lambda r: (busiest_cu_cost(nx, ny, r, cus, waves), -r))
+    for (nx, ny) in ((4096, 4096), (3072, 3072), (8192, 2048), (6000, 4100)):
+        for cus, waves in ((256, 3), (256, 2), (304, 3)):
+            assert banded_at_16(HYD, F64, nx, ny, cus, waves)
+            best = min(range(8, 33), key=lambda r: (rounds_cost(nx, ny, r, cus, waves), -r))
             assert rule(HYD, F64, nx, ny, cus, waves) == best, (nx, ny, cus, waves)
 
 
+def test_smaller_levels_take_the_least_modelled_busiest_cu_time(rule):
+    # heights timed as the defaults on one MI355X (at or below the best of
+    # the swept heights 4-32): the smallest height of a band of equal
+    # workgroups per CU
+    assert rule(ROE, F64, 384, 384, 256, 3) == 4        # one workgroup per CU
+    assert rule(ROE, F64, 1280, 1280, 256, 3) == 9      # 751 workgroups: 3 a CU
+    assert rule(ROE, F64, 2048, 2048, 256, 3) == 23
+    assert rule(ROE, F32, 2048, 2048, 256, 6) == 14
+    assert rule(HYD, F64, 512, 512, 256, 3) == 8        # the shortest it takes
+    assert rule(HYD, F64, 1024, 1024, 256, 3) == 10
+    assert rule(HYD, F32, 1024, 1024, 256, 5) == 5
+    # brute force of the stated cost, larger height on ties
+    shapes = ((384, 384), (512, 512), (1000, 3000), (4096, 1024), (2048, 2048), (3000, 700),
+              (130, 9000))
+    for solver, dtype in WAVES:
+        for (nx, ny) in shapes:
+            for cus in (256, 304, 80):
+                waves = WAVES[(solver, dtype)]
+                if banded_at_16(solver, dtype, nx, ny, cus, waves):
+                    continue
+                lo = 8 if (solver, dtype) == (HYD, F64) else 4
+                best = min(range(lo, 33),
+                           key=lambda r: (small_level_cost(solver, dtype, nx, ny, r, cus), -r))
+                assert rule(solver, dtype, nx, ny, cus, waves) == best, (solver, dtype, nx, ny, cus)
+
+
 def test_unknown_sizes_fall_back_to_16(rule):
-    assert rule(ROE, F64, 0, 0) == 16
+    assert rule(ROE, F64, 0, 0, 256, 3) == 16
     assert rule(HYD, F64, 2048, 2048, 256, 0) == 16      # occupancy query failed
+    assert rule(ROE, F32, 512, 512, 256, 0) == 16
 
 
 OFFSETS_SRC = r"""
