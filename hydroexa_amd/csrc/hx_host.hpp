@@ -82,8 +82,8 @@ namespace host {
 //     k = ceil(ceil(workgroups / 8) / (CUs / 8)) of them; a strip of r rows
 //     marches r + 2 (plain Roe) or r + 4 (hydrostatic) rows at a per-row cost
 //     row_cost(k) fitted to heights 4-32 on 384^2 to 3072^2 levels of each
-//     march.  The height is the r in [4, 32] ([8, 32] hydrostatic FP64) of
-//     least (r + prologue) x row_cost(k), the larger on ties.  On those
+//     march.  The height is the r in [4, 32] of least (r + prologue) x
+//     row_cost(k), the larger on ties.  On those
 //     levels its choice measured 0.1-0.6 % above the best height on average
 //     (at most 5 %), where the rule it replaces (6 rows below a fill of
 //     3 wavefronts per SIMD, else 16) was 2-11 % above (at most 22 %).
@@ -136,7 +136,7 @@ inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd
         }
         return best;
     }
-    for (int r = hydro64 ? 8 : 4; r <= 32; ++r) {
+    for (int r = 4; r <= 32; ++r) {
         const int64_t wgs = (nsx * ((ny + r - 1) / r) + 3) / 4;
         const int64_t k = ((wgs + 7) / 8 + cus_xcd - 1) / cus_xcd;
         const int64_t c = (r + prologue) * row_cost(solver, dtype, k);

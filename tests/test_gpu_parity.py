@@ -520,8 +520,8 @@ def test_fast_arith_y_invariant_keeps_hv_zero(fast_arith, model):
 @pytest.mark.parametrize("model", ["Roe", "HydrostaticRoe"])
 def test_strip_height_leaves_results_unchanged(gpu, arith, model):
     """The strip height only changes which wavefront marches which rows: the
-    default (for the hydrostatic solver, chosen from the level size) and
-    heights of 1, 8, 13 and 32 rows give bit-identical states and dt after 20
+    default (chosen from the level size) and heights of 1, 8, 13 and 32
+    rows give bit-identical states and dt after 20
     steps of a rough, partly dry level with reflective and transmissive sides."""
     from hydroexa_amd import set_arith, set_tuning
     st = random_state(np.random.default_rng(3), 97, 190, dry_frac=0.1, friction=True)
