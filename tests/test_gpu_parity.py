@@ -675,6 +675,50 @@ def test_resumed_run_equals_one_run(gpu, model, first):
     np.testing.assert_array_equal(np.concatenate([v1, v2]), v_one)
 
 
+@pytest.mark.parametrize("ny", [2048, 2047])
+@pytest.mark.parametrize("model,dtype", [("Roe", torch.float32), ("HydrostaticRoe", torch.float64)])
+def test_fused_loop_on_a_level_of_2_23_cells_equals_stepping(gpu, model, dtype, ny):
+    """A level of 2^23 cells and more folds the fused loop's CFL partials into
+    one sub-slot and reads that one; smaller levels spread them over the 64
+    per-CU sub-slots.  On a 4096 x 2048 level (2^23 cells) and a 4096 x 2047
+    one (just below) the fused loop's
+    dt history and state equal, bit for bit, ComputeDt (hx_level_dt, which
+    folds every sub-slot on the host) + AdvanceLevel step by step, in the
+    exact build (the fast build's fused reduction takes sqrt by a refined
+    reciprocal square root, hx_level_dt's kernel the correctly rounded one,
+    so there the two may differ in the last bit of a)."""
+    from hydroexa_amd import set_arith
+    nx, steps = 4096, 6
+    rng = np.random.default_rng(23)
+    X = (np.arange(nx) + 0.5)[None, :] / nx
+    h = np.where(X < 0.5, 1.0, 0.5) * np.ones((ny, 1)) * rng.uniform(0.95, 1.05, (ny, nx))
+    hu = 0.05 * h * rng.normal(size=(ny, nx))
+    hv = 0.05 * h * rng.normal(size=(ny, nx))
+    z = 0.01 * rng.uniform(size=(ny, nx)) if model == "HydrostaticRoe" else np.zeros((ny, nx))
+    n = np.zeros((ny, nx))
+    args = (h, hu, hv, z, n, 1.0 / nx, 1.0 / ny, (1, 0), (0, 1))
+    fused = stepped = None
+    set_arith("exact")
+    try:
+        print(f"level of 4096 x {ny} cells: {model} {dtype}", flush=True)
+        fused = Level(*args, dtype=dtype)
+        sv = SolverVariant(model)
+        d_fused, v_fused = sv.run(fused, 0.45, steps)
+        stepped = Level(*args, dtype=dtype)
+        d_step = []
+        for _ in range(steps):
+            dt, _ = sv.compute_dt_level(stepped, 0.45)
+            sv.compute_fluxes(stepped, dt)
+            d_step.append(dt)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_fused, np.asarray(d_step))
+        assert torch.equal(fused.U_new, stepped.U_new)
+    finally:
+        set_arith("exact")
+        del fused, stepped
+        torch.cuda.empty_cache()
+
+
 def test_resume_requires_the_previous_rollout(gpu):
     w = W.dam_break(64, 1)
     a = Level(w.h, w.hu, w.hv, w.z, w.n, w.dx, w.dy, w.bc_lo, w.bc_hi)
