@@ -147,7 +147,7 @@ inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd
 // Whether the fused loop's steps fold their CFL partials into every
 // sub-slot of a slot (1) or into sub-slot 0 (0), the next step reading the
 // same.  Spreading takes the queue of same-line atomics away from the
-// workgroups that finish together (levels up to 2048^2: 5-45 % faster per
+// workgroups that finish together (levels up to 2048^2: up to 2x faster per
 // step); on levels that fill the device twice the partials arrive spread
 // over the launch, and reading 64 lines in every wavefront cost more than
 // the queue (4096^2: 221.4 against 219.7 us per step FP64, 124.7 against
