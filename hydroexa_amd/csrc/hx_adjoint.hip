@@ -715,6 +715,14 @@ extern "C" int hx_adjoint_batch_create(hx_ctx* c, int solver, int nscen, const h
         gpre(nscen + 1, 0), upre(nscen + 1, 0);
     std::vector<int64_t> allg, ucell;
     std::vector<int> uptr(1, 0), uidx;
+    // one launch marches every scenario's strips: a batch takes one height
+    // for their sum, a single scenario its own
+    int batch_rows = 0;
+    if (nscen > 1) {
+        std::vector<int> bx(nscen), by(nscen);
+        for (int s = 0; s < nscen; ++s) { bx[s] = geoms[s].nx; by[s] = geoms[s].ny; }
+        batch_rows = host::eff_rows_multi(c, solver, HX_F64, bx.data(), by.data(), nscen);
+    }
     bool ok = true;
     for (int s = 0; s < nscen && ok; ++s) {
         const hx_patch& gm = geoms[s];
@@ -730,7 +738,8 @@ extern "C" int hx_adjoint_batch_create(hx_ctx* c, int solver, int nscen, const h
         StepArgs<double> sa{};
         sa.p = d.p;
         sa.opitch = gm.nx;
-        host::fill_geometry(sa, solver, host::eff_rows(c, solver, HX_F64, gm.nx, gm.ny), gm.dx, gm.dy);
+        host::fill_geometry(sa, solver, batch_rows ? batch_rows : host::eff_rows(c, solver, HX_F64, gm.nx, gm.ny),
+                            gm.dx, gm.dy);
         d.dx = sa.dx; d.dy = sa.dy; d.rdx = sa.rdx; d.rdy = sa.rdy; d.dx_pow2 = sa.dx_pow2;
         d.rows = sa.rows; d.nsx = sa.nsx; d.nstrips = sa.nstrips;
         d.strip0 = strip[s];

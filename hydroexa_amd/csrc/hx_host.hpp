@@ -113,22 +113,23 @@ inline int64_t row_cost(int solver, int dtype, int64_t k)
     if (k <= 8) return g[k < 1 ? 0 : k - 1];
     return g[7] + (k - 8) * (g[7] - g[3]) / 4;
 }
-inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd_waves)
+// The rule of default_rows over the strips one launch deals out:
+// strips(r) is their number at height r (summed over the patches of a
+// launch that covers several).
+template <typename F>
+inline int rows_rule(int solver, int dtype, F strips, int cus, int simd_waves)
 {
-    if (nx <= 0 || ny <= 0 || cus <= 0 || simd_waves <= 0) return 16;
     const bool hydro = (solver == HX_SOLVER_HYDROSTATIC_ROE);
     const bool hydro64 = hydro && dtype == HX_F64;
-    const int cols = hydro ? kHydroCols : kRoeCols;
     const int prologue = hydro ? 4 : 2;
-    const int64_t nsx = (nx + cols - 1) / cols;
     const int64_t cus_xcd = cus >= 8 ? cus / 8 : 1;
     const int64_t slots_xcd = cus_xcd * 4 * simd_waves;
     int best = 32;
     int64_t cost = INT64_MAX;
-    if (nsx * ((ny + 15) / 16) >= 2 * 8 * slots_xcd) {
+    if (strips(16) >= 2 * 8 * slots_xcd) {
         if (!hydro64) return 16;
         for (int r = 8; r <= 32; ++r) {
-            const int64_t wgs = (nsx * ((ny + r - 1) / r) + 3) / 4;
+            const int64_t wgs = (strips(r) + 3) / 4;
             const int64_t k = ((wgs + 7) / 8 + cus_xcd - 1) / cus_xcd;
             const int64_t units = (k / simd_waves) * round_units(simd_waves) + round_units(k % simd_waves);
             const int64_t c = units * (r + 4);
@@ -137,12 +138,34 @@ inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd
         return best;
     }
     for (int r = 4; r <= 32; ++r) {
-        const int64_t wgs = (nsx * ((ny + r - 1) / r) + 3) / 4;
+        const int64_t wgs = (strips(r) + 3) / 4;
         const int64_t k = ((wgs + 7) / 8 + cus_xcd - 1) / cus_xcd;
         const int64_t c = (r + prologue) * row_cost(solver, dtype, k);
         if (c <= cost) { cost = c; best = r; }
     }
     return best;
+}
+inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd_waves)
+{
+    if (nx <= 0 || ny <= 0 || cus <= 0 || simd_waves <= 0) return 16;
+    const int cols = (solver == HX_SOLVER_HYDROSTATIC_ROE) ? kHydroCols : kRoeCols;
+    const int64_t nsx = (nx + cols - 1) / cols;
+    return rows_rule(solver, dtype, [&](int r) { return nsx * ((ny + r - 1) / r); }, cus, simd_waves);
+}
+// One height for a launch over n patches (a batch of scenarios): their
+// strips are dealt together, so the rule applies to their sum.
+inline int default_rows_multi(int solver, int dtype, const int* nx, const int* ny, int n, int cus,
+                              int simd_waves)
+{
+    if (n <= 0 || cus <= 0 || simd_waves <= 0) return 16;
+    const int cols = (solver == HX_SOLVER_HYDROSTATIC_ROE) ? kHydroCols : kRoeCols;
+    for (int q = 0; q < n; ++q)
+        if (nx[q] <= 0 || ny[q] <= 0) return 16;
+    return rows_rule(solver, dtype, [&](int r) {
+        int64_t t = 0;
+        for (int q = 0; q < n; ++q) t += (int64_t)((nx[q] + cols - 1) / cols) * ((ny[q] + r - 1) / r);
+        return t;
+    }, cus, simd_waves);
 }
 // Whether the fused loop's steps fold their CFL partials into every
 // sub-slot of a slot (1) or into sub-slot 0 (0), the next step reading the
@@ -170,6 +193,12 @@ inline int eff_rows(const hx_ctx* c, int solver, int dtype, int nx, int ny)
     if (r) return r;
     if (nx <= 0 || ny <= 0) return 16;
     return default_rows(solver, dtype, nx, ny, c->cus, advance_simd_waves(c, solver, dtype));
+}
+inline int eff_rows_multi(const hx_ctx* c, int solver, int dtype, const int* nx, const int* ny, int n)
+{
+    const int r = c->rows[solver][dtype];
+    if (r) return r;
+    return default_rows_multi(solver, dtype, nx, ny, n, c->cus, advance_simd_waves(c, solver, dtype));
 }
 
 }  // namespace host

@@ -134,6 +134,93 @@ def test_unknown_sizes_fall_back_to_16(rule):
     assert rule(ROE, F32, 512, 512, 256, 0) == 16
 
 
+MULTI_SRC = r"""
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#include "hx_host.hpp"
+// argv: solver dtype cus waves n nx0 ny0 nx1 ny1 ...
+int main(int argc, char** argv)
+{
+    (void)argc;
+    const int solver = atoi(argv[1]), dtype = atoi(argv[2]), cus = atoi(argv[3]);
+    const int waves = atoi(argv[4]), n = atoi(argv[5]);
+    std::vector<int> nx(n), ny(n);
+    for (int q = 0; q < n; ++q) { nx[q] = atoi(argv[6 + 2 * q]); ny[q] = atoi(argv[7 + 2 * q]); }
+    printf("%d\n", hx::host::default_rows_multi(solver, dtype, nx.data(), ny.data(), n, cus, waves));
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def multi_rule(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not found")
+    d = tmp_path_factory.mktemp("launch_shape_multi")
+    src, exe = d / "multi.cpp", d / "multi"
+    src.write_text(MULTI_SRC)
+    subprocess.run([HIPCC, "-std=c++17", f"-I{ROOT}/include", f"-I{ROOT}/hydroexa_amd/csrc",
+                    str(src), "-o", str(exe)], check=True, capture_output=True)
+
+    def run(solver, dtype, sizes, cus=256, waves=3):
+        flat = [str(v) for wh in sizes for v in wh]
+        out = subprocess.run([str(exe), str(solver), str(dtype), str(cus), str(waves),
+                              str(len(sizes)), *flat], check=True, capture_output=True,
+                             text=True).stdout
+        return int(out)
+    return run
+
+
+def batch_rule(solver, dtype, sizes, cus, waves):
+    """The stated rule over a batch: strips summed over the patches."""
+    cols = 60 if solver == HYD else 62
+
+    def strips(r):
+        return sum(-(-nx // cols) * -(-ny // r) for nx, ny in sizes)
+
+    def k_of(r):
+        return -(-(-(-(-(-strips(r) // 4)) // 8)) // max(1, cus // 8))
+    if strips(16) >= 2 * 8 * max(1, cus // 8) * 4 * waves:
+        if (solver, dtype) != (HYD, F64):
+            return 16
+        unit = {0: 0, 1: 78, 2: 110, 3: 160}
+
+        def rounds(r):
+            k = k_of(r)
+            parts = [waves] * (k // waves) + [k % waves]
+            return sum(unit[m] if m <= 3 else 160 * m // 3 for m in parts) * (r + 4)
+        return min(range(8, 33), key=lambda r: (rounds(r), -r))
+    g = ROW_COST[(solver, dtype)]
+    prologue = 4 if solver == HYD else 2
+
+    def cost(r):
+        k = k_of(r)
+        return (r + prologue) * (g[k - 1] if k <= 8 else g[7] + (k - 8) * (g[7] - g[3]) // 4)
+    return min(range(4, 33), key=lambda r: (cost(r), -r))
+
+
+def test_one_patch_batch_takes_the_level_height(rule, multi_rule):
+    for solver, dtype in WAVES:
+        for (nx, ny) in ((384, 384), (1280, 1280), (2048, 2048), (4096, 4096), (3000, 700)):
+            w = WAVES[(solver, dtype)]
+            assert multi_rule(solver, dtype, [(nx, ny)], 256, w) == rule(solver, dtype, nx, ny, 256, w)
+
+
+def test_a_batch_takes_one_height_for_its_summed_strips(multi_rule):
+    import random
+    rnd = random.Random(128)
+    batch = [(rnd.randint(128, 512), rnd.randint(128, 512)) for _ in range(128)]
+    small = batch[:6]
+    for solver, dtype in WAVES:
+        w = WAVES[(solver, dtype)]
+        for sizes in (batch, small):
+            assert multi_rule(solver, dtype, sizes, 256, w) == batch_rule(solver, dtype, sizes, 256, w)
+    # 128 scenarios of 128^2-512^2 fill the device twice at 16 rows: the
+    # hydrostatic FP64 batch takes the banded rule, not a small scenario's 4-5 rows
+    assert multi_rule(HYD, F64, batch, 256, 3) >= 8
+
+
 OFFSETS_SRC = r"""
 #include <cstdio>
 #include <cstdlib>
