@@ -1349,6 +1349,24 @@ static void ens_free(hx_ensemble* e)
     delete e;
 }
 
+// Default strip height of an ensemble launch: the context's tuning, else one
+// height for the strips of all its scenarios together (default_rows_multi),
+// no taller than any scenario's offsets allow.
+static int ens_default_rows(const hx_ensemble* e)
+{
+    std::vector<int> nx(e->nscen), ny(e->nscen);
+    int64_t cap = INT64_MAX;
+    for (int s = 0; s < e->nscen; ++s) {
+        const hx_patch& p = e->patches[s];
+        nx[s] = p.nx;
+        ny[s] = p.ny;
+        cap = std::min<int64_t>(cap, e->dtype == HX_F64 ? max_strip_rows<double>(p.pitch, p.pitch)
+                                                        : max_strip_rows<float>(p.pitch, p.pitch));
+    }
+    const int r = eff_rows_multi(e->ctx, e->solver, e->dtype, nx.data(), ny.data(), e->nscen);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(r, cap));
+}
+
 extern "C" int hx_ensemble_create(hx_ctx* c, int solver, int dtype, int nscen, const hx_patch* scen,
                                   void* const* alt, hx_ensemble** out)
 {
@@ -1373,10 +1391,10 @@ extern "C" int hx_ensemble_create(hx_ctx* c, int solver, int dtype, int nscen, c
     e->solver = solver;
     e->dtype = dtype;
     e->nscen = nscen;
-    e->rows = eff_rows(c, solver, dtype, 0, 0);
     e->occ = eff_occ(c, solver, dtype);
     e->persistent = 0;
     e->patches.assign(scen, scen + nscen);
+    e->rows = ens_default_rows(e);
     e->alt.assign(alt, alt + 3 * nscen);
     const size_t dsz = (dtype == HX_F64) ? sizeof(ScenDesc<double>) : sizeof(ScenDesc<float>);
     e->cap = 1024;
@@ -1416,7 +1434,7 @@ extern "C" int hx_ensemble_set_tuning(hx_ensemble* e, int rows, int occupancy, i
     if (!known || rows < 0 || rows > 1 << 20 || (persistent != 0 && persistent != 1))
         return HX_E_BADARG;
     HX_TRY(hipSetDevice(e->ctx->device));
-    e->rows = rows ? rows : 16;
+    e->rows = rows ? rows : ens_default_rows(e);
     e->occ = occupancy ? occupancy : default_occ(e->solver, e->dtype);
     e->persistent = persistent;
     return ens_upload_any(e);
