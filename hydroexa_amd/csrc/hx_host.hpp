@@ -73,10 +73,11 @@ namespace host {
 //     marches them, plan_bands): plain Roe and FP32 stream HBM at 16 rows
 //     (14-20 rows within 1-3 % on the 4096^2 dam break, 24-128 rows 5-30 %
 //     slower; 2560^2 and 3072^2 FP64 best at 14-16); the FP64 hydrostatic
-//     march is arithmetic-bound and ends on a tail of idle SIMDs, so its
-//     height is the r in [8, 32] of least (sum over rounds of resident
-//     wavefronts of the round's per-row cost, round_units) x (r + 4): 31 rows
-//     at 4096^2 (5 % faster than 16);
+//     march, and the exact build's FP64 plain-Roe march, are arithmetic-bound
+//     and end on a tail of idle SIMDs, so their height is the r in [8, 32] of
+//     least (sum over rounds of resident wavefronts of the round's per-row
+//     cost, round_units) x (r + prologue rows): 31 rows at 4096^2 hydrostatic
+//     (5 % faster than 16), 30 plain Roe exact (4.5 % faster);
 //   - smaller levels: a launch lasts as long as its busiest CU.  Workgroups
 //     are dealt round-robin to the 8 XCDs, so that CU holds
 //     k = ceil(ceil(workgroups / 8) / (CUs / 8)) of them; a strip of r rows
@@ -115,24 +116,27 @@ inline int64_t row_cost(int solver, int dtype, int64_t k)
 }
 // The rule of default_rows over the strips one launch deals out:
 // strips(r) is their number at height r (summed over the patches of a
-// launch that covers several).
+// launch that covers several).  exact: the exact arithmetic build, whose FP64
+// plain-Roe march is arithmetic-bound too (VALU 92 % busy at 4096^2): on
+// banded levels it takes the hydrostatic FP64 rule (30 rows at 4096^2:
+// 293.7-295.4 us per step at 32 rows, 307-309 at 16).
 template <typename F>
-inline int rows_rule(int solver, int dtype, F strips, int cus, int simd_waves)
+inline int rows_rule(int solver, int dtype, F strips, int cus, int simd_waves, bool exact = false)
 {
     const bool hydro = (solver == HX_SOLVER_HYDROSTATIC_ROE);
-    const bool hydro64 = hydro && dtype == HX_F64;
+    const bool arith_bound = dtype == HX_F64 && (hydro || exact);
     const int prologue = hydro ? 4 : 2;
     const int64_t cus_xcd = cus >= 8 ? cus / 8 : 1;
     const int64_t slots_xcd = cus_xcd * 4 * simd_waves;
     int best = 32;
     int64_t cost = INT64_MAX;
     if (strips(16) >= 2 * 8 * slots_xcd) {
-        if (!hydro64) return 16;
+        if (!arith_bound) return 16;
         for (int r = 8; r <= 32; ++r) {
             const int64_t wgs = (strips(r) + 3) / 4;
             const int64_t k = ((wgs + 7) / 8 + cus_xcd - 1) / cus_xcd;
             const int64_t units = (k / simd_waves) * round_units(simd_waves) + round_units(k % simd_waves);
-            const int64_t c = units * (r + 4);
+            const int64_t c = units * (r + prologue);
             if (c <= cost) { cost = c; best = r; }
         }
         return best;
@@ -145,17 +149,19 @@ inline int rows_rule(int solver, int dtype, F strips, int cus, int simd_waves)
     }
     return best;
 }
-inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd_waves)
+inline int default_rows(int solver, int dtype, int nx, int ny, int cus, int simd_waves,
+                        bool exact = false)
 {
     if (nx <= 0 || ny <= 0 || cus <= 0 || simd_waves <= 0) return 16;
     const int cols = (solver == HX_SOLVER_HYDROSTATIC_ROE) ? kHydroCols : kRoeCols;
     const int64_t nsx = (nx + cols - 1) / cols;
-    return rows_rule(solver, dtype, [&](int r) { return nsx * ((ny + r - 1) / r); }, cus, simd_waves);
+    return rows_rule(solver, dtype, [&](int r) { return nsx * ((ny + r - 1) / r); }, cus, simd_waves,
+                     exact);
 }
 // One height for a launch over n patches (a batch of scenarios): their
 // strips are dealt together, so the rule applies to their sum.
 inline int default_rows_multi(int solver, int dtype, const int* nx, const int* ny, int n, int cus,
-                              int simd_waves)
+                              int simd_waves, bool exact = false)
 {
     if (n <= 0 || cus <= 0 || simd_waves <= 0) return 16;
     const int cols = (solver == HX_SOLVER_HYDROSTATIC_ROE) ? kHydroCols : kRoeCols;
@@ -165,7 +171,7 @@ inline int default_rows_multi(int solver, int dtype, const int* nx, const int* n
         int64_t t = 0;
         for (int q = 0; q < n; ++q) t += (int64_t)((nx[q] + cols - 1) / cols) * ((ny[q] + r - 1) / r);
         return t;
-    }, cus, simd_waves);
+    }, cus, simd_waves, exact);
 }
 // Whether the fused loop's steps fold their CFL partials into every
 // sub-slot of a slot (1) or into sub-slot 0 (0), the next step reading the
@@ -192,13 +198,15 @@ inline int eff_rows(const hx_ctx* c, int solver, int dtype, int nx, int ny)
     const int r = c->rows[solver][dtype];
     if (r) return r;
     if (nx <= 0 || ny <= 0) return 16;
-    return default_rows(solver, dtype, nx, ny, c->cus, advance_simd_waves(c, solver, dtype));
+    return default_rows(solver, dtype, nx, ny, c->cus, advance_simd_waves(c, solver, dtype),
+                        c->arith == HX_ARITH_EXACT);
 }
 inline int eff_rows_multi(const hx_ctx* c, int solver, int dtype, const int* nx, const int* ny, int n)
 {
     const int r = c->rows[solver][dtype];
     if (r) return r;
-    return default_rows_multi(solver, dtype, nx, ny, n, c->cus, advance_simd_waves(c, solver, dtype));
+    return default_rows_multi(solver, dtype, nx, ny, n, c->cus, advance_simd_waves(c, solver, dtype),
+                              c->arith == HX_ARITH_EXACT);
 }
 
 }  // namespace host

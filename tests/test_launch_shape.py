@@ -19,7 +19,8 @@ int main(int argc, char** argv)
     (void)argc;
     const int solver = atoi(argv[1]), dtype = atoi(argv[2]), nx = atoi(argv[3]);
     const int ny = atoi(argv[4]), cus = atoi(argv[5]), waves = atoi(argv[6]);
-    printf("%d\n", hx::host::default_rows(solver, dtype, nx, ny, cus, waves));
+    const bool exact = argc > 7 && atoi(argv[7]) != 0;
+    printf("%d\n", hx::host::default_rows(solver, dtype, nx, ny, cus, waves, exact));
     return 0;
 }
 """
@@ -38,8 +39,8 @@ def rule(tmp_path_factory):
     subprocess.run([HIPCC, "-std=c++17", f"-I{ROOT}/include", f"-I{ROOT}/hydroexa_amd/csrc",
                     str(src), "-o", str(exe)], check=True, capture_output=True)
 
-    def run(solver, dtype, nx, ny, cus=256, waves=0):
-        out = subprocess.run([str(exe), *map(str, (solver, dtype, nx, ny, cus, waves))],
+    def run(solver, dtype, nx, ny, cus=256, waves=0, exact=False):
+        out = subprocess.run([str(exe), *map(str, (solver, dtype, nx, ny, cus, waves, int(exact)))],
                              check=True, capture_output=True, text=True).stdout
         return int(out)
     return run
@@ -126,6 +127,27 @@ def test_smaller_levels_take_the_least_modelled_busiest_cu_time(rule):
                 best = min(range(4, 33),
                            key=lambda r: (small_level_cost(solver, dtype, nx, ny, r, cus), -r))
                 assert rule(solver, dtype, nx, ny, cus, waves) == best, (solver, dtype, nx, ny, cus)
+
+
+def test_exact_fp64_plain_roe_on_banded_levels_takes_the_arithmetic_bound_rule(rule):
+    # the exact build's FP64 march is arithmetic-bound: 30 rows at 4096^2
+    # (measured 293.7-295.4 us at 32 rows against 307-309 at 16)
+    assert rule(ROE, F64, 4096, 4096, 256, 3, exact=True) == 30
+    assert rule(ROE, F64, 4096, 4096, 256, 3, exact=False) == 16
+    for (nx, ny) in ((4096, 4096), (3072, 3072), (8192, 2048)):
+        best = min(range(8, 33), key=lambda r: (rounds_cost_cols(nx, ny, r, 256, 3, 62, 2), -r))
+        assert rule(ROE, F64, nx, ny, 256, 3, exact=True) == best
+    # FP32 and levels below the banded tiling keep their rules
+    assert rule(ROE, F32, 4096, 4096, 256, 6, exact=True) == 16
+    for n in (384, 1024, 2048):
+        assert rule(ROE, F64, n, n, 256, 3, exact=True) == rule(ROE, F64, n, n, 256, 3)
+
+
+def rounds_cost_cols(nx, ny, r, cus, waves, cols, prologue):
+    k = busiest_cu(nx, ny, r, cols, cus)
+    unit = {0: 0, 1: 78, 2: 110, 3: 160}
+    rounds = [waves] * (k // waves) + [k % waves]
+    return sum(unit[m] if m <= 3 else 160 * m // 3 for m in rounds) * (r + prologue)
 
 
 def test_unknown_sizes_fall_back_to_16(rule):
